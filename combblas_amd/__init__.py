@@ -1,0 +1,290 @@
+"""Host-side mirror of CombBLAS's node-local kernel interface over libcbx.so.
+
+The product is the C ABI in ``include/cbx.h`` (built into
+``combblas_amd/libcbx.so`` from ``combblas_amd/csrc``).  This module binds it
+with ctypes so Python tests, the benchmark and users can call it with numpy
+arrays:
+
+* :class:`Dcsc` -- a host DCSC matrix (``cp, jc, ir, val`` as numpy arrays),
+  the layout of ``combblas::Dcsc<int64_t, NT>`` (include/CombBLAS/dcsc.h:125-131).
+* :class:`DeviceDcsc` -- the same arrays resident in device memory.
+* :func:`LocalHybridSpGEMM` / :func:`LocalSpGEMMHash` -- the reference's
+  local products (include/CombBLAS/mtSpGEMM.h:213-217, :463-467) on the GPU.
+* :func:`spgemm` -- the device-resident form used by the benchmark.
+
+There is no CPU fallback: every compute call goes to the HIP library and
+raises :class:`CbxError` when it is missing or when no GPU is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libcbx.so"
+
+# ---------------------------------------------------------------- constants
+OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_LIMIT = range(6)
+
+SEMIRINGS = {"plus_times": 0, "min_plus": 1, "select_max": 2}
+DTYPES = {"int64": 0, "float64": 1, "float32": 2, "bool": 3}
+NP_DTYPES = {"int64": np.int64, "float64": np.float64, "float32": np.float32, "bool": np.uint8}
+MODES = {"two_phase": 0, "unsorted": 1, "learned": 2}
+HOST, DEVICE = 0, 1
+GEN_SYMMETRIZE, GEN_NO_SELF_LOOPS, GEN_PATTERN = 1, 2, 4
+
+
+class CbxError(RuntimeError):
+    """Raised when a cbx call returns a non-zero status."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"cbx status {status}: {msg}")
+        self.status = status
+
+
+class cbx_dcsc(C.Structure):
+    _fields_ = [
+        ("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64), ("nzc", C.c_int64),
+        ("cp", C.c_void_p), ("jc", C.c_void_p), ("ir", C.c_void_p), ("val", C.c_void_p),
+        ("dtype", C.c_int32), ("location", C.c_int32),
+    ]
+
+
+class cbx_spgemm_stats(C.Structure):
+    _fields_ = [
+        ("flops", C.c_int64), ("nnz", C.c_int64), ("nzc", C.c_int64),
+        ("overflow_cols", C.c_int64), ("predicted_nnz", C.c_int64),
+        ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+
+_lib_handle = None
+
+
+def lib() -> C.CDLL:
+    """Load libcbx.so (built in-tree); raises if it is missing."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not LIB_PATH.exists():
+            raise CbxError(ERR_UNSUPPORTED, f"{LIB_PATH} is not built; run `python -m combblas_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        L.cbx_last_error.restype = C.c_char_p
+        L.cbx_device_count.restype = C.c_int32
+        L.cbx_device_synchronize.restype = C.c_int32
+        L.cbx_dcsc_free.argtypes = [P(cbx_dcsc)]
+        L.cbx_dcsc_free.restype = None
+        L.cbx_dcsc_to_device.argtypes = [P(cbx_dcsc), P(cbx_dcsc)]
+        L.cbx_dcsc_to_host.argtypes = [P(cbx_dcsc), P(cbx_dcsc)]
+        L.cbx_gen_rmat.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                   C.c_uint32, P(cbx_dcsc)]
+        L.cbx_spgemm.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, P(cbx_dcsc),
+                                 P(cbx_spgemm_stats), C.c_void_p]
+        for name in ("cbx_dcsc_to_device", "cbx_dcsc_to_host", "cbx_gen_rmat", "cbx_spgemm"):
+            getattr(L, name).restype = C.c_int32
+        _lib_handle = L
+    return _lib_handle
+
+
+def _check(status: int) -> None:
+    if status != OK:
+        raise CbxError(status, lib().cbx_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    return int(lib().cbx_device_count())
+
+
+def synchronize() -> None:
+    _check(lib().cbx_device_synchronize())
+
+
+# ------------------------------------------------------------------ matrices
+@dataclass
+class Dcsc:
+    """Host DCSC matrix: the arrays of combblas::Dcsc (dcsc.h:125-131)."""
+
+    m: int
+    n: int
+    cp: np.ndarray  # int64[nzc+1]
+    jc: np.ndarray  # int64[nzc]
+    ir: np.ndarray  # int64[nnz]
+    val: np.ndarray  # dtype[nnz]
+    dtype: str = "int64"
+
+    @property
+    def nnz(self) -> int:
+        return int(self.ir.shape[0])
+
+    @property
+    def nzc(self) -> int:
+        return int(self.jc.shape[0])
+
+    def astype(self, dtype: str) -> "Dcsc":
+        v = self.val
+        if dtype == "bool":
+            v = (v != 0).astype(np.uint8)
+        else:
+            v = v.astype(NP_DTYPES[dtype])
+        return Dcsc(self.m, self.n, self.cp, self.jc, self.ir, v, dtype)
+
+    def _cstruct(self) -> tuple[cbx_dcsc, list]:
+        arrs = [np.ascontiguousarray(self.cp, np.int64), np.ascontiguousarray(self.jc, np.int64),
+                np.ascontiguousarray(self.ir, np.int64),
+                np.ascontiguousarray(self.val, NP_DTYPES[self.dtype])]
+        s = cbx_dcsc(self.m, self.n, self.nnz, self.nzc, *(a.ctypes.data for a in arrs),
+                     DTYPES[self.dtype], HOST)
+        return s, arrs
+
+    def to_device(self) -> "DeviceDcsc":
+        s, keep = self._cstruct()
+        out = cbx_dcsc()
+        _check(lib().cbx_dcsc_to_device(C.byref(s), C.byref(out)))
+        del keep
+        return DeviceDcsc(out)
+
+    @staticmethod
+    def from_struct(s: cbx_dcsc, free_fn) -> "Dcsc":
+        """Copy a host cbx_dcsc into numpy arrays, then release it with free_fn."""
+        dt = {v: k for k, v in DTYPES.items()}[s.dtype]
+        nzc, nnz = s.nzc, s.nnz
+
+        def arr(ptr, n, t):
+            if n == 0 or not ptr:
+                return np.zeros(0, t)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(t))), (n,)).copy()
+
+        cp = arr(s.cp, nzc + 1, np.int64) if s.cp else np.zeros(1, np.int64)
+        out = Dcsc(int(s.m), int(s.n), cp, arr(s.jc, nzc, np.int64), arr(s.ir, nnz, np.int64),
+                   arr(s.val, nnz, NP_DTYPES[dt]), dt)
+        free_fn(C.byref(s))
+        return out
+
+    # ---- conversions used by tests
+    def col_ids(self) -> np.ndarray:
+        """Column index of every stored entry."""
+        return np.repeat(self.jc, np.diff(self.cp))
+
+    def to_dense(self) -> np.ndarray:
+        d = np.zeros((self.m, self.n), dtype=NP_DTYPES[self.dtype])
+        d[self.ir, self.col_ids()] = self.val
+        return d
+
+    @staticmethod
+    def from_coo(m: int, n: int, rows, cols, vals, dtype: str = "int64") -> "Dcsc":
+        """Build from unique (row, col) entries (sorted here by column, row)."""
+        rows = np.asarray(rows, np.int64)
+        cols = np.asarray(cols, np.int64)
+        vals = np.asarray(vals)
+        order = np.lexsort((rows, cols))
+        rows, cols, vals = rows[order], cols[order], vals[order]
+        if rows.size:
+            key = cols * max(m, 1) + rows
+            if np.any(np.diff(key) == 0):
+                raise ValueError("duplicate entries")
+        jc, starts = np.unique(cols, return_index=True)
+        cp = np.append(starts, rows.size).astype(np.int64)
+        if dtype == "bool":
+            v = (vals != 0).astype(np.uint8)
+        else:
+            v = vals.astype(NP_DTYPES[dtype])
+        return Dcsc(m, n, cp, jc.astype(np.int64), rows, v, dtype)
+
+    @staticmethod
+    def from_dense(d: np.ndarray, dtype: str = "int64", mask: np.ndarray | None = None) -> "Dcsc":
+        """Entries where mask (default d != 0) holds; explicit zeros allowed via mask."""
+        if mask is None:
+            mask = d != 0
+        r, c = np.nonzero(mask)
+        return Dcsc.from_coo(d.shape[0], d.shape[1], r, c, d[r, c], dtype)
+
+
+class DeviceDcsc:
+    """A DCSC whose arrays live in device memory (owned, freed on release)."""
+
+    def __init__(self, s: cbx_dcsc):
+        self.s = s
+
+    @property
+    def m(self) -> int:
+        return int(self.s.m)
+
+    @property
+    def n(self) -> int:
+        return int(self.s.n)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.s.nnz)
+
+    @property
+    def nzc(self) -> int:
+        return int(self.s.nzc)
+
+    def to_host(self) -> Dcsc:
+        out = cbx_dcsc()
+        _check(lib().cbx_dcsc_to_host(C.byref(self.s), C.byref(out)))
+        return Dcsc.from_struct(out, lib().cbx_dcsc_free)
+
+    def release(self) -> None:
+        if self.s is not None and _lib_handle is not None:
+            _lib_handle.cbx_dcsc_free(C.byref(self.s))
+        self.s = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- generators
+def rmat_seed(scale: int) -> int:
+    """Seed of the R-MAT product workloads: 0xCB1A5 xor scale."""
+    return 0xCB1A5 ^ scale
+
+
+def gen_rmat(scale: int, edgefactor: int, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int | None = None, symmetrize: bool = False, no_self_loops: bool = False,
+             pattern: bool = False) -> Dcsc:
+    """R-MAT matrix (values uniform in [1,9], duplicates summed) as host DCSC int64."""
+    flags = (GEN_SYMMETRIZE if symmetrize else 0) | (GEN_NO_SELF_LOOPS if no_self_loops else 0) | \
+        (GEN_PATTERN if pattern else 0)
+    s = cbx_dcsc()
+    _check(lib().cbx_gen_rmat(scale, edgefactor, a, b, c, rmat_seed(scale) if seed is None else seed, flags,
+                              C.byref(s)))
+    return Dcsc.from_struct(s, lib().cbx_dcsc_free)
+
+
+# -------------------------------------------------------------------- SpGEMM
+def spgemm(A: DeviceDcsc, B: DeviceDcsc, semiring: str = "plus_times", mode: str = "two_phase",
+           stream: int | None = None) -> tuple[DeviceDcsc, dict]:
+    """C = A*B on device-resident DCSC; returns (C on device, stats)."""
+    out = cbx_dcsc()
+    st = cbx_spgemm_stats()
+    _check(lib().cbx_spgemm(C.byref(A.s), C.byref(B.s), SEMIRINGS[semiring], MODES[mode], C.byref(out),
+                            C.byref(st), stream))
+    stats = {k: getattr(st, k) for k, _ in cbx_spgemm_stats._fields_}
+    return DeviceDcsc(out), stats
+
+
+def LocalHybridSpGEMM(A: Dcsc, B: Dcsc, semiring: str = "plus_times") -> Dcsc:
+    """GPU counterpart of combblas::LocalHybridSpGEMM<SR,NT>(A, B, false, false)
+    (mtSpGEMM.h:213-460): the product with rows sorted within each column,
+    returned in DCSC like SpDCCols(SpTuples) (SpDCCols.cpp:109-184)."""
+    dA, dB = A.to_device(), B.to_device()
+    dC, _ = spgemm(dA, dB, semiring, "two_phase")
+    return dC.to_host()
+
+
+def LocalSpGEMMHash(A: Dcsc, B: Dcsc, semiring: str = "plus_times", sort: bool = True) -> Dcsc:
+    """GPU counterpart of combblas::LocalSpGEMMHash (mtSpGEMM.h:463-656);
+    with sort=False the rows of a column come in no particular order."""
+    dA, dB = A.to_device(), B.to_device()
+    dC, _ = spgemm(dA, dB, semiring, "two_phase" if sort else "unsorted")
+    return dC.to_host()

@@ -1,0 +1,78 @@
+// Device helpers shared by the cbx kernels (wave64 scans, block scans,
+// error checks).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "cbx_internal.h"
+
+#define CBX_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return ::cbx::set_error(CBX_ERR_HIP, "%s failed: %s (%s:%d)", #call,            \
+                              hipGetErrorString(e_), __FILE__, __LINE__);             \
+  } while (0)
+
+namespace cbx {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    int t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+  return v;
+}
+
+// Exclusive prefix sum over a block of NT threads; `wsum` is LDS scratch of
+// NT/64 ints.  Returns this thread's exclusive prefix and the block total.
+// Contains barriers: every thread of the block must call it.
+template <int NT>
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
+  constexpr int NW = NT / kWave;
+  const int lane = lane_id(), w = wave_id();
+  const int inc = wave_incl_scan(v);
+  if (lane == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    int x = lane < NW ? wsum[lane] : 0;
+    x = wave_incl_scan(x);
+    if (lane < NW) wsum[lane] = x;
+  }
+  __syncthreads();
+  const int base = w ? wsum[w - 1] : 0;
+  total = wsum[NW - 1];
+  __syncthreads();
+  return base + inc - v;
+}
+
+// Sum over a block of NT threads (result valid in every thread).
+template <int NT>
+__device__ __forceinline__ long long block_sum_ll(long long v, long long* wsum) {
+  constexpr int NW = NT / kWave;
+  v = wave_sum_ll(v);
+  if (lane_id() == 0) wsum[wave_id()] = v;
+  __syncthreads();
+  long long t = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) t += wsum[i];
+  __syncthreads();
+  return t;
+}
+
+}  // namespace cbx

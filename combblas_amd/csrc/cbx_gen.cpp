@@ -1,0 +1,137 @@
+// Deterministic synthetic-matrix generators (host side, OpenMP).
+//
+// R-MAT follows the quadrant recursion of the Graph500 / CombBLAS generator
+// (graph500-1.2/generator, used through DistEdgeList.cpp:223 GenGraph500Data)
+// but without noise or vertex scrambling: every edge draws `scale` quadrants
+// with probabilities (a, b, c, 1-a-b-c).  Randomness is counter based
+// (splitmix64 of seed and edge id), so the matrix does not depend on the
+// number of threads.  Output is DCSC with int64 indices, rows sorted and
+// unique within each column, duplicates combined as the flags ask.
+#include "cbx_internal.h"
+
+#include <parallel/algorithm>
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace cbx {
+
+static inline uint64_t splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+struct EdgeRng {
+  uint64_t s;
+  EdgeRng(uint64_t seed, uint64_t edge) : s(splitmix(seed ^ splitmix(edge * 0xD1B54A32D192ED03ULL))) {}
+  inline uint64_t next() { s += 0x9E3779B97F4A7C15ULL; return splitmix(s); }
+  inline double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+struct Triple { uint64_t key; int64_t val; };  // key = col << 32 | row
+
+// Sort triples by (col,row) and fold duplicates, then emit host DCSC.
+static cbx_status triples_to_dcsc(std::vector<Triple>& t, int64_t m, int64_t n, bool pattern,
+                                  cbx_dcsc* out) {
+  __gnu_parallel::sort(t.begin(), t.end(),
+                       [](const Triple& x, const Triple& y) { return x.key < y.key; });
+  // fold duplicates in place
+  size_t w = 0;
+  for (size_t i = 0; i < t.size(); ++i) {
+    if (w > 0 && t[w - 1].key == t[i].key) {
+      if (!pattern) t[w - 1].val += t[i].val;
+    } else {
+      t[w++] = t[i];
+    }
+  }
+  t.resize(w);
+  int64_t nnz = (int64_t)w;
+  int64_t nzc = 0;
+  for (size_t i = 0; i < w; ++i)
+    if (i == 0 || (t[i].key >> 32) != (t[i - 1].key >> 32)) ++nzc;
+  std::memset(out, 0, sizeof(*out));
+  out->m = m; out->n = n; out->nnz = nnz; out->nzc = nzc;
+  out->dtype = CBX_I64; out->location = CBX_HOST;
+  out->cp = (int64_t*)std::malloc(sizeof(int64_t) * (nzc + 1));
+  out->jc = (int64_t*)std::malloc(sizeof(int64_t) * (nzc > 0 ? nzc : 1));
+  out->ir = (int64_t*)std::malloc(sizeof(int64_t) * (nnz > 0 ? nnz : 1));
+  out->val = std::malloc(sizeof(int64_t) * (nnz > 0 ? nnz : 1));
+  if (!out->cp || !out->jc || !out->ir || !out->val) {
+    cbx_dcsc_free(out);
+    return set_error(CBX_ERR_NOMEM, "generator: host allocation failed");
+  }
+  int64_t* val = (int64_t*)out->val;
+  int64_t c = 0;
+  for (int64_t i = 0; i < nnz; ++i) {
+    int64_t col = (int64_t)(t[i].key >> 32);
+    if (i == 0 || col != (int64_t)(t[i - 1].key >> 32)) {
+      out->jc[c] = col;
+      out->cp[c] = i;
+      ++c;
+    }
+    out->ir[i] = (int64_t)(t[i].key & 0xffffffffULL);
+    val[i] = pattern ? 1 : t[i].val;
+  }
+  out->cp[nzc] = nnz;
+  return CBX_OK;
+}
+
+}  // namespace cbx
+
+using namespace cbx;
+
+extern "C" cbx_status cbx_gen_rmat(int32_t scale, int32_t edgefactor, double a, double b, double c,
+                                   uint64_t seed, uint32_t flags, cbx_dcsc* out) {
+  if (!out) return set_error(CBX_ERR_INVALID, "cbx_gen_rmat: out is null");
+  if (scale < 1 || scale > 30 || edgefactor < 1)
+    return set_error(CBX_ERR_INVALID, "cbx_gen_rmat: scale must be in [1,30], edgefactor >= 1");
+  if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0))
+    return set_error(CBX_ERR_INVALID, "cbx_gen_rmat: quadrant probabilities out of range");
+  const int64_t n = (int64_t)1 << scale;
+  const int64_t nedges = n * (int64_t)edgefactor;
+  const bool sym = flags & CBX_GEN_SYMMETRIZE;
+  const bool noloops = flags & CBX_GEN_NO_SELF_LOOPS;
+  const bool pattern = flags & CBX_GEN_PATTERN;
+  const double ab = a + b, abc = a + b + c;
+  std::vector<Triple> t;
+  try {
+    t.resize(sym ? 2 * nedges : nedges);
+  } catch (...) {
+    return set_error(CBX_ERR_NOMEM, "cbx_gen_rmat: host allocation failed");
+  }
+  std::vector<uint8_t> keep(nedges, 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < nedges; ++e) {
+    EdgeRng rng(seed, (uint64_t)e);
+    uint64_t row = 0, col = 0;
+    for (int l = 0; l < scale; ++l) {
+      double u = rng.uniform();
+      uint64_t rb, cb;
+      if (u < a) { rb = 0; cb = 0; }
+      else if (u < ab) { rb = 0; cb = 1; }
+      else if (u < abc) { rb = 1; cb = 0; }
+      else { rb = 1; cb = 1; }
+      row = (row << 1) | rb;
+      col = (col << 1) | cb;
+    }
+    int64_t v = 1 + (int64_t)(((rng.next() >> 32) * 9ULL) >> 32);  // uniform in [1,9]
+    if (noloops && row == col) keep[e] = 0;
+    t[e] = Triple{(col << 32) | row, v};
+    if (sym) t[nedges + e] = Triple{(row << 32) | col, v};
+  }
+  if (noloops) {
+    size_t w = 0;
+    const size_t total = t.size();
+    for (size_t i = 0; i < total; ++i) {
+      int64_t e = (int64_t)(i < (size_t)nedges ? i : i - nedges);
+      if (keep[e]) t[w++] = t[i];
+    }
+    t.resize(w);
+  }
+  return triples_to_dcsc(t, n, n, pattern, out);
+}

@@ -1,0 +1,159 @@
+// Runtime entry points of the C ABI: devices, matrix transfer, argument
+// checks and the cbx_spgemm front door.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+#include "cbx_device.h"
+#include "cbx_spgemm.h"
+
+namespace cbx {
+
+static size_t dtype_size(cbx_dtype t) {
+  switch (t) {
+    case CBX_I64: return 8;
+    case CBX_F64: return 8;
+    case CBX_F32: return 4;
+    case CBX_BOOL: return 1;
+  }
+  return 0;
+}
+
+// Keep freed stream-ordered allocations in the device pool so repeated calls
+// do not go back to the driver.
+static cbx_status init_device_once() {
+  static std::once_flag flag;
+  static cbx_status result = CBX_OK;
+  std::call_once(flag, [] {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+      result = set_error(CBX_ERR_HIP, "no HIP device available");
+      return;
+    }
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t thr = ~0ull;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+  });
+  return result;
+}
+
+// Structural checks shared by all entry points that take a DCSC.
+static cbx_status check_dcsc(const cbx_dcsc* X, const char* name, cbx_location loc) {
+  CBX_RETURN_IF(!X, CBX_ERR_INVALID, "%s is null", name);
+  CBX_RETURN_IF(X->location != loc, CBX_ERR_INVALID, "%s: expected %s memory", name,
+                loc == CBX_DEVICE ? "device" : "host");
+  CBX_RETURN_IF(X->m < 0 || X->n < 0 || X->nnz < 0 || X->nzc < 0, CBX_ERR_INVALID, "%s: negative size", name);
+  CBX_RETURN_IF(X->m >= (1ll << 31) || X->n >= (1ll << 31), CBX_ERR_LIMIT,
+                "%s: dimensions must be below 2^31 (got %lld x %lld)", name, (long long)X->m, (long long)X->n);
+  CBX_RETURN_IF(X->nzc > X->n || X->nzc > X->nnz, CBX_ERR_INVALID, "%s: nzc=%lld inconsistent with n/nnz", name,
+                (long long)X->nzc);
+  CBX_RETURN_IF(dtype_size(X->dtype) == 0, CBX_ERR_INVALID, "%s: unknown dtype %d", name, (int)X->dtype);
+  CBX_RETURN_IF(X->nnz > 0 && (!X->cp || !X->jc || !X->ir || !X->val), CBX_ERR_INVALID, "%s: null array", name);
+  return CBX_OK;
+}
+
+}  // namespace cbx
+
+using namespace cbx;
+
+extern "C" int32_t cbx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" cbx_status cbx_device_synchronize(void) {
+  CBX_HIP(hipDeviceSynchronize());
+  return CBX_OK;
+}
+
+extern "C" void cbx_dcsc_free(cbx_dcsc* A) {
+  if (!A) return;
+  void* arrs[4] = {A->cp, A->jc, A->ir, A->val};
+  for (void* p : arrs) {
+    if (!p) continue;
+    if (A->location == CBX_DEVICE) (void)hipFree(p);
+    else std::free(p);
+  }
+  std::memset(A, 0, sizeof(*A));
+}
+
+static cbx_status copy_dcsc(const cbx_dcsc* src, cbx_dcsc* dst, cbx_location to) {
+  const size_t vs = dtype_size(src->dtype);
+  std::memset(dst, 0, sizeof(*dst));
+  dst->m = src->m;
+  dst->n = src->n;
+  dst->nnz = src->nnz;
+  dst->nzc = src->nzc;
+  dst->dtype = src->dtype;
+  dst->location = to;
+  const size_t bytes[4] = {sizeof(int64_t) * (size_t)(src->nzc + 1), sizeof(int64_t) * (size_t)src->nzc,
+                           sizeof(int64_t) * (size_t)src->nnz, vs * (size_t)src->nnz};
+  const void* s_arr[4] = {src->cp, src->jc, src->ir, src->val};
+  void** d_arr[4] = {(void**)&dst->cp, (void**)&dst->jc, (void**)&dst->ir, &dst->val};
+  const hipMemcpyKind kind = to == CBX_DEVICE ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  for (int i = 0; i < 4; ++i) {
+    const size_t b = bytes[i] ? bytes[i] : 8;
+    if (to == CBX_DEVICE) {
+      if (hipMalloc(d_arr[i], b) != hipSuccess) {
+        cbx_dcsc_free(dst);
+        return set_error(CBX_ERR_NOMEM, "device allocation of %zu bytes failed", b);
+      }
+    } else {
+      *d_arr[i] = std::malloc(b);
+      if (!*d_arr[i]) {
+        cbx_dcsc_free(dst);
+        return set_error(CBX_ERR_NOMEM, "host allocation of %zu bytes failed", b);
+      }
+    }
+    if (bytes[i] && s_arr[i]) {
+      hipError_t e = hipMemcpy(*d_arr[i], s_arr[i], bytes[i], kind);
+      if (e != hipSuccess) {
+        cbx_dcsc_free(dst);
+        return set_error(CBX_ERR_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
+      }
+    } else if (i == 0 && to == CBX_DEVICE) {
+      (void)hipMemset(*d_arr[i], 0, b);
+    } else if (i == 0) {
+      std::memset(*d_arr[i], 0, b);
+    }
+  }
+  return CBX_OK;
+}
+
+extern "C" cbx_status cbx_dcsc_to_device(const cbx_dcsc* host_src, cbx_dcsc* dev_dst) {
+  CBX_RETURN_IF(!dev_dst, CBX_ERR_INVALID, "cbx_dcsc_to_device: dst is null");
+  cbx_status st = check_dcsc(host_src, "src", CBX_HOST);
+  if (st) return st;
+  st = init_device_once();
+  if (st) return st;
+  return copy_dcsc(host_src, dev_dst, CBX_DEVICE);
+}
+
+extern "C" cbx_status cbx_dcsc_to_host(const cbx_dcsc* dev_src, cbx_dcsc* host_dst) {
+  CBX_RETURN_IF(!host_dst, CBX_ERR_INVALID, "cbx_dcsc_to_host: dst is null");
+  cbx_status st = check_dcsc(dev_src, "src", CBX_DEVICE);
+  if (st) return st;
+  return copy_dcsc(dev_src, host_dst, CBX_HOST);
+}
+
+extern "C" cbx_status cbx_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring semiring, cbx_spgemm_mode mode,
+                                 cbx_dcsc* C, cbx_spgemm_stats* stats, void* stream) {
+  cbx_status st;
+  if ((st = check_dcsc(A, "A", CBX_DEVICE))) return st;
+  if ((st = check_dcsc(B, "B", CBX_DEVICE))) return st;
+  CBX_RETURN_IF(!C, CBX_ERR_INVALID, "cbx_spgemm: C is null");
+  CBX_RETURN_IF(A->n != B->m, CBX_ERR_INVALID, "cbx_spgemm: inner dimensions differ (A is %lld x %lld, B is %lld x %lld)",
+                (long long)A->m, (long long)A->n, (long long)B->m, (long long)B->n);
+  CBX_RETURN_IF(A->dtype != B->dtype, CBX_ERR_UNSUPPORTED, "cbx_spgemm: A and B must share a dtype");
+  CBX_RETURN_IF(mode != CBX_SPGEMM_TWO_PHASE && mode != CBX_SPGEMM_UNSORTED && mode != CBX_SPGEMM_LEARNED,
+                CBX_ERR_INVALID, "cbx_spgemm: unknown mode %d", (int)mode);
+  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED, CBX_ERR_UNSUPPORTED, "cbx_spgemm: learned mode is not available yet");
+  if ((st = init_device_once())) return st;
+  SpgemmConfig cfg;
+  return spgemm_dispatch(A, B, semiring, mode, cfg, C, stats, (hipStream_t)stream);
+}

@@ -1,0 +1,21 @@
+// Internal interface of the SpGEMM driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cbx_internal.h"
+
+namespace cbx {
+
+// Tunables of the SpGEMM driver.
+struct SpgemmConfig {
+  // Columns with flop_j <= small_flop_max use the LDS hash tier, the rest the
+  // row-tiled dense tier.  At most 4096 (the largest hash table holds 8192
+  // slots and is kept at most half full).
+  int64_t small_flop_max = 4096;
+};
+
+cbx_status spgemm_dispatch(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring sr, cbx_spgemm_mode mode,
+                           const SpgemmConfig& cfg, cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s);
+
+}  // namespace cbx

@@ -1,0 +1,751 @@
+// Local semiring SpGEMM C = A*B on DCSC blocks for gfx950.
+//
+// Replaces CombBLAS's LocalHybridSpGEMM<SR,NTO> (include/CombBLAS/mtSpGEMM.h:213-460)
+// and LocalSpGEMMHash (mtSpGEMM.h:463-656).  The reference runs, per
+// non-empty column j of B:
+//   estimateFLOP     flop_j = sum_{k in B(:,j)} nnz(A(:,k))      (mtSpGEMM.h:1057-1134)
+//   estimateNNZ_Hash nnz_j  = distinct rows of the products       (mtSpGEMM.h:806-933)
+//   prefixsum        column pointers of C                          (mtSpGEMM.h:23-70)
+//   numeric          heap merge or hash + sort, rows ascending     (mtSpGEMM.h:289-441)
+// This file keeps those phases but maps them to the GPU as follows.
+//
+// Column tiers (by flop_j, after sorting columns by flop descending):
+//   hash tier  flop_j <= small_flop_max: one workgroup per column with an LDS
+//              linear-probing hash table of 2^ceil(log2(2*size)) slots, then
+//              in-LDS compaction and a bitonic sort of the surviving rows.
+//   dense tier the rest: one workgroup per column walks the rows in tiles of
+//              TR rows; each tile accumulates into a dense LDS array with a
+//              presence bitmap and is emitted in row order by a block scan
+//              of the bitmap, so no sort is needed.  The A-entries of a tile
+//              are found through a precomputed tile-pointer table
+//              P[a][s] = #entries of A's a-th stored column with row < s*TR.
+// Within every wave, the A-column segments of up to 64 B entries are
+// flattened with a prefix sum so that all 64 lanes work on products even
+// when most segments are short (R-MAT columns are).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "cbx_device.h"
+#include "cbx_semiring.h"
+#include "cbx_spgemm.h"
+
+namespace cbx {
+
+// ------------------------------------------------------------------ utils
+template <class T>
+__device__ __forceinline__ T shfl_any(T v, int src) {
+  if constexpr (sizeof(T) == 8) {
+    long long x = __builtin_bit_cast(long long, v);
+    x = __shfl(x, src, kWave);
+    return __builtin_bit_cast(T, x);
+  } else if constexpr (sizeof(T) == 4) {
+    int x = __builtin_bit_cast(int, v);
+    x = __shfl(x, src, kWave);
+    return __builtin_bit_cast(T, x);
+  } else {
+    int x = (int)v;
+    x = __shfl(x, src, kWave);
+    return (T)x;
+  }
+}
+
+// Visits the A entries of up to 64 segments, one segment per lane
+// ([lo, lo+len) with multiplier bv).  All lanes of the wave must call it.
+// f(idx, bv) runs once per entry; entries are spread over all lanes.
+template <class T, class F>
+__device__ __forceinline__ void wave_visit(int64_t lo, int len, T bv, F&& f) {
+  const int lane = lane_id();
+  const int incl = wave_incl_scan(len);
+  const int total = __shfl(incl, kWave - 1, kWave);
+  const int excl = incl - len;
+  for (int tb = 0; tb < total; tb += kWave) {
+    const int t = tb + lane;
+    int L = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      const int cand = L + step;
+      const int ex = __shfl(excl, cand, kWave);
+      if (ex <= t) L = cand;
+    }
+    const int64_t loL = shfl_any(lo, L);
+    const int exL = __shfl(excl, L, kWave);
+    const T bvL = shfl_any(bv, L);
+    if (t < total) f(loL + (int64_t)(t - exL), bvL);
+  }
+}
+
+__device__ __forceinline__ uint32_t hash_slot(int32_t r, int lg) {
+  return ((uint32_t)r * 2654435761u) >> (32 - lg);
+}
+
+__device__ __forceinline__ int ceil_log2_dev(long long x) {
+  int lg = 0;
+  while ((1ll << lg) < x) ++lg;
+  return lg;
+}
+
+// ------------------------------------------------------------------ setup
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_iota_i32(int32_t* p, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int32_t)i;
+}
+
+// colmap[jcA[p]] = p  (dense lookup replacing Dcsc::ConstructAux/FillColInds,
+// dcsc.cpp:1042-1070 and :1361-1423)
+__global__ void k_colmap(const int64_t* __restrict__ jc, int64_t nzc, int32_t* __restrict__ colmap) {
+  int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < nzc) colmap[jc[p]] = (int32_t)p;
+}
+
+__global__ void k_rows32(const int64_t* __restrict__ ir, int64_t nnz, int32_t* __restrict__ out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nnz) out[i] = (int32_t)ir[i];
+}
+
+// bpos[e] = stored-column position in A of B's row index irB[e] (-1 if A(:,k) is empty).
+__global__ void k_bpos(const int64_t* __restrict__ irB, int64_t nnzB, const int32_t* __restrict__ colmap,
+                       int32_t* __restrict__ bpos) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < nnzB) bpos[e] = colmap[irB[e]];
+}
+
+// flop_j (estimateFLOP, mtSpGEMM.h:1112-1130): one wave per column of B.
+__global__ void k_flop(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
+                       const int64_t* __restrict__ cpA, int64_t* __restrict__ flop) {
+  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  if (col >= nzcB) return;
+  const int lane = lane_id();
+  long long f = 0;
+  for (int64_t e = cpB[col] + lane; e < cpB[col + 1]; e += kWave) {
+    const int32_t p = bpos[e];
+    if (p >= 0) f += cpA[p + 1] - cpA[p];
+  }
+  f = wave_sum_ll(f);
+  if (lane == 0) flop[col] = f;
+}
+
+// Tile pointers: P[p*(S+1)+s] = number of entries of A's p-th stored column
+// with row < s*TR.  One thread per stored column.
+__global__ void k_ptable(const int64_t* __restrict__ cpA, int64_t nzcA, const int32_t* __restrict__ rowsA,
+                         int S, int TR, int32_t* __restrict__ P) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= nzcA) return;
+  const int64_t b = cpA[p];
+  const int len = (int)(cpA[p + 1] - b);
+  int32_t* out = P + p * (int64_t)(S + 1);
+  int t = 0;
+  for (int i = 0; i < len; ++i) {
+    const int tile = rowsA[b + i] / TR;
+    while (t <= tile) out[t++] = i;
+  }
+  while (t <= S) out[t++] = len;
+}
+
+// Class boundaries over columns sorted by flop descending:
+// bounds[c] = number of columns with flop > thr[c].
+__global__ void k_class_bounds(const int64_t* __restrict__ flop_sorted, int64_t n, const int64_t* thr, int nthr,
+                               int64_t* bounds) {
+  const int c = threadIdx.x;
+  if (c >= nthr) return;
+  int64_t lo = 0, hi = n;  // first index with flop <= thr[c]
+  while (lo < hi) {
+    int64_t mid = (lo + hi) / 2;
+    if (flop_sorted[mid] > thr[c]) lo = mid + 1;
+    else hi = mid;
+  }
+  bounds[c] = lo;
+}
+
+__global__ void k_cp_finish(int64_t* nnzcol, int64_t nzcB) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) nnzcol[nzcB] = 0;
+}
+
+__global__ void k_nonempty_flags(const int64_t* __restrict__ nnzcol, int64_t n, uint8_t* __restrict__ flags) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = nnzcol[i] > 0;
+}
+
+__global__ void k_set_last(int64_t* cp, const int64_t* nsel, const int64_t* total_src, int64_t total_idx) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) cp[*nsel] = total_src[total_idx];
+}
+
+// Input validation (the reference's preconditions, mtSpGEMM.h / dcsc.h):
+// cp starts at 0, ends at nnz and never decreases; jc strictly increasing and
+// below ncols; rows of every column strictly increasing and below nrows.
+// One wave per stored column.  Sets bit `flagbit` of *err on a violation.
+__global__ void k_validate(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
+                           const int64_t* __restrict__ ir, int64_t nzc, int64_t nnz, int64_t nrows,
+                           int64_t ncols, int flagbit, int* __restrict__ err) {
+  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  if (col >= nzc) return;
+  const int lane = lane_id();
+  const int64_t b = cp[col], e = cp[col + 1];
+  bool bad = false;
+  if (lane == 0) {
+    bad |= b < 0 || e < b || e > nnz;
+    bad |= jc[col] < 0 || jc[col] >= ncols;
+    if (col + 1 < nzc) bad |= jc[col + 1] <= jc[col];
+    if (col == 0) bad |= b != 0;
+    if (col == nzc - 1) bad |= e != nnz;
+  }
+  if (!(b < 0 || e < b || e > nnz)) {
+    for (int64_t i = b + lane; i < e; i += kWave) {
+      const int64_t r = ir[i];
+      bad |= r < 0 || r >= nrows;
+      if (i + 1 < e) bad |= ir[i + 1] <= r;
+    }
+  }
+  if (bad) atomicOr(err, 1 << flagbit);
+}
+
+// --------------------------------------------------------- symbolic: hash
+// estimateNNZ_Hash restated for LDS (mtSpGEMM.h:861-927): insert each product's
+// row into a linear-probing table, count first insertions.
+template <int HMAX, int NT>
+__global__ __launch_bounds__(NT) void k_sym_hash(const int32_t* __restrict__ order, int64_t first,
+                                                 const int64_t* __restrict__ flop, const int64_t* __restrict__ cpB,
+                                                 const int32_t* __restrict__ bpos, const int64_t* __restrict__ cpA,
+                                                 const int32_t* __restrict__ rowsA, int64_t* __restrict__ nnzcol,
+                                                 int* __restrict__ err) {
+  __shared__ int32_t keys[HMAX];
+  __shared__ long long wsum[NT / kWave];
+  const int col = order[first + blockIdx.x];
+  const int lg = min(ceil_log2_dev(max(2 * flop[col], 64ll)), ceil_log2_dev(HMAX));
+  const int H = 1 << lg;
+  for (int i = threadIdx.x; i < H; i += NT) keys[i] = -1;
+  __syncthreads();
+  long long cnt = 0;
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  for (int64_t eb = b0 + wave_id() * kWave; eb < b1; eb += NT) {
+    const int64_t e = eb + lane_id();
+    int64_t lo = 0;
+    int len = 0;
+    if (e < b1) {
+      const int32_t p = bpos[e];
+      if (p >= 0) {
+        lo = cpA[p];
+        len = (int)(cpA[p + 1] - lo);
+      }
+    }
+    wave_visit(lo, len, 0, [&](int64_t idx, int) {
+      const int32_t r = rowsA[idx];
+      uint32_t h = hash_slot(r, lg);
+      int probe = 0;
+      for (; probe < H; ++probe) {  // the table is at least twice the column's flops
+        const int32_t old = atomicCAS(&keys[h], -1, r);
+        if (old == -1) { ++cnt; break; }
+        if (old == r) break;
+        h = (h + 1) & (H - 1);
+      }
+      if (probe == H) atomicOr(err, 1);
+    });
+  }
+  const long long tot = block_sum_ll<NT>(cnt, wsum);
+  if (threadIdx.x == 0) nnzcol[col] = tot;
+}
+
+// -------------------------------------------------------- symbolic: dense
+// Distinct-row count with an LDS bitmap of RBW*32 rows per pass; a pass
+// covers G row tiles of the tile-pointer table.
+template <int NT, int RBW>
+__global__ __launch_bounds__(NT) void k_sym_dense(const int32_t* __restrict__ order, int64_t first,
+                                                  const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+                                                  const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+                                                  const int32_t* __restrict__ P, int S, int G, int TR,
+                                                  int64_t* __restrict__ nnzcol) {
+  __shared__ uint32_t bits[RBW];
+  __shared__ long long wsum[NT / kWave];
+  const int col = order[first + blockIdx.x];
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  long long cnt = 0;
+  const int npass = (S + G - 1) / G;
+  for (int sp = 0; sp < npass; ++sp) {
+    for (int i = threadIdx.x; i < RBW; i += NT) bits[i] = 0u;
+    __syncthreads();
+    const int s_lo = sp * G, s_hi = min(S, s_lo + G);
+    const int32_t row0 = s_lo * TR;
+    for (int64_t eb = b0 + wave_id() * kWave; eb < b1; eb += NT) {
+      const int64_t e = eb + lane_id();
+      int64_t lo = 0;
+      int len = 0;
+      if (e < b1) {
+        const int32_t p = bpos[e];
+        if (p >= 0) {
+          if (npass == 1) {
+            lo = cpA[p];
+            len = (int)(cpA[p + 1] - lo);
+          } else {
+            const int32_t* pp = P + (int64_t)p * (S + 1);
+            const int a = pp[s_lo];
+            lo = cpA[p] + a;
+            len = pp[s_hi] - a;
+          }
+        }
+      }
+      wave_visit(lo, len, 0, [&](int64_t idx, int) {
+        const int32_t r = rowsA[idx] - row0;
+        const uint32_t bit = 1u << (r & 31);
+        const uint32_t old = atomicOr(&bits[r >> 5], bit);
+        if (!(old & bit)) ++cnt;
+      });
+    }
+    __syncthreads();
+  }
+  const long long tot = block_sum_ll<NT>(cnt, wsum);
+  if (threadIdx.x == 0) nnzcol[col] = tot;
+}
+
+// ---------------------------------------------------------- numeric: hash
+// Hash accumulation (mtSpGEMM.h:362-423 / :595-622), compaction (:425-432)
+// and sort by row (:434); SORT=false keeps hash order like LocalSpGEMMHash
+// with sort=false (:624-634).
+template <int K, class T, int HMAX, int NT, bool SORT>
+__global__ __launch_bounds__(NT) void k_num_hash(const int32_t* __restrict__ order, int64_t first,
+                                                 const int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpB,
+                                                 const int32_t* __restrict__ bpos, const T* __restrict__ valB,
+                                                 const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+                                                 const T* __restrict__ valA, const int64_t* __restrict__ cpC,
+                                                 int64_t* __restrict__ irC, T* __restrict__ valC,
+                                                 int* __restrict__ err) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  __shared__ int32_t keys[HMAX];
+  __shared__ Acc acc[HMAX];
+  __shared__ int wsum[NT / kWave];
+  const int col = order[first + blockIdx.x];
+  const int64_t nz = nnzcol[col];
+  const int lg = min(ceil_log2_dev(max(2 * nz, 64ll)), ceil_log2_dev(HMAX));
+  const int H = 1 << lg;
+  for (int i = threadIdx.x; i < H; i += NT) {
+    keys[i] = -1;
+    acc[i] = SR::ident();
+  }
+  __syncthreads();
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  for (int64_t eb = b0 + wave_id() * kWave; eb < b1; eb += NT) {
+    const int64_t e = eb + lane_id();
+    int64_t lo = 0;
+    int len = 0;
+    T bv{};
+    if (e < b1) {
+      const int32_t p = bpos[e];
+      if (p >= 0) {
+        lo = cpA[p];
+        len = (int)(cpA[p + 1] - lo);
+        bv = valB[e];
+      }
+    }
+    wave_visit(lo, len, bv, [&](int64_t idx, T b) {
+      const int32_t r = rowsA[idx];
+      const Acc v = SR::mul(valA[idx], b);
+      uint32_t h = hash_slot(r, lg);
+      int probe = 0;
+      for (; probe < H; ++probe) {  // the table is at least twice the column's nnz
+        const int32_t old = atomicCAS(&keys[h], -1, r);
+        if (old == -1 || old == r) break;
+        h = (h + 1) & (H - 1);
+      }
+      if (probe < H) SR::acc(&acc[h], v);
+      else atomicOr(err, 2);
+    });
+  }
+  __syncthreads();
+  // in-place compaction, NT slots at a time
+  int base = 0;
+  for (int c0 = 0; c0 < H; c0 += NT) {
+    const int i = c0 + threadIdx.x;
+    const int32_t k = i < H ? keys[i] : -1;
+    const Acc a = i < H ? acc[i] : SR::ident();
+    int tot;
+    const int ex = block_excl_scan<NT>(k != -1 ? 1 : 0, wsum, tot);
+    if (k != -1) {
+      keys[base + ex] = k;
+      acc[base + ex] = a;
+    }
+    base += tot;
+    __syncthreads();
+  }
+  if (base != nz && threadIdx.x == 0) atomicOr(err, 8);
+  const int cnt = min(base, (int)nz);
+  if constexpr (SORT) {
+    int N = 1;
+    while (N < cnt) N <<= 1;
+    for (int i = cnt + threadIdx.x; i < N; i += NT) keys[i] = 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= N; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < N; i += NT) {
+          const int ixj = i ^ j;
+          if (ixj > i) {
+            const int32_t ki = keys[i], kx = keys[ixj];
+            const bool up = (i & k) == 0;
+            if ((ki > kx) == up) {
+              keys[i] = kx;
+              keys[ixj] = ki;
+              const Acc t = acc[i];
+              acc[i] = acc[ixj];
+              acc[ixj] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  const int64_t out = cpC[col];
+  for (int i = threadIdx.x; i < cnt; i += NT) {
+    irC[out + i] = keys[i];
+    valC[out + i] = SR::out(acc[i]);
+  }
+}
+
+// --------------------------------------------------------- numeric: dense
+// Row-tiled dense accumulation: TR rows per tile, TR = 32*NT so that each
+// thread owns one 32-bit word of the presence bitmap during emission.
+template <int K, class T, int NT>
+__global__ __launch_bounds__(NT) void k_num_dense(const int32_t* __restrict__ order, int64_t first,
+                                                  const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+                                                  const T* __restrict__ valB, const int64_t* __restrict__ cpA,
+                                                  const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
+                                                  const int32_t* __restrict__ P, int S,
+                                                  const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
+                                                  T* __restrict__ valC, int* __restrict__ err) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int TR = 32 * NT;
+  __shared__ Acc acc[TR];
+  __shared__ uint32_t bits[NT];
+  __shared__ int wsum[NT / kWave];
+  const int col = order[first + blockIdx.x];
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  int64_t out = cpC[col];
+  const int64_t out_end = cpC[col + 1];
+  for (int i = threadIdx.x; i < TR; i += NT) acc[i] = SR::ident();
+  bits[threadIdx.x] = 0u;
+  __syncthreads();
+  for (int s = 0; s < S; ++s) {
+    const int32_t row0 = s * TR;
+    for (int64_t eb = b0 + wave_id() * kWave; eb < b1; eb += NT) {
+      const int64_t e = eb + lane_id();
+      int64_t lo = 0;
+      int len = 0;
+      T bv{};
+      if (e < b1) {
+        const int32_t p = bpos[e];
+        if (p >= 0) {
+          const int32_t* pp = P + (int64_t)p * (S + 1) + s;
+          const int a = pp[0];
+          lo = cpA[p] + a;
+          len = pp[1] - a;
+          bv = valB[e];
+        }
+      }
+      wave_visit(lo, len, bv, [&](int64_t idx, T b) {
+        const int32_t r = rowsA[idx] - row0;
+        SR::acc(&acc[r], SR::mul(valA[idx], b));
+        atomicOr(&bits[r >> 5], 1u << (r & 31));
+      });
+    }
+    __syncthreads();
+    uint32_t w = bits[threadIdx.x];
+    int tot;
+    const int ex = block_excl_scan<NT>(__popc(w), wsum, tot);
+    int64_t o = out + ex;
+    while (w) {
+      const int b = __ffs(w) - 1;
+      w &= w - 1;
+      const int r = threadIdx.x * 32 + b;
+      if (o < out_end) {  // symbolic and numeric agree; guard anyway
+        irC[o] = (int64_t)row0 + r;
+        valC[o] = SR::out(acc[r]);
+      } else {
+        atomicOr(err, 4);
+      }
+      acc[r] = SR::ident();
+      ++o;
+    }
+    bits[threadIdx.x] = 0u;
+    out += tot;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ driver
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+};
+
+template <class X>
+cbx_status dalloc(DevBuf& b, size_t count, hipStream_t s, X** out) {
+  b.s = s;
+  size_t bytes = std::max<size_t>(count * sizeof(X), 256);
+  hipError_t e = hipMallocAsync(&b.p, bytes, s);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return set_error(CBX_ERR_NOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  *out = (X*)b.p;
+  return CBX_OK;
+}
+
+#define CBX_TRY(x)                 \
+  do {                             \
+    cbx_status st_ = (x);          \
+    if (st_ != CBX_OK) return st_; \
+  } while (0)
+
+inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct Timer {
+  hipEvent_t ev[5];
+  bool on;
+  explicit Timer(bool enable) : on(enable) {
+    if (on)
+      for (auto& e : ev) (void)hipEventCreate(&e);
+  }
+  ~Timer() {
+    if (on)
+      for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  void mark(int i, hipStream_t s) {
+    if (on) (void)hipEventRecord(ev[i], s);
+  }
+  double ms(int a, int b) {
+    float f = 0;
+    if (on) (void)hipEventElapsedTime(&f, ev[a], ev[b]);
+    return f;
+  }
+};
+
+constexpr int kHashNT = 256;
+constexpr int kDenseNT = 512;                 // dense numeric threads
+constexpr int kDenseTR = 32 * kDenseNT;       // rows per numeric tile (16384)
+constexpr int kSymNT = 512;
+constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass (2^20 rows)
+
+}  // namespace
+
+template <int K, class T>
+static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_mode mode, const SpgemmConfig& cfg,
+                               cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s) {
+  const int64_t m = A->m, n = B->n;
+  const int64_t nzcA = A->nzc, nzcB = B->nzc, nnzA = A->nnz, nnzB = B->nnz;
+  Timer tm(stats != nullptr);
+  tm.mark(0, s);
+
+  std::memset(C, 0, sizeof(*C));
+  C->m = m;
+  C->n = n;
+  C->dtype = A->dtype;
+  C->location = CBX_DEVICE;
+
+  if (nnzA == 0 || nnzB == 0 || nzcA == 0 || nzcB == 0) {  // mtSpGEMM.h:224-227
+    CBX_HIP(hipMalloc(&C->cp, sizeof(int64_t)));
+    CBX_HIP(hipMemsetAsync(C->cp, 0, sizeof(int64_t), s));
+    CBX_HIP(hipStreamSynchronize(s));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return CBX_OK;
+  }
+
+  const int TR = kDenseTR;
+  const int S = (int)((m + TR - 1) / TR);
+  const int G = kSymRBW * 32 / TR;  // tiles per symbolic pass
+
+  DevBuf b_colmap, b_rows, b_bpos, b_flop, b_order, b_flop_s, b_order_s, b_P, b_nnz, b_cp, b_tmp, b_thr, b_bounds,
+      b_flags, b_nsel, b_err;
+  int32_t *colmap, *rowsA, *bpos, *order, *order_s, *P;
+  int* err;
+  int64_t *flop, *flop_s, *nnzcol, *cpC, *thr, *bounds, *nsel;
+  uint8_t* flags;
+  CBX_TRY(dalloc(b_colmap, A->n, s, &colmap));
+  CBX_TRY(dalloc(b_rows, nnzA, s, &rowsA));
+  CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
+  CBX_TRY(dalloc(b_flop, nzcB, s, &flop));
+  CBX_TRY(dalloc(b_flop_s, nzcB, s, &flop_s));
+  CBX_TRY(dalloc(b_order, nzcB, s, &order));
+  CBX_TRY(dalloc(b_order_s, nzcB, s, &order_s));
+  CBX_TRY(dalloc(b_P, (size_t)nzcA * (S + 1), s, &P));
+  CBX_TRY(dalloc(b_nnz, nzcB + 1, s, &nnzcol));
+  CBX_TRY(dalloc(b_cp, nzcB + 1, s, &cpC));
+  CBX_TRY(dalloc(b_thr, 8, s, &thr));
+  CBX_TRY(dalloc(b_bounds, 8, s, &bounds));
+  CBX_TRY(dalloc(b_flags, nzcB, s, &flags));
+  CBX_TRY(dalloc(b_nsel, 2, s, &nsel));
+  CBX_TRY(dalloc(b_err, 1, s, &err));
+  CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+
+  const T* valA = (const T*)A->val;
+  const T* valB = (const T*)B->val;
+
+  // --- validate both operands before any indexed access
+  k_validate<<<nblk(nzcA * kWave, 256), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
+  k_validate<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
+  {
+    int herr = 0;
+    CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    CBX_HIP(hipStreamSynchronize(s));
+    if (herr)
+      return set_error(CBX_ERR_INVALID,
+                       "cbx_spgemm: %s is not a valid DCSC (cp/jc/ir out of range, or rows not strictly "
+                       "increasing within a column)", (herr & 16) ? "A" : "B");
+  }
+
+  // --- setup: column map, 32-bit rows, B->A positions, flops, tile pointers
+  k_fill_i32<<<nblk(A->n, 256), 256, 0, s>>>(colmap, A->n, -1);
+  k_colmap<<<nblk(nzcA, 256), 256, 0, s>>>(A->jc, nzcA, colmap);
+  k_rows32<<<nblk(nnzA, 256), 256, 0, s>>>(A->ir, nnzA, rowsA);
+  k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
+  k_flop<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, flop);
+  k_ptable<<<nblk(nzcA, 256), 256, 0, s>>>(A->cp, nzcA, rowsA, S, TR, P);
+  k_iota_i32<<<nblk(nzcB, 256), 256, 0, s>>>(order, nzcB);
+
+  // sort columns by flop, descending (largest work first)
+  size_t tmp_bytes = 0, tmp2 = 0;
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, flop, flop_s, order, order_s, (int)nzcB, 0,
+                                                       64, s));
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, nnzcol, cpC, (int)(nzcB + 1), s));
+  tmp_bytes = std::max(tmp_bytes, tmp2);
+  CBX_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp2, B->jc, flags, (int64_t*)nullptr, nsel, (int)nzcB, s));
+  tmp_bytes = std::max(tmp_bytes, tmp2);
+  void* tmp;
+  CBX_TRY(dalloc(b_tmp, tmp_bytes, s, (uint8_t**)&tmp));
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, flop, flop_s, order, order_s, (int)nzcB, 0, 64,
+                                                       s));
+
+  // class thresholds: dense > small_max ; hash8k > 1024 ; hash2k > 128 ; hash256 > 0
+  int64_t hthr[4] = {cfg.small_flop_max, 1024, 128, 0};
+  CBX_HIP(hipMemcpyAsync(thr, hthr, sizeof(hthr), hipMemcpyHostToDevice, s));
+  k_class_bounds<<<1, 64, 0, s>>>(flop_s, nzcB, thr, 4, bounds);
+  int64_t hb[4];
+  CBX_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemsetAsync(nnzcol, 0, sizeof(int64_t) * (nzcB + 1), s));
+  CBX_HIP(hipStreamSynchronize(s));
+  const int64_t n_dense = hb[0], n_h8 = hb[1] - hb[0], n_h2 = hb[2] - hb[1], n_h256 = hb[3] - hb[2];
+  int64_t total_flop = 0;
+  tm.mark(1, s);
+
+  // --- symbolic
+  if (n_dense) k_sym_dense<kSymNT, kSymRBW><<<(unsigned)n_dense, kSymNT, 0, s>>>(order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, G, TR, nnzcol);
+  if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+  if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+  if (n_h256) k_sym_hash<256, kHashNT><<<(unsigned)n_h256, kHashNT, 0, s>>>(order_s, hb[2], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+  CBX_HIP(hipGetLastError());
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
+  int64_t nnzC = 0;
+  CBX_HIP(hipMemcpyAsync(&nnzC, cpC + nzcB, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  if (stats) {
+    // flop total: reuse the scan on the flop array is unnecessary; sum on host-side copy of sorted flops is costly,
+    // so compute with a device reduction into bounds[4].
+    size_t rb = 0;
+    CBX_HIP(hipcub::DeviceReduce::Sum(nullptr, rb, flop, bounds + 4, (int)nzcB, s));
+    if (rb <= tmp_bytes) {
+      CBX_HIP(hipcub::DeviceReduce::Sum(tmp, rb, flop, bounds + 4, (int)nzcB, s));
+      CBX_HIP(hipMemcpyAsync(&total_flop, bounds + 4, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    }
+  }
+  CBX_HIP(hipStreamSynchronize(s));
+  tm.mark(2, s);
+
+  // --- output allocation
+  CBX_HIP(hipMalloc(&C->cp, sizeof(int64_t) * (nzcB + 1)));
+  CBX_HIP(hipMalloc(&C->jc, sizeof(int64_t) * std::max<int64_t>(nzcB, 1)));
+  CBX_HIP(hipMalloc(&C->ir, sizeof(int64_t) * std::max<int64_t>(nnzC, 1)));
+  CBX_HIP(hipMalloc(&C->val, sizeof(T) * std::max<int64_t>(nnzC, 1)));
+  C->nnz = nnzC;
+
+  // --- numeric
+  if (n_dense)
+    k_num_dense<K, T, kDenseNT><<<(unsigned)n_dense, kDenseNT, 0, s>>>(order_s, 0, B->cp, bpos, valB, A->cp, rowsA, valA, P,
+                                                                     S, cpC, C->ir, (T*)C->val, err);
+  const bool sort = mode != CBX_SPGEMM_UNSORTED;
+#define CBX_LAUNCH_HASH(HM, NCOL, FIRST)                                                                          \
+  if (NCOL) {                                                                                                     \
+    if (sort)                                                                                                     \
+      k_num_hash<K, T, HM, kHashNT, true><<<(unsigned)(NCOL), kHashNT, 0, s>>>(                                   \
+          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);                 \
+    else                                                                                                          \
+      k_num_hash<K, T, HM, kHashNT, false><<<(unsigned)(NCOL), kHashNT, 0, s>>>(                                  \
+          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);                 \
+  }
+  CBX_LAUNCH_HASH(8192, n_h8, hb[0]);
+  CBX_LAUNCH_HASH(2048, n_h2, hb[1]);
+  CBX_LAUNCH_HASH(256, n_h256, hb[2]);
+#undef CBX_LAUNCH_HASH
+  CBX_HIP(hipGetLastError());
+
+  // --- DCSC of C: keep the non-empty columns
+  k_nonempty_flags<<<nblk(nzcB, 256), 256, 0, s>>>(nnzcol, nzcB, flags);
+  CBX_HIP(hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, B->jc, flags, C->jc, nsel, (int)nzcB, s));
+  CBX_HIP(hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, cpC, flags, C->cp, nsel + 1, (int)nzcB, s));
+  k_set_last<<<1, 1, 0, s>>>(C->cp, nsel, cpC, nzcB);
+  int64_t nzcC = 0;
+  int herr = 0;
+  CBX_HIP(hipMemcpyAsync(&nzcC, nsel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  tm.mark(3, s);
+  CBX_HIP(hipStreamSynchronize(s));
+  C->nzc = nzcC;
+  if (herr) {
+    cbx_dcsc_free(C);
+    return set_error(CBX_ERR_HIP, "cbx_spgemm: internal hash table overflow (flag %d)", herr);
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->flops = total_flop;
+    stats->nnz = nnzC;
+    stats->nzc = nzcC;
+    stats->ms_setup = tm.ms(0, 1);
+    stats->ms_symbolic = tm.ms(1, 2);
+    stats->ms_numeric = tm.ms(2, 3);
+    stats->ms_total = tm.ms(0, 3);
+  }
+  return CBX_OK;
+}
+
+cbx_status spgemm_dispatch(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring sr, cbx_spgemm_mode mode,
+                           const SpgemmConfig& cfg, cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s) {
+  switch (sr) {
+    case CBX_SR_PLUS_TIMES:
+      switch (A->dtype) {
+        case CBX_I64: return spgemm_typed<kPlusTimes, int64_t>(A, B, mode, cfg, C, stats, s);
+        case CBX_F64: return spgemm_typed<kPlusTimes, double>(A, B, mode, cfg, C, stats, s);
+        case CBX_F32: return spgemm_typed<kPlusTimes, float>(A, B, mode, cfg, C, stats, s);
+        case CBX_BOOL: return spgemm_typed<kPlusTimes, uint8_t>(A, B, mode, cfg, C, stats, s);
+      }
+      break;
+    case CBX_SR_MIN_PLUS:
+      switch (A->dtype) {
+        case CBX_I64: return spgemm_typed<kMinPlus, int64_t>(A, B, mode, cfg, C, stats, s);
+        case CBX_F64: return spgemm_typed<kMinPlus, double>(A, B, mode, cfg, C, stats, s);
+        case CBX_F32: return spgemm_typed<kMinPlus, float>(A, B, mode, cfg, C, stats, s);
+      }
+      break;
+    case CBX_SR_SELECT_MAX:
+      switch (A->dtype) {
+        case CBX_I64: return spgemm_typed<kSelectMax, int64_t>(A, B, mode, cfg, C, stats, s);
+        case CBX_F64: return spgemm_typed<kSelectMax, double>(A, B, mode, cfg, C, stats, s);
+        case CBX_F32: return spgemm_typed<kSelectMax, float>(A, B, mode, cfg, C, stats, s);
+      }
+      break;
+  }
+  return set_error(CBX_ERR_UNSUPPORTED, "cbx_spgemm: semiring %d with dtype %d is not provided", (int)sr,
+                   (int)A->dtype);
+}
+
+}  // namespace cbx

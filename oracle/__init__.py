@@ -1,0 +1,169 @@
+"""ORACLE -- test infrastructure only (the checker, never the product).
+
+Python binding of the CPU restatement in ``oracle/cbx_oracle.cpp``
+(``oracle/liboracle.so``) and, when it was built in this checkout, of the
+genuine CombBLAS kernels compiled from the reference tree
+(``oracle/_ref/libcbref.so``).  Only tests/, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU-baseline leg use this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from combblas_amd import Dcsc, cbx_dcsc, SEMIRINGS
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libcbref.so"
+REF_TREE = Path("/root/reference")
+
+MODES = {"hybrid": 0, "hash": 1, "hash_unsorted": 2}
+
+_orc = None
+_ref = None
+
+
+def build(ref: bool | None = None) -> None:
+    """Compile liboracle.so, and _ref/libcbref.so when the reference tree is present."""
+    targets = ["all"]
+    if ref is None:
+        ref = (REF_TREE / "include" / "CombBLAS" / "CombBLAS.h").exists()
+    if ref:
+        targets.append("ref")
+    r = subprocess.run(["make", "-s", "-C", str(HERE), *targets], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+
+
+def _load_oracle() -> C.CDLL:
+    global _orc
+    if _orc is None:
+        if not ORACLE_LIB.exists():
+            build(ref=False)
+        L = C.CDLL(str(ORACLE_LIB))
+        P = C.POINTER
+        L.orc_spgemm.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, C.c_int64, C.c_int64,
+                                 P(cbx_dcsc), P(C.c_int64)]
+        L.orc_spgemm.restype = C.c_int32
+        L.orc_symbolic.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
+        L.orc_symbolic.restype = C.c_int32
+        L.orc_free.argtypes = [P(cbx_dcsc)]
+        L.orc_free.restype = None
+        L.orc_max_threads.restype = C.c_int32
+        L.orc_set_threads.argtypes = [C.c_int32]
+        _orc = L
+    return _orc
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def _load_ref() -> C.CDLL:
+    global _ref
+    if _ref is None:
+        if not REF_LIB.exists():
+            raise RuntimeError("oracle/_ref/libcbref.so is not built (needs the reference tree)")
+        L = C.CDLL(str(REF_LIB))
+        P = C.POINTER
+        L.ref_spgemm.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, P(cbx_dcsc)]
+        L.ref_spgemm.restype = C.c_int32
+        L.ref_symbolic.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
+        L.ref_symbolic.restype = C.c_int32
+        L.ref_free.argtypes = [P(cbx_dcsc)]
+        L.ref_free.restype = None
+        _ref = L
+    return _ref
+
+
+def spgemm(A: Dcsc, B: Dcsc, semiring: str = "plus_times", mode: str = "hybrid",
+           cols: tuple[int, int] | None = None) -> tuple[Dcsc, int]:
+    """Oracle product (restatement).  cols=(c0, c1) restricts to B's stored
+    columns [c0, c1).  Returns (C, flops)."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    out = cbx_dcsc()
+    fl = C.c_int64(0)
+    c0, c1 = cols if cols is not None else (0, -1)
+    st = L.orc_spgemm(C.byref(sa), C.byref(sb), SEMIRINGS[semiring], MODES[mode], c0, c1, C.byref(out),
+                      C.byref(fl))
+    if st != 0:
+        raise RuntimeError(f"orc_spgemm failed with status {st}")
+    return Dcsc.from_struct(out, L.orc_free), int(fl.value)
+
+
+def symbolic(A: Dcsc, B: Dcsc) -> tuple[np.ndarray, np.ndarray]:
+    """(flop_j, nnz_j) for every stored column of B (estimateFLOP, estimateNNZ_Hash)."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    flop = np.zeros(B.nzc, np.int64)
+    nnz = np.zeros(B.nzc, np.int64)
+    st = L.orc_symbolic(C.byref(sa), C.byref(sb), flop.ctypes.data, nnz.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"orc_symbolic failed with status {st}")
+    return flop, nnz
+
+
+def set_threads(n: int) -> None:
+    _load_oracle().orc_set_threads(n)
+
+
+def max_threads() -> int:
+    return int(_load_oracle().orc_max_threads())
+
+
+def ref_spgemm(A: Dcsc, B: Dcsc, semiring: str = "plus_times", mode: str = "hybrid") -> Dcsc:
+    """The genuine reference kernel (only where oracle/_ref was built)."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    out = cbx_dcsc()
+    st = L.ref_spgemm(C.byref(sa), C.byref(sb), SEMIRINGS[semiring], MODES[mode], C.byref(out))
+    if st != 0:
+        raise RuntimeError(f"ref_spgemm failed with status {st}")
+    return Dcsc.from_struct(out, L.ref_free)
+
+
+def ref_symbolic(A: Dcsc, B: Dcsc) -> tuple[np.ndarray, np.ndarray]:
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    flop = np.zeros(B.nzc, np.int64)
+    nnz = np.zeros(B.nzc, np.int64)
+    L.ref_symbolic(C.byref(sa), C.byref(sb), flop.ctypes.data, nnz.ctypes.data)
+    return flop, nnz
+
+
+# ------------------------------------------------------------------ checks
+def _mix(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def column_digests(X: Dcsc) -> np.ndarray:
+    """Order-sensitive digest of every stored column: sum over the column's
+    entries t = 0, 1, ... of (t+1) * mix(row, value bits), modulo 2^64.
+    Returns uint64[nzc]."""
+    with np.errstate(over="ignore"):
+        if X.nnz == 0:
+            return np.zeros(X.nzc, np.uint64)
+        vb = X.val
+        if vb.dtype.itemsize == 8:
+            bits = vb.view(np.uint64)
+        elif vb.dtype.itemsize == 4:
+            bits = vb.view(np.uint32).astype(np.uint64)
+        else:
+            bits = vb.astype(np.uint64)
+        h = _mix(X.ir.astype(np.uint64) ^ _mix(bits))
+        pos = np.arange(X.nnz, dtype=np.int64) - np.repeat(X.cp[:-1], np.diff(X.cp))
+        h = h * (pos.astype(np.uint64) + np.uint64(1))
+        return np.add.reduceat(h, X.cp[:-1]) if X.nzc else np.zeros(0, np.uint64)

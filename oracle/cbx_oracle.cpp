@@ -1,0 +1,369 @@
+// ORACLE -- test infrastructure only.
+//
+// A CPU restatement of CombBLAS's node-local kernels, used by tests/, by
+// __graft_entry__.smoke() and by bench.py's CPU-baseline leg as the checker.
+// Nothing in the product (combblas_amd/, include/) may call it.
+//
+// Every function cites the reference code it restates (paths relative to the
+// CombBLAS tree).  The arithmetic order is kept wherever it decides floating
+// point results: the heap branch uses std::make_heap/pop_heap/push_heap with
+// the reference's swapped comparison (HeapEntry.h:50-53, a min-heap on the
+// row key) so ties pop in the same order; the hash branch uses the same
+// table size rule, hash (key*107 & (size-1)) and linear probing, so slot
+// order and accumulation order are the same; the final sort compares keys
+// only (sort_less, mtSpGEMM.h:205-209).
+//
+// Pinning (what fixes each function independently of this code):
+//   oracle/_ref (the reference headers compiled as they lie) on generated
+//   inputs for every semiring and both modes; brute-force dense products on
+//   tiny inputs; the closed form nnz(C) = sum_j nnz_j; flop_j closed form;
+//   see tests/test_oracle.py.
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "../include/cbx.h"
+
+namespace orc {
+
+// ------------------------------------------------------------ semirings
+// Semirings.h:212-232 PlusTimesSRing, :235-255 MinPlusSRing (with inf_plus
+// :40-47), :165-190 SelectMaxSRing; bool PlusTimes through promote_trait.
+template <class T>
+struct PlusTimes {
+  static T add(const T& a, const T& b) { return a + b; }
+  static T multiply(const T& a, const T& b) { return a * b; }
+};
+template <>
+struct PlusTimes<uint8_t> {  // bool + bool -> OR, bool * bool -> AND
+  static uint8_t add(const uint8_t& a, const uint8_t& b) { return (uint8_t)((a + b) != 0); }
+  static uint8_t multiply(const uint8_t& a, const uint8_t& b) { return (uint8_t)((a != 0) && (b != 0)); }
+};
+template <>
+struct PlusTimes<int64_t> {  // wrap like the reference's int64 arithmetic on x86
+  static int64_t add(const int64_t& a, const int64_t& b) { return (int64_t)((uint64_t)a + (uint64_t)b); }
+  static int64_t multiply(const int64_t& a, const int64_t& b) { return (int64_t)((uint64_t)a * (uint64_t)b); }
+};
+template <class T>
+struct MinPlus {
+  static T add(const T& a, const T& b) { return std::min(a, b); }
+  static T multiply(const T& a, const T& b) {
+    const T inf = std::numeric_limits<T>::max();
+    if (a == inf || b == inf) return inf;
+    return a + b;
+  }
+};
+template <>
+struct MinPlus<int64_t> {
+  static int64_t add(const int64_t& a, const int64_t& b) { return std::min(a, b); }
+  static int64_t multiply(const int64_t& a, const int64_t& b) {
+    const int64_t inf = std::numeric_limits<int64_t>::max();
+    if (a == inf || b == inf) return inf;
+    return (int64_t)((uint64_t)a + (uint64_t)b);
+  }
+};
+template <class T>
+struct SelectMax {
+  static T add(const T& a, const T& b) { return std::max(a, b); }
+  static T multiply(const T& a, const T& b) { return a * b; }
+};
+template <>
+struct SelectMax<int64_t> {
+  static int64_t add(const int64_t& a, const int64_t& b) { return std::max(a, b); }
+  static int64_t multiply(const int64_t& a, const int64_t& b) { return (int64_t)((uint64_t)a * (uint64_t)b); }
+};
+
+// ------------------------------------------------------------ A lookup
+// Dcsc::FillColInds (dcsc.cpp:1361-1423) returns, for each row index k of a
+// B column, the range [cp[p], cp[p+1]) of A's stored column p with jc[p]==k,
+// or an empty range.  A dense map from column id to p gives the same ranges.
+struct ColMap {
+  std::vector<int64_t> pos;
+  explicit ColMap(const cbx_dcsc& A) : pos(A.n, -1) {
+    for (int64_t p = 0; p < A.nzc; ++p) pos[A.jc[p]] = p;
+  }
+  inline std::pair<int64_t, int64_t> range(const cbx_dcsc& A, int64_t k) const {
+    const int64_t p = pos[k];
+    if (p < 0) return {0, 0};
+    return {A.cp[p], A.cp[p + 1]};
+  }
+};
+
+// estimateFLOP (mtSpGEMM.h:1057-1134) for B's stored columns [c0, c1).
+static void estimate_flop(const cbx_dcsc& A, const cbx_dcsc& B, const ColMap& cm, int64_t c0, int64_t c1,
+                          int64_t* flop) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = c0; i < c1; ++i) {
+    int64_t f = 0;
+    for (int64_t e = B.cp[i]; e < B.cp[i + 1]; ++e) {
+      auto r = cm.range(A, B.ir[e]);
+      f += r.second - r.first;
+    }
+    flop[i - c0] = f;
+  }
+}
+
+// estimateNNZ_Hash (mtSpGEMM.h:806-933): linear-probing table of
+// 2^k >= max(16, flop_i) slots, hash = key*107 & (size-1), -1 marks empty.
+static void estimate_nnz_hash(const cbx_dcsc& A, const cbx_dcsc& B, const ColMap& cm, int64_t c0, int64_t c1,
+                              const int64_t* flop, int64_t* nnz) {
+#pragma omp parallel
+  {
+    std::vector<int64_t> table;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = c0; i < c1; ++i) {
+      int64_t ht = 16;
+      while (ht < flop[i - c0]) ht <<= 1;
+      if ((int64_t)table.size() < ht) table.resize(ht);
+      std::fill(table.begin(), table.begin() + ht, (int64_t)-1);
+      int64_t cnt = 0;
+      for (int64_t e = B.cp[i]; e < B.cp[i + 1]; ++e) {
+        auto r = cm.range(A, B.ir[e]);
+        for (int64_t k = r.first; k < r.second; ++k) {
+          const int64_t key = A.ir[k];
+          int64_t h = (key * 107) & (ht - 1);
+          while (true) {
+            if (table[h] == key) break;
+            if (table[h] == -1) {
+              table[h] = key;
+              ++cnt;
+              break;
+            }
+            h = (h + 1) & (ht - 1);
+          }
+        }
+      }
+      nnz[i - c0] = cnt;
+    }
+  }
+}
+
+// HeapEntry (HeapEntry.h:36-56): ordering is reversed so that the standard
+// max-heap algorithms keep the smallest row on top.
+template <class T>
+struct HeapItem {
+  int64_t key;
+  int64_t runr;
+  T num;
+};
+template <class T>
+static inline bool heap_less(const HeapItem<T>& a, const HeapItem<T>& b) {
+  return a.key > b.key;
+}
+
+struct Entry {
+  int64_t row;
+};
+
+// One output column of LocalHybridSpGEMM (mtSpGEMM.h:289-441) or of
+// LocalSpGEMMHash (mtSpGEMM.h:556-646), written to rows/vals at `out`.
+template <class SR, class T>
+static void numeric_column(const cbx_dcsc& A, const cbx_dcsc& B, const ColMap& cm, int64_t i, int64_t flop_i,
+                           int64_t nnz_i, bool allow_heap, bool sort, int64_t* rows, T* vals,
+                           std::vector<std::pair<int64_t, int64_t>>& colinds, std::vector<HeapItem<T>>& heap,
+                           std::vector<std::pair<int64_t, T>>& table) {
+  const T* Av = (const T*)A.val;
+  const T* Bv = (const T*)B.val;
+  const int64_t b0 = B.cp[i];
+  const int64_t nb = B.cp[i + 1] - b0;
+  if ((int64_t)colinds.size() < nb) colinds.resize(nb);
+  for (int64_t j = 0; j < nb; ++j) colinds[j] = cm.range(A, B.ir[b0 + j]);
+  const double cr = (double)flop_i / (double)nnz_i;
+  int64_t cur = 0;
+  if (allow_heap && cr < 2.0) {  // heap merge, mtSpGEMM.h:311-360
+    if ((int64_t)heap.size() < nb) heap.resize(nb);
+    int64_t hsize = 0;
+    for (int64_t j = 0; j < nb; ++j)
+      if (colinds[j].first != colinds[j].second)
+        heap[hsize++] = HeapItem<T>{A.ir[colinds[j].first], j, Av[colinds[j].first]};
+    std::make_heap(heap.data(), heap.data() + hsize, heap_less<T>);
+    while (hsize > 0) {
+      std::pop_heap(heap.data(), heap.data() + hsize, heap_less<T>);
+      HeapItem<T>& top = heap[hsize - 1];
+      const int64_t locb = top.runr;
+      const T mrhs = SR::multiply(top.num, Bv[b0 + locb]);
+      if (cur > 0 && rows[cur - 1] == top.key) {
+        vals[cur - 1] = SR::add(vals[cur - 1], mrhs);
+      } else {
+        rows[cur] = top.key;
+        vals[cur] = mrhs;
+        ++cur;
+      }
+      if (++colinds[locb].first != colinds[locb].second) {
+        top.key = A.ir[colinds[locb].first];
+        top.num = Av[colinds[locb].first];
+        std::push_heap(heap.data(), heap.data() + hsize, heap_less<T>);
+      } else {
+        --hsize;
+      }
+    }
+    return;
+  }
+  // hash, mtSpGEMM.h:362-440 (and :556-646 for LocalSpGEMMHash)
+  int64_t ht = 16;
+  while (ht < nnz_i) ht <<= 1;
+  if ((int64_t)table.size() < ht) table.resize(ht);
+  for (int64_t s = 0; s < ht; ++s) table[s].first = -1;
+  for (int64_t j = 0; j < nb; ++j) {
+    const T bval = Bv[b0 + j];
+    for (int64_t k = colinds[j].first; k < colinds[j].second; ++k) {
+      const T mrhs = SR::multiply(Av[k], bval);
+      const int64_t key = A.ir[k];
+      int64_t h = (key * 107) & (ht - 1);
+      while (true) {
+        if (table[h].first == key) {
+          table[h].second = SR::add(mrhs, table[h].second);
+          break;
+        } else if (table[h].first == -1) {
+          table[h].first = key;
+          table[h].second = mrhs;
+          break;
+        }
+        h = (h + 1) & (ht - 1);
+      }
+    }
+  }
+  if (sort) {
+    int64_t w = 0;
+    for (int64_t s = 0; s < ht; ++s)
+      if (table[s].first != -1) table[w++] = table[s];
+    std::sort(table.begin(), table.begin() + w,
+              [](const std::pair<int64_t, T>& a, const std::pair<int64_t, T>& b) { return a.first < b.first; });
+    for (int64_t s = 0; s < w; ++s) {
+      rows[cur] = table[s].first;
+      vals[cur] = table[s].second;
+      ++cur;
+    }
+  } else {
+    for (int64_t s = 0; s < ht; ++s)
+      if (table[s].first != -1) {
+        rows[cur] = table[s].first;
+        vals[cur] = table[s].second;
+        ++cur;
+      }
+  }
+}
+
+// The whole local product restricted to B's stored columns [c0, c1), as
+// LocalHybridSpGEMM (allow_heap=true, sort=true) or LocalSpGEMMHash
+// (allow_heap=false).  Output: host DCSC with the non-empty columns, like
+// SpDCCols(const SpTuples&, false) (SpDCCols.cpp:109-184).
+template <class SR, class T>
+static cbx_status spgemm_cols(const cbx_dcsc& A, const cbx_dcsc& B, int64_t c0, int64_t c1, bool allow_heap,
+                              bool sort, cbx_dcsc* C, int64_t* flops_out) {
+  std::memset(C, 0, sizeof(*C));
+  C->m = A.m;
+  C->n = B.n;
+  C->dtype = A.dtype;
+  C->location = CBX_HOST;
+  const int64_t nc = c1 - c0;
+  ColMap cm(A);
+  std::vector<int64_t> flop(nc), nnz(nc), ptr(nc + 1, 0);
+  estimate_flop(A, B, cm, c0, c1, flop.data());
+  estimate_nnz_hash(A, B, cm, c0, c1, flop.data(), nnz.data());
+  int64_t total_flop = 0;
+  for (int64_t i = 0; i < nc; ++i) {
+    ptr[i + 1] = ptr[i] + nnz[i];
+    total_flop += flop[i];
+  }
+  if (flops_out) *flops_out = total_flop;
+  const int64_t nnzC = ptr[nc];
+  int64_t nzc = 0;
+  for (int64_t i = 0; i < nc; ++i) nzc += nnz[i] > 0;
+  C->nnz = nnzC;
+  C->nzc = nzc;
+  C->cp = (int64_t*)std::malloc(sizeof(int64_t) * (nzc + 1));
+  C->jc = (int64_t*)std::malloc(sizeof(int64_t) * std::max<int64_t>(nzc, 1));
+  C->ir = (int64_t*)std::malloc(sizeof(int64_t) * std::max<int64_t>(nnzC, 1));
+  C->val = std::malloc(sizeof(T) * std::max<int64_t>(nnzC, 1));
+  if (!C->cp || !C->jc || !C->ir || !C->val) return CBX_ERR_NOMEM;
+  int64_t q = 0;
+  for (int64_t i = 0; i < nc; ++i)
+    if (nnz[i] > 0) {
+      C->jc[q] = B.jc[c0 + i];
+      C->cp[q] = ptr[i];
+      ++q;
+    }
+  C->cp[nzc] = nnzC;
+  T* vals = (T*)C->val;
+#pragma omp parallel
+  {
+    std::vector<std::pair<int64_t, int64_t>> colinds;
+    std::vector<HeapItem<T>> heap;
+    std::vector<std::pair<int64_t, T>> table;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < nc; ++i) {
+      if (nnz[i] == 0) continue;
+      numeric_column<SR, T>(A, B, cm, c0 + i, flop[i], nnz[i], allow_heap, sort, C->ir + ptr[i], vals + ptr[i],
+                            colinds, heap, table);
+    }
+  }
+  return CBX_OK;
+}
+
+template <template <class> class SRT>
+static cbx_status dispatch_dtype(const cbx_dcsc& A, const cbx_dcsc& B, int64_t c0, int64_t c1, bool heap, bool sort,
+                                 cbx_dcsc* C, int64_t* flops) {
+  switch (A.dtype) {
+    case CBX_I64: return spgemm_cols<SRT<int64_t>, int64_t>(A, B, c0, c1, heap, sort, C, flops);
+    case CBX_F64: return spgemm_cols<SRT<double>, double>(A, B, c0, c1, heap, sort, C, flops);
+    case CBX_F32: return spgemm_cols<SRT<float>, float>(A, B, c0, c1, heap, sort, C, flops);
+    default: return CBX_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// LocalHybridSpGEMM (mode 0) / LocalSpGEMMHash sorted (mode 1) / unsorted
+// (mode 2) on B's stored columns [c0, c1) (c1 < 0 means all).
+cbx_status orc_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semiring, int32_t mode, int64_t c0, int64_t c1,
+                      cbx_dcsc* C, int64_t* flops) {
+  if (!A || !B || !C || A->n != B->m || A->dtype != B->dtype) return CBX_ERR_INVALID;
+  if (c1 < 0 || c1 > B->nzc) c1 = B->nzc;
+  if (c0 < 0 || c0 > c1) return CBX_ERR_INVALID;
+  const bool heap = mode == 0;
+  const bool sort = mode != 2;
+  switch (semiring) {
+    case CBX_SR_PLUS_TIMES:
+      if (A->dtype == CBX_BOOL)
+        return spgemm_cols<PlusTimes<uint8_t>, uint8_t>(*A, *B, c0, c1, heap, sort, C, flops);
+      return dispatch_dtype<PlusTimes>(*A, *B, c0, c1, heap, sort, C, flops);
+    case CBX_SR_MIN_PLUS: return dispatch_dtype<MinPlus>(*A, *B, c0, c1, heap, sort, C, flops);
+    case CBX_SR_SELECT_MAX: return dispatch_dtype<SelectMax>(*A, *B, c0, c1, heap, sort, C, flops);
+  }
+  return CBX_ERR_UNSUPPORTED;
+}
+
+// estimateFLOP and estimateNNZ_Hash for every stored column of B.
+cbx_status orc_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop, int64_t* nnz) {
+  if (!A || !B || A->n != B->m) return CBX_ERR_INVALID;
+  ColMap cm(*A);
+  estimate_flop(*A, *B, cm, 0, B->nzc, flop);
+  if (nnz) estimate_nnz_hash(*A, *B, cm, 0, B->nzc, flop, nnz);
+  return CBX_OK;
+}
+
+void orc_free(cbx_dcsc* C) {
+  if (!C) return;
+  std::free(C->cp);
+  std::free(C->jc);
+  std::free(C->ir);
+  std::free(C->val);
+  std::memset(C, 0, sizeof(*C));
+}
+
+int32_t orc_max_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int32_t n) { omp_set_num_threads(n); }
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// ORACLE -- test infrastructure only.
+//
+// A thin C entry layer over the genuine CombBLAS headers, compiled from the
+// reference tree where it lies (see oracle/Makefile) into
+// oracle/_ref/libcbref.so.  It exists to pin the restatement in
+// oracle/cbx_oracle.cpp and to write golden fixtures; it is never part of
+// the product and never runs on a machine without the reference tree at
+// build time.
+#include <mpi.h>
+
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "CombBLAS/CombBLAS.h"
+#include "../include/cbx.h"
+
+// Globals the CombBLAS headers declare extern (CombBLAS.h:76-102).
+int cblas_splits = 1;
+double cblas_alltoalltime, cblas_allgathertime, cblas_localspmvtime, cblas_mergeconttime, cblas_transvectime;
+double mcl_Abcasttime, mcl_Bbcasttime, mcl_localspgemmtime, mcl_multiwaymergetime, mcl_kselecttime,
+    mcl_prunecolumntime, mcl_symbolictime;
+double mcl3d_conversiontime, mcl3d_symbolictime, mcl3d_Abcasttime, mcl3d_Bbcasttime, mcl3d_SUMMAtime,
+    mcl3d_localspgemmtime, mcl3d_SUMMAmergetime, mcl3d_reductiontime, mcl3d_3dmergetime, mcl3d_kselecttime;
+
+using namespace combblas;
+
+namespace {
+
+void ensure_mpi() {
+  int inited = 0;
+  MPI_Initialized(&inited);
+  if (!inited) {
+    int provided;
+    MPI_Init_thread(nullptr, nullptr, MPI_THREAD_SERIALIZED, &provided);
+  }
+}
+
+template <class NT, class ST>
+SpDCCols<int64_t, NT>* make_mat(const cbx_dcsc& X) {
+  auto* M = new SpDCCols<int64_t, NT>(X.nnz, X.m, X.n, X.nzc);
+  if (X.nnz == 0) return M;
+  Dcsc<int64_t, NT>* d = M->GetDCSC();
+  std::memcpy(d->cp, X.cp, sizeof(int64_t) * (X.nzc + 1));
+  std::memcpy(d->jc, X.jc, sizeof(int64_t) * X.nzc);
+  std::memcpy(d->ir, X.ir, sizeof(int64_t) * X.nnz);
+  const ST* v = (const ST*)X.val;
+  for (int64_t i = 0; i < X.nnz; ++i) d->numx[i] = (NT)v[i];
+  return M;
+}
+
+template <class NT, class ST>
+void export_mat(const SpTuples<int64_t, NT>& T, int64_t m, int64_t n, cbx_dtype dt, cbx_dcsc* C) {
+  std::memset(C, 0, sizeof(*C));
+  C->m = m;
+  C->n = n;
+  C->dtype = dt;
+  C->location = CBX_HOST;
+  const int64_t nnz = T.getnnz();
+  int64_t nzc = 0;
+  for (int64_t i = 0; i < nnz; ++i)
+    if (i == 0 || T.colindex(i) != T.colindex(i - 1)) ++nzc;
+  C->nnz = nnz;
+  C->nzc = nzc;
+  C->cp = (int64_t*)std::malloc(sizeof(int64_t) * (nzc + 1));
+  C->jc = (int64_t*)std::malloc(sizeof(int64_t) * (nzc ? nzc : 1));
+  C->ir = (int64_t*)std::malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+  ST* v = (ST*)std::malloc(sizeof(ST) * (nnz ? nnz : 1));
+  C->val = v;
+  int64_t c = 0;
+  for (int64_t i = 0; i < nnz; ++i) {
+    if (i == 0 || T.colindex(i) != T.colindex(i - 1)) {
+      C->jc[c] = T.colindex(i);
+      C->cp[c] = i;
+      ++c;
+    }
+    C->ir[i] = T.rowindex(i);
+    v[i] = (ST)T.numvalue(i);
+  }
+  C->cp[nzc] = nnz;
+}
+
+template <class SR, class NT, class ST>
+int run(const cbx_dcsc* A, const cbx_dcsc* B, int mode, cbx_dcsc* C) {
+  ensure_mpi();
+  auto* MA = make_mat<NT, ST>(*A);
+  auto* MB = make_mat<NT, ST>(*B);
+  SpTuples<int64_t, NT>* T;
+  if (mode == 0)
+    T = LocalHybridSpGEMM<SR, NT>(*MA, *MB, false, false);
+  else
+    T = LocalSpGEMMHash<SR, NT>(*MA, *MB, false, false, mode == 1);
+  export_mat<NT, ST>(*T, A->m, B->n, A->dtype, C);
+  delete T;
+  delete MA;
+  delete MB;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// mode 0: LocalHybridSpGEMM; 1: LocalSpGEMMHash(sort=true); 2: sort=false.
+int ref_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semiring, int32_t mode, cbx_dcsc* C) {
+  switch (semiring) {
+    case CBX_SR_PLUS_TIMES:
+      switch (A->dtype) {
+        case CBX_I64: return run<PlusTimesSRing<int64_t, int64_t>, int64_t, int64_t>(A, B, mode, C);
+        case CBX_F64: return run<PlusTimesSRing<double, double>, double, double>(A, B, mode, C);
+        case CBX_F32: return run<PlusTimesSRing<float, float>, float, float>(A, B, mode, C);
+        case CBX_BOOL: return run<PlusTimesSRing<bool, bool>, bool, uint8_t>(A, B, mode, C);
+      }
+      break;
+    case CBX_SR_MIN_PLUS:
+      switch (A->dtype) {
+        case CBX_I64: return run<MinPlusSRing<int64_t, int64_t>, int64_t, int64_t>(A, B, mode, C);
+        case CBX_F64: return run<MinPlusSRing<double, double>, double, double>(A, B, mode, C);
+        case CBX_F32: return run<MinPlusSRing<float, float>, float, float>(A, B, mode, C);
+      }
+      break;
+    case CBX_SR_SELECT_MAX:
+      switch (A->dtype) {
+        case CBX_I64: return run<SelectMaxSRing<int64_t, int64_t>, int64_t, int64_t>(A, B, mode, C);
+        case CBX_F64: return run<SelectMaxSRing<double, double>, double, double>(A, B, mode, C);
+        case CBX_F32: return run<SelectMaxSRing<float, float>, float, float>(A, B, mode, C);
+      }
+      break;
+  }
+  return -1;
+}
+
+// estimateFLOP + estimateNNZ_Hash of the reference.
+int ref_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop, int64_t* nnz) {
+  ensure_mpi();
+  auto* MA = make_mat<int64_t, int64_t>(*A);
+  auto* MB = make_mat<int64_t, int64_t>(*B);
+  int64_t* f = estimateFLOP(*MA, *MB);
+  int64_t* z = estimateNNZ_Hash(*MA, *MB, f);
+  std::memcpy(flop, f, sizeof(int64_t) * B->nzc);
+  std::memcpy(nnz, z, sizeof(int64_t) * B->nzc);
+  delete[] f;
+  delete[] z;
+  delete MA;
+  delete MB;
+  return 0;
+}
+
+void ref_free(cbx_dcsc* C) {
+  std::free(C->cp);
+  std::free(C->jc);
+  std::free(C->ir);
+  std::free(C->val);
+  std::memset(C, 0, sizeof(*C));
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+"""The C ABI library loads, exports every symbol include/cbx.h declares, and
+rejects bad arguments before touching a device (CPU-only checks)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import combblas_amd as cb
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "cbx.h"
+
+
+def declared_functions() -> list[str]:
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(cbx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = cb.lib()
+    names = declared_functions()
+    assert len(names) >= 8
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, f"missing exports: {missing}"
+
+
+def test_device_count_never_fails():
+    assert cb.device_count() >= 0
+
+
+def test_null_and_host_arguments_rejected():
+    L = cb.lib()
+    out = cb.cbx_dcsc()
+    assert L.cbx_spgemm(None, None, 0, 0, C.byref(out), None, None) == cb.ERR_INVALID
+    A = cb.gen_rmat(6, 4)
+    s, keep = A._cstruct()  # host-resident struct: cbx_spgemm wants device memory
+    assert L.cbx_spgemm(C.byref(s), C.byref(s), 0, 0, C.byref(out), None, None) == cb.ERR_INVALID
+    assert b"device" in L.cbx_last_error()
+
+
+def test_generator_arguments_checked():
+    with pytest.raises(cb.CbxError):
+        cb.gen_rmat(0, 4)
+    with pytest.raises(cb.CbxError):
+        cb.gen_rmat(5, 4, a=0.9, b=0.2, c=0.2)
+
+
+def test_generator_rmat_shape():
+    A = cb.gen_rmat(12, 8)
+    assert A.m == A.n == 4096
+    assert np.all(np.diff(A.cp) > 0) and np.all(np.diff(A.jc) > 0)
+    cols = A.col_ids()
+    key = cols * A.m + A.ir
+    assert np.all(np.diff(key) > 0), "rows strictly increasing within columns"
+    assert A.val.min() >= 1

@@ -1,0 +1,178 @@
+"""GPU parity of cbx_spgemm (through the C ABI in libcbx.so) against the
+oracle and the golden fixtures written from the genuine reference.
+
+Integer, boolean, min-plus and select-max results must be bit-exact; FP64
+and FP32 plus-times must lie within the a-priori bound
+    |c_hat - c| <= gamma_k(u) * (|A| |B|)_ij,   gamma_k = k u / (1 - k u),
+with k the column's flop count (the length of the longest possible sum),
+u = 2^-53 for FP64 and 2^-24 for FP32 (FP32 compared against an FP64 oracle
+of the same FP32 inputs).  For positive inputs this is the north star's
+"1e-12 relative per entry" for FP64 at every k below 4.5e3.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import combblas_amd as cb
+import oracle
+from tests.helpers import bit_exact, random_dcsc, same_pattern, sort_columns
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+SR_DTYPES = [("plus_times", "int64"), ("plus_times", "float64"), ("plus_times", "float32"),
+             ("plus_times", "bool"), ("min_plus", "int64"), ("min_plus", "float64"), ("min_plus", "float32"),
+             ("select_max", "int64"), ("select_max", "float64"), ("select_max", "float32")]
+
+
+def gpu_spgemm(A, B, semiring="plus_times", mode="two_phase"):
+    dA, dB = A.to_device(), B.to_device()
+    dC, stats = cb.spgemm(dA, dB, semiring, mode)
+    return dC.to_host(), stats
+
+
+def check_fp(got: cb.Dcsc, A: cb.Dcsc, B: cb.Dcsc) -> None:
+    """Pattern exact, values within gamma_k * (|A||B|) against an FP64 oracle."""
+    A64, B64 = A.astype("float64"), B.astype("float64")
+    want, _ = oracle.spgemm(A64, B64, "plus_times")
+    same_pattern(got, want)
+    absA = cb.Dcsc(A64.m, A64.n, A64.cp, A64.jc, A64.ir, np.abs(A64.val), "float64")
+    absB = cb.Dcsc(B64.m, B64.n, B64.cp, B64.jc, B64.ir, np.abs(B64.val), "float64")
+    mag, _ = oracle.spgemm(absA, absB, "plus_times")
+    same_pattern(mag, want)
+    k = np.repeat(_col_flops(A64, B64, want), np.diff(want.cp))
+    u = 2.0 ** -53 if got.dtype == "float64" else 2.0 ** -24
+    ku = (k + 1) * u
+    bound = ku / (1 - ku) * mag.val + 1e-300
+    err = np.abs(got.val.astype(np.float64) - want.val)
+    assert np.all(err <= bound), f"max err/bound = {np.max(err / bound):.3g}"
+
+
+def _col_flops(A, B, C):
+    flop, _ = oracle.symbolic(A, B)
+    # map B's stored columns to C's stored columns
+    pos = np.searchsorted(B.jc, C.jc)
+    return flop[pos]
+
+
+def signed(X: cb.Dcsc, rng) -> cb.Dcsc:
+    Y = X.astype(X.dtype)
+    if X.dtype.startswith("float"):
+        Y.val = rng.standard_normal(X.nnz).astype(Y.val.dtype)
+    elif X.dtype == "int64":
+        Y.val = rng.integers(-9, 10, X.nnz).astype(np.int64)
+    return Y
+
+
+def test_device_present():
+    assert cb.device_count() >= 1, "no GPU visible to libcbx.so"
+
+
+def test_c1_rmat_s14_ef8_matches_golden():
+    """Config 1: R-MAT s14 ef8, PlusTimes<int64>, two-phase, bit-exact."""
+    g = np.load(GOLDEN / "spgemm_rmat_s14_ef8_plus_times_int64.npz")
+    A = cb.gen_rmat(14, 8)
+    got, stats = gpu_spgemm(A, A)
+    assert stats["flops"] == int(g["flops"])
+    assert got.nnz == int(g["nnz_c"]) and stats["nnz"] == got.nnz
+    assert np.array_equal(got.jc, g["jc"])
+    assert np.array_equal(np.diff(got.cp), g["col_nnz"])
+    assert np.array_equal(oracle.column_digests(got), g["col_digest"])
+    want, _ = oracle.spgemm(A, A)
+    bit_exact(got, want)
+
+
+@pytest.mark.parametrize("semiring,dtype", SR_DTYPES)
+def test_semirings_rmat(semiring, dtype):
+    rng = np.random.default_rng(11)
+    A = signed(cb.gen_rmat(12, 8).astype(dtype), rng)
+    B = signed(cb.gen_rmat(12, 8, seed=5).astype(dtype), rng)
+    got, _ = gpu_spgemm(A, B, semiring)
+    if semiring == "plus_times" and dtype.startswith("float"):
+        check_fp(got, A, B)
+    else:
+        want, _ = oracle.spgemm(A, B, semiring)
+        bit_exact(got, want)
+
+
+@pytest.mark.parametrize("name", sorted(p.name for p in GOLDEN.glob("spgemm_tiny_*.npz")))
+def test_tiny_golden(name):
+    g = np.load(GOLDEN / name)
+    semiring, dtype, mode = str(g["semiring"]), str(g["dtype"]), str(g["mode"])
+    A = cb.Dcsc(int(g["a_m"]), int(g["a_n"]), g["a_cp"], g["a_jc"], g["a_ir"], g["a_val"], dtype)
+    B = cb.Dcsc(int(g["b_m"]), int(g["b_n"]), g["b_cp"], g["b_jc"], g["b_ir"], g["b_val"], dtype)
+    want = cb.Dcsc(A.m, B.n, g["c_cp"], g["c_jc"], g["c_ir"], g["c_val"], dtype)
+    got, _ = gpu_spgemm(A, B, semiring, "two_phase" if mode == "hybrid" else "unsorted")
+    if mode != "hybrid":
+        got, want = sort_columns(got), sort_columns(want)
+    if semiring == "plus_times" and dtype.startswith("float"):
+        same_pattern(got, want)
+        check_fp(got, A, B)
+    else:
+        bit_exact(got, want)
+
+
+def test_dense_tier_multiple_row_tiles():
+    """s16: 65536 rows = 4 row tiles of the dense tier, heavy columns."""
+    A = cb.gen_rmat(16, 8)
+    got, stats = gpu_spgemm(A, A)
+    want, fl = oracle.spgemm(A, A)
+    assert stats["flops"] == fl
+    bit_exact(got, want)
+
+
+def test_tall_matrix_multiple_symbolic_passes():
+    """3M rows: 184 numeric row tiles and 3 symbolic bitmap passes."""
+    rng = np.random.default_rng(3)
+    m, k, n = 3_000_000, 400, 150
+    per_col = 160
+    rows = np.concatenate([np.sort(rng.choice(m, per_col, replace=False)) for _ in range(k)])
+    cols = np.repeat(np.arange(k), per_col)
+    A = cb.Dcsc.from_coo(m, k, rows, cols, rng.integers(1, 10, rows.size))
+    B = random_dcsc(rng, k, n, 0.12)
+    got, _ = gpu_spgemm(A, B)
+    want, _ = oracle.spgemm(A, B)
+    bit_exact(got, want)
+
+
+def test_unsorted_mode_same_entries():
+    A = cb.gen_rmat(13, 8)
+    got, _ = gpu_spgemm(A, A, mode="unsorted")
+    want, _ = oracle.spgemm(A, A)
+    bit_exact(sort_columns(got), want)
+
+
+def test_empty_and_ragged():
+    rng = np.random.default_rng(5)
+    # A with empty columns; B touching only some of them -> some empty output columns
+    A = cb.Dcsc.from_coo(50, 40, [1, 3, 7], [2, 2, 9], [1, 2, 3])
+    B = random_dcsc(rng, 40, 30, 0.2)
+    got, _ = gpu_spgemm(A, B)
+    want, _ = oracle.spgemm(A, B)
+    bit_exact(got, want)
+    Z = cb.Dcsc.from_coo(50, 40, [], [], [])
+    got, stats = gpu_spgemm(Z, B)
+    assert got.nnz == 0 and got.nzc == 0 and stats["nnz"] == 0
+
+
+def test_explicit_zeros_kept():
+    A = cb.Dcsc.from_coo(2, 2, [0, 0], [0, 1], [1, -1])
+    B = cb.Dcsc.from_coo(2, 1, [0, 1], [0, 0], [1, 1])
+    got, _ = gpu_spgemm(A, B)
+    assert got.nnz == 1 and got.ir[0] == 0 and got.val[0] == 0
+
+
+def test_invalid_input_rejected():
+    A = cb.gen_rmat(8, 4)
+    bad = cb.Dcsc(A.m, A.n, A.cp.copy(), A.jc.copy(), A.ir.copy(), A.val.copy())
+    c = int(np.argmax(np.diff(bad.cp) >= 2))
+    b0 = bad.cp[c]
+    bad.ir[b0], bad.ir[b0 + 1] = bad.ir[b0 + 1], bad.ir[b0]  # rows out of order
+    with pytest.raises(cb.CbxError) as ei:
+        gpu_spgemm(bad, A)
+    assert ei.value.status == cb.ERR_INVALID
+    with pytest.raises(cb.CbxError):
+        gpu_spgemm(A, cb.gen_rmat(9, 4))  # inner dimensions differ
