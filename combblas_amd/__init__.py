@@ -58,8 +58,9 @@ class cbx_spgemm_stats(C.Structure):
     _fields_ = [
         ("flops", C.c_int64), ("nnz", C.c_int64), ("nzc", C.c_int64),
         ("overflow_cols", C.c_int64), ("predicted_nnz", C.c_int64),
-        ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_numeric", C.c_double),
-        ("ms_total", C.c_double),
+        ("units_light", C.c_int64), ("units_heavy", C.c_int64),
+        ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_alloc", C.c_double),
+        ("ms_numeric", C.c_double), ("ms_main_kernel", C.c_double), ("ms_total", C.c_double),
     ]
 
 
@@ -77,6 +78,7 @@ def lib() -> C.CDLL:
         L.cbx_last_error.restype = C.c_char_p
         L.cbx_device_count.restype = C.c_int32
         L.cbx_device_synchronize.restype = C.c_int32
+        L.cbx_release_cached_memory.restype = None
         L.cbx_dcsc_free.argtypes = [P(cbx_dcsc)]
         L.cbx_dcsc_free.restype = None
         L.cbx_dcsc_to_device.argtypes = [P(cbx_dcsc), P(cbx_dcsc)]
@@ -102,6 +104,10 @@ def device_count() -> int:
 
 def synchronize() -> None:
     _check(lib().cbx_device_synchronize())
+
+
+def release_cached_memory() -> None:
+    lib().cbx_release_cached_memory()
 
 
 # ------------------------------------------------------------------ matrices

@@ -61,6 +61,36 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
   return base + inc - v;
 }
 
+__device__ __forceinline__ long long wave_incl_scan_ll(long long v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    long long t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// 64-bit exclusive prefix sum over a block of NT threads (wsum: NT/64 slots).
+template <int NT>
+__device__ __forceinline__ long long block_excl_scan_ll(long long v, long long* wsum, long long& total) {
+  constexpr int NW = NT / kWave;
+  const int lane = lane_id(), w = wave_id();
+  const long long inc = wave_incl_scan_ll(v);
+  if (lane == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    long long x = lane < NW ? wsum[lane] : 0;
+    x = wave_incl_scan_ll(x);
+    if (lane < NW) wsum[lane] = x;
+  }
+  __syncthreads();
+  const long long base = w ? wsum[w - 1] : 0;
+  total = wsum[NW - 1];
+  __syncthreads();
+  return base + inc - v;
+}
+
 // Sum over a block of NT threads (result valid in every thread).
 template <int NT>
 __device__ __forceinline__ long long block_sum_ll(long long v, long long* wsum) {

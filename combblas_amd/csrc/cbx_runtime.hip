@@ -2,9 +2,12 @@
 // checks and the cbx_spgemm front door.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 
 #include "cbx_device.h"
 #include "cbx_spgemm.h"
@@ -56,9 +59,82 @@ static cbx_status check_dcsc(const cbx_dcsc* X, const char* name, cbx_location l
   return CBX_OK;
 }
 
+// ------------------------------------------------------ caching allocator
+// Device arrays handed to callers (matrix copies, SpGEMM outputs) come from
+// a small caching allocator: cbx_dcsc_free returns blocks to a per-process
+// free list and the next allocation of a similar size reuses them, so
+// repeated products of the same shape do not pay hipMalloc/hipFree (which
+// costs seconds for outputs of hundreds of GB).
+namespace {
+struct Cache {
+  std::mutex mu;
+  std::unordered_map<void*, size_t> live;
+  std::multimap<size_t, void*> idle;
+  size_t idle_bytes = 0;
+};
+Cache& cache() {
+  static Cache* c = new Cache();  // never destroyed: outlives static teardown order
+  return *c;
+}
+}  // namespace
+
+cbx_status dev_alloc(void** p, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 256);
+  Cache& c = cache();
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.idle.lower_bound(bytes);
+    if (it != c.idle.end() && it->first <= bytes + bytes / 4 + (1u << 20)) {
+      *p = it->second;
+      c.live[*p] = it->first;
+      c.idle_bytes -= it->first;
+      c.idle.erase(it);
+      return CBX_OK;
+    }
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    release_cache();
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return set_error(CBX_ERR_NOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  std::lock_guard<std::mutex> g(c.mu);
+  c.live[*p] = bytes;
+  return CBX_OK;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  Cache& c = cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  auto it = c.live.find(p);
+  if (it == c.live.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  c.idle.emplace(it->second, p);
+  c.idle_bytes += it->second;
+  c.live.erase(it);
+}
+
+void release_cache() {
+  Cache& c = cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  for (auto& kv : c.idle) (void)hipFree(kv.second);
+  c.idle.clear();
+  c.idle_bytes = 0;
+}
+
 }  // namespace cbx
 
 using namespace cbx;
+
+extern "C" void cbx_release_cached_memory(void) { release_cache(); }
 
 extern "C" int32_t cbx_device_count(void) {
   int n = 0;
@@ -76,7 +152,7 @@ extern "C" void cbx_dcsc_free(cbx_dcsc* A) {
   void* arrs[4] = {A->cp, A->jc, A->ir, A->val};
   for (void* p : arrs) {
     if (!p) continue;
-    if (A->location == CBX_DEVICE) (void)hipFree(p);
+    if (A->location == CBX_DEVICE) dev_free(p);
     else std::free(p);
   }
   std::memset(A, 0, sizeof(*A));
@@ -99,9 +175,9 @@ static cbx_status copy_dcsc(const cbx_dcsc* src, cbx_dcsc* dst, cbx_location to)
   for (int i = 0; i < 4; ++i) {
     const size_t b = bytes[i] ? bytes[i] : 8;
     if (to == CBX_DEVICE) {
-      if (hipMalloc(d_arr[i], b) != hipSuccess) {
+      if (dev_alloc(d_arr[i], b) != CBX_OK) {
         cbx_dcsc_free(dst);
-        return set_error(CBX_ERR_NOMEM, "device allocation of %zu bytes failed", b);
+        return CBX_ERR_NOMEM;
       }
     } else {
       *d_arr[i] = std::malloc(b);
@@ -150,6 +226,8 @@ extern "C" cbx_status cbx_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semir
   CBX_RETURN_IF(A->n != B->m, CBX_ERR_INVALID, "cbx_spgemm: inner dimensions differ (A is %lld x %lld, B is %lld x %lld)",
                 (long long)A->m, (long long)A->n, (long long)B->m, (long long)B->n);
   CBX_RETURN_IF(A->dtype != B->dtype, CBX_ERR_UNSUPPORTED, "cbx_spgemm: A and B must share a dtype");
+  CBX_RETURN_IF(A->nnz >= (1ll << 31) || B->nnz >= (1ll << 31), CBX_ERR_LIMIT,
+                "cbx_spgemm: operands must hold fewer than 2^31 nonzeros each");
   CBX_RETURN_IF(mode != CBX_SPGEMM_TWO_PHASE && mode != CBX_SPGEMM_UNSORTED && mode != CBX_SPGEMM_LEARNED,
                 CBX_ERR_INVALID, "cbx_spgemm: unknown mode %d", (int)mode);
   CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED, CBX_ERR_UNSUPPORTED, "cbx_spgemm: learned mode is not available yet");

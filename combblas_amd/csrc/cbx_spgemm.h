@@ -13,7 +13,14 @@ struct SpgemmConfig {
   // row-tiled dense tier.  At most 4096 (the largest hash table holds 8192
   // slots and is kept at most half full).
   int64_t small_flop_max = 4096;
+  // Unit tier: columns whose output exceeds heavy_nnz use the large units.
+  int64_t heavy_nnz = 65536;
 };
+
+// Cached device allocations for arrays handed to callers (see cbx_runtime.hip).
+cbx_status dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
+void release_cache();
 
 cbx_status spgemm_dispatch(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring sr, cbx_spgemm_mode mode,
                            const SpgemmConfig& cfg, cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s);

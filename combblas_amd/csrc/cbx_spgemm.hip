@@ -480,6 +480,381 @@ __global__ __launch_bounds__(NT) void k_num_dense(const int32_t* __restrict__ or
   }
 }
 
+// ============================================================ unit tier
+// Row-range units.  The symbolic pass builds, per column, the full presence
+// bitmap of its output rows in LDS, counts the rows of every block of kBR
+// rows, and cuts the column into units: runs of consecutive blocks holding
+// at most KMAX outputs and spanning at most RBLK blocks.  The numeric pass
+// runs one workgroup per unit.  A unit's accumulator is compact: the value of
+// row r lives at rank(r) = number of output rows of the unit below r, read
+// from the unit's presence bitmap with per-word prefix counts, so the unit
+// is emitted in row order without sorting and LDS holds only real outputs.
+constexpr int kBR = 4096;        // rows per block (also the tile-pointer granularity)
+constexpr int kBRW = kBR / 32;   // 32-bit bitmap words per block
+
+struct Unit {
+  int32_t col;     // B stored-column position
+  int32_t blk_lo;  // first block
+  int32_t blk_hi;  // one past the last block
+  int32_t cnt;     // outputs in the unit
+  int32_t off;     // offset of the unit's first output within the column
+};
+
+// Per-chunk segment staging for block-level flattening of products.
+template <class T, int NT>
+struct SegStage {
+  int off[NT + 1];
+  int64_t base[NT];
+  T bval[NT];
+};
+
+// Segment of B entry e restricted to P tiles [t_lo, t_hi) (or the whole A
+// column when `full`).
+template <class T, bool VAL>
+__device__ __forceinline__ void load_seg(int64_t e, int64_t b1, const int32_t* __restrict__ bpos,
+                                         const int64_t* __restrict__ cpA, const int32_t* __restrict__ P, int S,
+                                         int t_lo, int t_hi, bool full, const T* __restrict__ valB, int64_t& base,
+                                         int& len, T& bv) {
+  base = 0;
+  len = 0;
+  bv = T{};
+  if (e < b1) {
+    const int32_t p = bpos[e];
+    if (p >= 0) {
+      if (full) {
+        base = cpA[p];
+        len = (int)(cpA[p + 1] - base);
+      } else {
+        const int32_t* pp = P + (int64_t)p * (S + 1);
+        const int a0 = pp[t_lo];
+        base = cpA[p] + a0;
+        len = pp[t_hi] - a0;
+      }
+      if constexpr (VAL) bv = valB[e];
+    }
+  }
+}
+
+// Stages one chunk (at most one segment per thread): the non-empty segments
+// are compacted in order with one packed 64-bit block scan (high half counts
+// segments, low half sums lengths), so staged offsets strictly increase.
+// Returns the number of products; `nseg` receives the number of segments.
+// Contains barriers.
+template <class T, int NT>
+__device__ __forceinline__ int stage_chunk(SegStage<T, NT>& st, long long* wsum, int64_t base, int len, T bv,
+                                           int& nseg) {
+  const long long packed = len > 0 ? ((1ll << 32) | (long long)(unsigned)len) : 0ll;
+  long long tot;
+  const long long ex = block_excl_scan_ll<NT>(packed, wsum, tot);
+  if (len > 0) {
+    const int k = (int)(ex >> 32);
+    st.off[k] = (int)(ex & 0xffffffffll);
+    st.base[k] = base;
+    st.bval[k] = bv;
+  }
+  nseg = (int)(tot >> 32);
+  const int total = (int)(tot & 0xffffffffll);
+  if (threadIdx.x == 0) st.off[nseg] = total;
+  __syncthreads();
+  return total;
+}
+
+// Visits all `tot` staged products.  Each thread takes U consecutive
+// products per round: one binary search for the first, then (segments being
+// non-empty) at most one step forward per product.  Loads are issued for all
+// U products before f runs.  f(idx, bv) gets idx = -1 for lanes past the end.
+// No barriers.
+template <class T, int NT, int U, class F>
+__device__ __forceinline__ void visit_staged(const SegStage<T, NT>& st, int nseg, int tot, F&& f) {
+  for (int xb = threadIdx.x * U; xb < tot; xb += NT * U) {
+    int lo = 0, hi = nseg;  // largest k with off[k] <= xb (off[nseg] = tot > xb)
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (st.off[mid] <= xb) lo = mid;
+      else hi = mid;
+    }
+    int k = lo;
+    int cur = st.off[k], nxt = st.off[k + 1];
+    int64_t idx[U];
+    T bvs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = xb + u;
+      idx[u] = -1;
+      if (x < tot) {
+        if (x >= nxt) {
+          ++k;
+          cur = nxt;
+          nxt = st.off[k + 1];
+        }
+        idx[u] = st.base[k] + (x - cur);
+        bvs[u] = st.bval[k];
+      }
+    }
+    f(idx, bvs);
+  }
+}
+
+// Symbolic pass of the unit tier: one workgroup per column; bitmap passes of
+// 2^20 rows.  Writes nnz_j and the column's units.
+template <int NT, int U>
+__global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ order, int64_t first,
+                                                  const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+                                                  const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+                                                  const int32_t* __restrict__ P, int S, int kmax_l, int rblk_l,
+                                                  int kmax_h, int rblk_h, int64_t heavy_thr,
+                                                  int64_t* __restrict__ nnzcol, Unit* __restrict__ units_l,
+                                                  int* __restrict__ n_l, int cap_l, Unit* __restrict__ units_h,
+                                                  int* __restrict__ n_h, int cap_h, int* __restrict__ err) {
+  constexpr int BMW = 32768;          // 2^20 rows per bitmap pass
+  constexpr int BPP = BMW / kBRW;     // 256 blocks per pass
+  __shared__ uint32_t bits[BMW];
+  __shared__ int bpre[BPP];           // inclusive prefix of block counts
+  __shared__ Unit su[BPP];
+  __shared__ SegStage<uint8_t, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  __shared__ int s_nu, s_slot;
+  const int col = order[first + blockIdx.x];
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  const int npass = (S + BPP - 1) / BPP;
+  int64_t colcnt = 0;
+  int unit_off = 0;  // meaningful in thread 0 only
+  for (int sp = 0; sp < npass; ++sp) {
+    const int t_lo = sp * BPP, t_hi = min(S, t_lo + BPP), nblk = t_hi - t_lo;
+    const int32_t row0 = t_lo * kBR;
+    for (int i = threadIdx.x; i < nblk * kBRW; i += NT) bits[i] = 0u;
+    // (the staging barrier below orders the clear before any insertion)
+    for (int64_t eb = b0; eb < b1; eb += NT) {
+      int64_t base;
+      int len;
+      uint8_t dummy;
+      load_seg<uint8_t, false>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, npass == 1, nullptr, base, len,
+                               dummy);
+      int nseg;
+      const int tot = stage_chunk<uint8_t, NT>(st, wsum, base, len, 0, nseg);
+      visit_staged<uint8_t, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const uint8_t(&)[U]) {
+        int32_t r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+      });
+    }
+    __syncthreads();
+    // per-block counts: one wave per block, two words per lane
+    for (int b = wave_id(); b < nblk; b += NT / kWave) {
+      const uint32_t* w = bits + b * kBRW;
+      int c = __popc(w[lane_id()]) + __popc(w[lane_id() + kWave]);
+      c = (int)wave_sum_ll(c);
+      if (lane_id() == 0) bpre[b] = c;
+    }
+    __syncthreads();
+    if (wave_id() == 0) {  // inclusive prefix over blocks
+      int carry = 0;
+      for (int b0i = 0; b0i < nblk; b0i += kWave) {
+        const int b = b0i + lane_id();
+        const int v = b < nblk ? bpre[b] : 0;
+        const int inc = wave_incl_scan(v) + carry;
+        if (b < nblk) bpre[b] = inc;
+        carry = __shfl(inc, kWave - 1, kWave);
+      }
+    }
+    __syncthreads();
+    const int pass_total = bpre[nblk - 1];
+    if (threadIdx.x == 0) {
+      // greedy cut into units (a single block never exceeds kBR <= kmax)
+      const bool heavy = pass_total > heavy_thr;
+      const int kmax = heavy ? kmax_h : kmax_l, rblk = heavy ? rblk_h : rblk_l;
+      int start = 0, before = 0, nu = 0;
+      while (start < nblk) {
+        const int lim = before + kmax;
+        const int endmax = min(nblk, start + rblk);
+        int lo = start, hi = endmax - 1;  // last b in [start, endmax) with bpre[b] <= lim
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (bpre[mid] <= lim) lo = mid;
+          else hi = mid - 1;
+        }
+        const int c = bpre[lo] - before;
+        if (c > 0) {
+          su[nu].col = col;
+          su[nu].blk_lo = t_lo + start;
+          su[nu].blk_hi = t_lo + lo + 1;
+          su[nu].cnt = c;
+          su[nu].off = unit_off;
+          unit_off += c;
+          ++nu;
+        }
+        before = bpre[lo];
+        start = lo + 1;
+      }
+      s_nu = nu;
+      int* cnt = heavy ? n_h : n_l;
+      s_slot = nu ? atomicAdd(cnt, nu) : 0;
+      if (s_slot + nu > (heavy ? cap_h : cap_l)) atomicOr(err, 32);
+    }
+    __syncthreads();
+    {
+      const bool heavy = pass_total > heavy_thr;
+      Unit* dst = heavy ? units_h : units_l;
+      const int cap = heavy ? cap_h : cap_l;
+      for (int i = threadIdx.x; i < s_nu; i += NT)
+        if (s_slot + i < cap) dst[s_slot + i] = su[i];
+    }
+    colcnt += pass_total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nnzcol[col] = colcnt;
+}
+
+// Numeric pass of the unit tier: one workgroup per unit.
+template <int K, class T, int NT, int U, int KMAX, int RBLK>
+__global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units, const int64_t* __restrict__ cpB,
+                                                  const int32_t* __restrict__ bpos, const T* __restrict__ valB,
+                                                  const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+                                                  const T* __restrict__ valA, const int32_t* __restrict__ P, int S,
+                                                  int64_t m, const int64_t* __restrict__ cpC,
+                                                  int64_t* __restrict__ irC, T* __restrict__ valC,
+                                                  int* __restrict__ err) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int NW32 = RBLK * kBRW;   // bitmap words
+  constexpr int NW64 = NW32 / 2;
+  constexpr int WPT = (NW64 + NT - 1) / NT;  // 64-bit words per thread in the prefix scan
+  __shared__ Acc vals[KMAX];
+  __shared__ uint32_t bits[NW32];
+  __shared__ uint16_t pre[NW64];
+  __shared__ SegStage<T, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  const Unit un = units[blockIdx.x];
+  const int col = un.col;
+  const int t_lo = un.blk_lo, t_hi = un.blk_hi;
+  const int32_t row0 = t_lo * kBR;
+  const int cnt = un.cnt;
+  const int64_t out = cpC[col] + un.off;
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  const bool single_chunk = (b1 - b0) <= NT;
+  if (cnt > KMAX || t_hi - t_lo > RBLK) {  // units are cut to fit; guard anyway
+    if (threadIdx.x == 0) atomicOr(err, 64);
+    return;
+  }
+  for (int i = threadIdx.x; i < cnt; i += NT) vals[i] = SR::ident();
+  const int nw32 = (t_hi - t_lo) * kBRW;
+  for (int i = threadIdx.x; i < nw32; i += NT) bits[i] = 0u;
+  // sweep 1: presence bitmap of the unit's rows
+  int tot = 0, nseg = 0;
+  for (int64_t eb = b0; eb < b1; eb += NT) {
+    int64_t base;
+    int len;
+    T bv;
+    load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+    tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+    visit_staged<T, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const T(&)[U]) {
+      int32_t r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+    });
+    if (!single_chunk) __syncthreads();
+  }
+  __syncthreads();
+  // exclusive prefix popcount per 64-bit word
+  const int nw64 = nw32 / 2;
+  {
+    int local[WPT];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+      const int w = threadIdx.x * WPT + q;
+      const int c = w < nw64 ? __popc(bits[2 * w]) + __popc(bits[2 * w + 1]) : 0;
+      local[q] = sum;
+      sum += c;
+    }
+    long long total;
+    const int ex = (int)block_excl_scan_ll<NT>(sum, wsum, total);
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+      const int w = threadIdx.x * WPT + q;
+      if (w < nw64) pre[w] = (uint16_t)(ex + local[q]);
+    }
+    if (threadIdx.x == 0 && total != cnt) atomicOr(err, 128);
+  }
+  __syncthreads();
+  // sweep 2: accumulate values at their ranks
+  auto accumulate = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
+    int32_t r[U];
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      r[u] = -1;
+      if (idx[u] >= 0) {
+        r[u] = rowsA[idx[u]] - row0;
+        a[u] = valA[idx[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r[u] < 0) continue;
+      const int w = r[u] >> 6, b = r[u] & 63;
+      const uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+      const int rank = pre[w] + __popcll(word & ((1ull << b) - 1ull));
+      if (rank < cnt) SR::acc(&vals[rank], SR::mul(a[u], bvs[u]));
+    }
+  };
+  if (single_chunk) {
+    visit_staged<T, NT, U>(st, nseg, tot, accumulate);
+  } else {
+    for (int64_t eb = b0; eb < b1; eb += NT) {
+      int64_t base;
+      int len;
+      T bv;
+      load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+      int ns2;
+      const int t2 = stage_chunk<T, NT>(st, wsum, base, len, bv, ns2);
+      visit_staged<T, NT, U>(st, ns2, t2, accumulate);
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  // emit: values in rank order (coalesced), rows from the bitmap
+  for (int i = threadIdx.x; i < cnt; i += NT) valC[out + i] = SR::out(vals[i]);
+  for (int w = threadIdx.x; w < nw64; w += NT) {
+    uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+    int q = pre[w];
+    while (word) {
+      const int b = __ffsll((unsigned long long)word) - 1;
+      word &= word - 1;
+      if (q < cnt) irC[out + q] = (int64_t)row0 + w * 64 + b;
+      ++q;
+    }
+  }
+}
+
+// Tile pointers at kBR granularity, one wave per stored A column:
+// P[p*(S+1)+t] = #entries of column p with row < t*kBR (binary search).
+__global__ void k_ptable_wave(const int64_t* __restrict__ cpA, int64_t nzcA, const int32_t* __restrict__ rowsA,
+                              int S, int32_t* __restrict__ P) {
+  const int64_t p = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  if (p >= nzcA) return;
+  const int64_t b = cpA[p];
+  const int len = (int)(cpA[p + 1] - b);
+  const int32_t* rows = rowsA + b;
+  int32_t* out = P + p * (int64_t)(S + 1);
+  for (int t = lane_id(); t <= S; t += kWave) {
+    const int64_t key = (int64_t)t * kBR;
+    int lo = 0, hi = len;  // first i with rows[i] >= key
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (rows[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    out[t] = lo;
+  }
+}
+
 // ------------------------------------------------------------------ driver
 namespace {
 
@@ -513,7 +888,7 @@ cbx_status dalloc(DevBuf& b, size_t count, hipStream_t s, X** out) {
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 struct Timer {
-  hipEvent_t ev[5];
+  hipEvent_t ev[6];
   bool on;
   explicit Timer(bool enable) : on(enable) {
     if (on)
@@ -538,6 +913,10 @@ constexpr int kDenseNT = 512;                 // dense numeric threads
 constexpr int kDenseTR = 32 * kDenseNT;       // rows per numeric tile (16384)
 constexpr int kSymNT = 512;
 constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass (2^20 rows)
+// unit tier
+constexpr int kSymUnitNT = 1024;
+constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 32;    // <= 4096 outputs, <= 128K rows
+constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
 
 }  // namespace
 
@@ -554,25 +933,31 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   C->n = n;
   C->dtype = A->dtype;
   C->location = CBX_DEVICE;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
 
   if (nnzA == 0 || nnzB == 0 || nzcA == 0 || nzcB == 0) {  // mtSpGEMM.h:224-227
-    CBX_HIP(hipMalloc(&C->cp, sizeof(int64_t)));
+    CBX_TRY(dev_alloc((void**)&C->cp, sizeof(int64_t)));
     CBX_HIP(hipMemsetAsync(C->cp, 0, sizeof(int64_t), s));
     CBX_HIP(hipStreamSynchronize(s));
-    if (stats) std::memset(stats, 0, sizeof(*stats));
     return CBX_OK;
   }
 
+  // Unit tier needs the tile-pointer table at kBR-row granularity; it is used
+  // when that table stays modest, otherwise the streaming dense tier runs.
+  const int S4 = (int)((m + kBR - 1) / kBR);
+  const bool use_units = m <= (1ll << 24) && (double)nzcA * (S4 + 1) * 4.0 <= 8.0e9;
   const int TR = kDenseTR;
-  const int S = (int)((m + TR - 1) / TR);
-  const int G = kSymRBW * 32 / TR;  // tiles per symbolic pass
+  const int S16 = (int)((m + TR - 1) / TR);
+  const int S = use_units ? S4 : S16;
+  const int G = kSymRBW * 32 / TR;  // tiles per symbolic pass of the streaming tier
 
   DevBuf b_colmap, b_rows, b_bpos, b_flop, b_order, b_flop_s, b_order_s, b_P, b_nnz, b_cp, b_tmp, b_thr, b_bounds,
-      b_flags, b_nsel, b_err;
+      b_flags, b_nsel, b_err, b_ul, b_uh, b_ucnt;
   int32_t *colmap, *rowsA, *bpos, *order, *order_s, *P;
-  int* err;
+  int *err, *ucnt;
   int64_t *flop, *flop_s, *nnzcol, *cpC, *thr, *bounds, *nsel;
   uint8_t* flags;
+  Unit *units_l = nullptr, *units_h = nullptr;
   CBX_TRY(dalloc(b_colmap, A->n, s, &colmap));
   CBX_TRY(dalloc(b_rows, nnzA, s, &rowsA));
   CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
@@ -588,7 +973,9 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_TRY(dalloc(b_flags, nzcB, s, &flags));
   CBX_TRY(dalloc(b_nsel, 2, s, &nsel));
   CBX_TRY(dalloc(b_err, 1, s, &err));
+  CBX_TRY(dalloc(b_ucnt, 2, s, &ucnt));
   CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  CBX_HIP(hipMemsetAsync(ucnt, 0, 2 * sizeof(int), s));
 
   const T* valA = (const T*)A->val;
   const T* valB = (const T*)B->val;
@@ -612,7 +999,10 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   k_rows32<<<nblk(nnzA, 256), 256, 0, s>>>(A->ir, nnzA, rowsA);
   k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
   k_flop<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, flop);
-  k_ptable<<<nblk(nzcA, 256), 256, 0, s>>>(A->cp, nzcA, rowsA, S, TR, P);
+  if (use_units)
+    k_ptable_wave<<<nblk(nzcA * kWave, 256), 256, 0, s>>>(A->cp, nzcA, rowsA, S, P);
+  else
+    k_ptable<<<nblk(nzcA, 256), 256, 0, s>>>(A->cp, nzcA, rowsA, S, TR, P);
   k_iota_i32<<<nblk(nzcB, 256), 256, 0, s>>>(order, nzcB);
 
   // sort columns by flop, descending (largest work first)
@@ -623,65 +1013,85 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   tmp_bytes = std::max(tmp_bytes, tmp2);
   CBX_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp2, B->jc, flags, (int64_t*)nullptr, nsel, (int)nzcB, s));
   tmp_bytes = std::max(tmp_bytes, tmp2);
+  CBX_HIP(hipcub::DeviceReduce::Sum(nullptr, tmp2, flop, bounds + 5, (int)nzcB, s));
+  tmp_bytes = std::max(tmp_bytes, tmp2);
   void* tmp;
   CBX_TRY(dalloc(b_tmp, tmp_bytes, s, (uint8_t**)&tmp));
   CBX_HIP(hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, flop, flop_s, order, order_s, (int)nzcB, 0, 64,
                                                        s));
+  CBX_HIP(hipcub::DeviceReduce::Sum(tmp, tmp_bytes, flop, bounds + 5, (int)nzcB, s));
 
-  // class thresholds: dense > small_max ; hash8k > 1024 ; hash2k > 128 ; hash256 > 0
-  int64_t hthr[4] = {cfg.small_flop_max, 1024, 128, 0};
+  // class thresholds: dense > small_max ; hash8k > 1024 ; hash2k > 128 ; hash256 > 0 ; heavy > heavy_nnz
+  int64_t hthr[5] = {cfg.small_flop_max, 1024, 128, 0, cfg.heavy_nnz};
   CBX_HIP(hipMemcpyAsync(thr, hthr, sizeof(hthr), hipMemcpyHostToDevice, s));
-  k_class_bounds<<<1, 64, 0, s>>>(flop_s, nzcB, thr, 4, bounds);
-  int64_t hb[4];
+  k_class_bounds<<<1, 64, 0, s>>>(flop_s, nzcB, thr, 5, bounds);
+  int64_t hb[6];
   CBX_HIP(hipMemcpyAsync(hb, bounds, sizeof(hb), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipMemsetAsync(nnzcol, 0, sizeof(int64_t) * (nzcB + 1), s));
   CBX_HIP(hipStreamSynchronize(s));
   const int64_t n_dense = hb[0], n_h8 = hb[1] - hb[0], n_h2 = hb[2] - hb[1], n_h256 = hb[3] - hb[2];
-  int64_t total_flop = 0;
+  const int64_t n_heavy_max = hb[4];  // columns whose flops (an upper bound of nnz) exceed heavy_nnz
+  const int64_t total_flop = hb[5];
   tm.mark(1, s);
 
   // --- symbolic
-  if (n_dense) k_sym_dense<kSymNT, kSymRBW><<<(unsigned)n_dense, kSymNT, 0, s>>>(order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, G, TR, nnzcol);
+  int64_t cap_l = 0, cap_h = 0;
+  if (use_units && n_dense) {
+    cap_l = std::max<int64_t>(1, std::min<int64_t>(n_dense * (int64_t)S, 1ll << 30));
+    cap_h = std::max<int64_t>(1, std::min<int64_t>(n_heavy_max * (int64_t)S, 1ll << 30));
+    CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
+    CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
+    k_sym_units<kSymUnitNT, 4><<<(unsigned)n_dense, kSymUnitNT, 0, s>>>(
+        order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.heavy_nnz, nnzcol,
+        units_l, ucnt, (int)cap_l, units_h, ucnt + 1, (int)cap_h, err);
+  } else if (n_dense) {
+    k_sym_dense<kSymNT, kSymRBW><<<(unsigned)n_dense, kSymNT, 0, s>>>(order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, G,
+                                                                       TR, nnzcol);
+  }
   if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
   if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
   if (n_h256) k_sym_hash<256, kHashNT><<<(unsigned)n_h256, kHashNT, 0, s>>>(order_s, hb[2], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
   int64_t nnzC = 0;
+  int hu[2] = {0, 0}, herr = 0;
   CBX_HIP(hipMemcpyAsync(&nnzC, cpC + nzcB, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  if (stats) {
-    // flop total: reuse the scan on the flop array is unnecessary; sum on host-side copy of sorted flops is costly,
-    // so compute with a device reduction into bounds[4].
-    size_t rb = 0;
-    CBX_HIP(hipcub::DeviceReduce::Sum(nullptr, rb, flop, bounds + 4, (int)nzcB, s));
-    if (rb <= tmp_bytes) {
-      CBX_HIP(hipcub::DeviceReduce::Sum(tmp, rb, flop, bounds + 4, (int)nzcB, s));
-      CBX_HIP(hipMemcpyAsync(&total_flop, bounds + 4, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    }
-  }
-  CBX_HIP(hipStreamSynchronize(s));
+  CBX_HIP(hipMemcpyAsync(hu, ucnt, sizeof(hu), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   tm.mark(2, s);
+  CBX_HIP(hipStreamSynchronize(s));
+  if (herr) return set_error(CBX_ERR_HIP, "cbx_spgemm: symbolic pass failed (flag %d)", herr);
 
-  // --- output allocation
-  CBX_HIP(hipMalloc(&C->cp, sizeof(int64_t) * (nzcB + 1)));
-  CBX_HIP(hipMalloc(&C->jc, sizeof(int64_t) * std::max<int64_t>(nzcB, 1)));
-  CBX_HIP(hipMalloc(&C->ir, sizeof(int64_t) * std::max<int64_t>(nnzC, 1)));
-  CBX_HIP(hipMalloc(&C->val, sizeof(T) * std::max<int64_t>(nnzC, 1)));
+  // --- output allocation (cached: repeated products reuse the arrays)
+  CBX_TRY(dev_alloc((void**)&C->cp, sizeof(int64_t) * (nzcB + 1)));
+  CBX_TRY(dev_alloc((void**)&C->jc, sizeof(int64_t) * std::max<int64_t>(nzcB, 1)));
+  CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * std::max<int64_t>(nnzC, 1)));
+  CBX_TRY(dev_alloc(&C->val, sizeof(T) * std::max<int64_t>(nnzC, 1)));
   C->nnz = nnzC;
+  tm.mark(3, s);
 
   // --- numeric
-  if (n_dense)
-    k_num_dense<K, T, kDenseNT><<<(unsigned)n_dense, kDenseNT, 0, s>>>(order_s, 0, B->cp, bpos, valB, A->cp, rowsA, valA, P,
-                                                                     S, cpC, C->ir, (T*)C->val, err);
+  if (use_units) {
+    if (hu[1])
+      k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+          units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, m, cpC, C->ir, (T*)C->val, err);
+    if (hu[0])
+      k_num_units<K, T, kLightNT, 4, kLightK, kLightRB><<<(unsigned)hu[0], kLightNT, 0, s>>>(
+          units_l, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, m, cpC, C->ir, (T*)C->val, err);
+  } else if (n_dense) {
+    k_num_dense<K, T, kDenseNT><<<(unsigned)n_dense, kDenseNT, 0, s>>>(order_s, 0, B->cp, bpos, valB, A->cp, rowsA,
+                                                                     valA, P, S, cpC, C->ir, (T*)C->val, err);
+  }
+  tm.mark(4, s);
   const bool sort = mode != CBX_SPGEMM_UNSORTED;
 #define CBX_LAUNCH_HASH(HM, NCOL, FIRST)                                                                          \
   if (NCOL) {                                                                                                     \
     if (sort)                                                                                                     \
       k_num_hash<K, T, HM, kHashNT, true><<<(unsigned)(NCOL), kHashNT, 0, s>>>(                                   \
-          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);                 \
+          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);           \
     else                                                                                                          \
       k_num_hash<K, T, HM, kHashNT, false><<<(unsigned)(NCOL), kHashNT, 0, s>>>(                                  \
-          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);                 \
+          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);           \
   }
   CBX_LAUNCH_HASH(8192, n_h8, hb[0]);
   CBX_LAUNCH_HASH(2048, n_h2, hb[1]);
@@ -695,25 +1105,27 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_HIP(hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, cpC, flags, C->cp, nsel + 1, (int)nzcB, s));
   k_set_last<<<1, 1, 0, s>>>(C->cp, nsel, cpC, nzcB);
   int64_t nzcC = 0;
-  int herr = 0;
   CBX_HIP(hipMemcpyAsync(&nzcC, nsel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-  tm.mark(3, s);
+  tm.mark(5, s);
   CBX_HIP(hipStreamSynchronize(s));
   C->nzc = nzcC;
   if (herr) {
     cbx_dcsc_free(C);
-    return set_error(CBX_ERR_HIP, "cbx_spgemm: internal hash table overflow (flag %d)", herr);
+    return set_error(CBX_ERR_HIP, "cbx_spgemm: numeric pass inconsistency (flag %d)", herr);
   }
   if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
     stats->flops = total_flop;
     stats->nnz = nnzC;
     stats->nzc = nzcC;
     stats->ms_setup = tm.ms(0, 1);
     stats->ms_symbolic = tm.ms(1, 2);
-    stats->ms_numeric = tm.ms(2, 3);
-    stats->ms_total = tm.ms(0, 3);
+    stats->ms_alloc = tm.ms(2, 3);
+    stats->ms_numeric = tm.ms(3, 5);
+    stats->ms_main_kernel = tm.ms(3, 4);
+    stats->ms_total = tm.ms(0, 5);
+    stats->units_light = hu[0];
+    stats->units_heavy = hu[1];
   }
   return CBX_OK;
 }

@@ -122,18 +122,27 @@ enum {
   CBX_SPGEMM_LEARNED = 2
 };
 
-/* Per-call counters and phase times (milliseconds, from device events). */
+/* Per-call counters and phase times (milliseconds, from device events on the
+ * call's stream). */
 typedef struct cbx_spgemm_stats {
   int64_t flops;          /* multiply-adds, sum over columns of estimateFLOP */
   int64_t nnz;            /* nnz of the product */
   int64_t nzc;            /* non-empty output columns */
   int64_t overflow_cols;  /* learned mode: columns re-run on the exact path */
   int64_t predicted_nnz;  /* learned mode: sum of the predicted column sizes */
-  double ms_setup;        /* A column map, flop count, binning */
+  int64_t units_light;    /* row-range units of the numeric pass (small/large) */
+  int64_t units_heavy;
+  double ms_setup;        /* validation, A column map, flop count, binning */
   double ms_symbolic;     /* symbolic pass (two-phase) or prediction (learned) */
-  double ms_numeric;      /* numeric pass including exact re-runs */
+  double ms_alloc;        /* output allocation */
+  double ms_numeric;      /* numeric pass including exact re-runs and DCSC assembly */
+  double ms_main_kernel;  /* the row-range-unit numeric kernels alone */
   double ms_total;
 } cbx_spgemm_stats;
+
+/* Return the device memory that freed matrices left in the library's cache
+ * (cbx_dcsc_free keeps device blocks for reuse by later calls). */
+void cbx_release_cached_memory(void);
 
 /* C = A*B on `semiring` (mtSpGEMM.h:213-217 LocalHybridSpGEMM<SR,NTO>(A, B,
  * clearA=false, clearB=false) and mtSpGEMM.h:463-467 LocalSpGEMMHash).
