@@ -58,7 +58,7 @@ class cbx_spgemm_stats(C.Structure):
     _fields_ = [
         ("flops", C.c_int64), ("nnz", C.c_int64), ("nzc", C.c_int64),
         ("overflow_cols", C.c_int64), ("predicted_nnz", C.c_int64),
-        ("units_light", C.c_int64), ("units_heavy", C.c_int64),
+        ("units_light", C.c_int64), ("units_heavy", C.c_int64), ("nnz_main", C.c_int64),
         ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_alloc", C.c_double),
         ("ms_numeric", C.c_double), ("ms_main_kernel", C.c_double), ("ms_total", C.c_double),
     ]
@@ -87,7 +87,12 @@ def lib() -> C.CDLL:
                                    C.c_uint32, P(cbx_dcsc)]
         L.cbx_spgemm.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, P(cbx_dcsc),
                                  P(cbx_spgemm_stats), C.c_void_p]
-        for name in ("cbx_dcsc_to_device", "cbx_dcsc_to_host", "cbx_gen_rmat", "cbx_spgemm"):
+        L.cbx_dcsc_check.argtypes = [P(cbx_dcsc)]
+        L.cbx_dcsc_structure_to_host.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p]
+        L.cbx_dcsc_structure_to_host.restype = C.c_int32
+        L.cbx_dcsc_column_checksums.argtypes = [P(cbx_dcsc), C.c_uint64, C.c_void_p]
+        for name in ("cbx_dcsc_to_device", "cbx_dcsc_to_host", "cbx_gen_rmat", "cbx_spgemm", "cbx_dcsc_check",
+                     "cbx_dcsc_column_checksums"):
             getattr(L, name).restype = C.c_int32
         _lib_handle = L
     return _lib_handle
@@ -236,6 +241,23 @@ class DeviceDcsc:
         out = cbx_dcsc()
         _check(lib().cbx_dcsc_to_host(C.byref(self.s), C.byref(out)))
         return Dcsc.from_struct(out, lib().cbx_dcsc_free)
+
+    def check(self) -> None:
+        """Raise CbxError unless the arrays form a valid DCSC (sorted rows)."""
+        _check(lib().cbx_dcsc_check(C.byref(self.s)))
+
+    def structure(self) -> tuple[np.ndarray, np.ndarray]:
+        """(cp, jc) copied to the host without the row/value arrays."""
+        cp = np.zeros(self.nzc + 1, np.int64)
+        jc = np.zeros(self.nzc, np.int64)
+        _check(lib().cbx_dcsc_structure_to_host(C.byref(self.s), cp.ctypes.data, jc.ctypes.data))
+        return cp, jc
+
+    def column_checksums(self, salt: int = 0) -> np.ndarray:
+        """uint64[nzc]: sum_i mix64(i ^ salt) * bits(X(i,j)) per stored column."""
+        out = np.zeros(self.nzc, np.uint64)
+        _check(lib().cbx_dcsc_column_checksums(C.byref(self.s), salt, out.ctypes.data))
+        return out
 
     def release(self) -> None:
         if self.s is not None and _lib_handle is not None:

@@ -170,6 +170,15 @@ __global__ void k_cp_finish(int64_t* nnzcol, int64_t nzcB) {
   if (threadIdx.x == 0 && blockIdx.x == 0) nnzcol[nzcB] = 0;
 }
 
+// sum of nnzcol over order[0..n) (outputs of the unit-tier columns)
+__global__ void k_sum_ordered(const int32_t* __restrict__ order, int64_t n, const int64_t* __restrict__ nnzcol,
+                              unsigned long long* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  long long v = i < n ? nnzcol[order[i]] : 0;
+  v = wave_sum_ll(v);
+  if (lane_id() == 0 && v) atomicAdd(out, (unsigned long long)v);
+}
+
 __global__ void k_nonempty_flags(const int64_t* __restrict__ nnzcol, int64_t n, uint8_t* __restrict__ flags) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) flags[i] = nnzcol[i] > 0;
@@ -1053,8 +1062,13 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (n_h256) k_sym_hash<256, kHashNT><<<(unsigned)n_h256, kHashNT, 0, s>>>(order_s, hb[2], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
-  int64_t nnzC = 0;
+  int64_t nnzC = 0, nnz_main = 0;
   int hu[2] = {0, 0}, herr = 0;
+  if (n_dense) {
+    CBX_HIP(hipMemsetAsync(bounds + 6, 0, sizeof(int64_t), s));
+    k_sum_ordered<<<nblk(n_dense, 256), 256, 0, s>>>(order_s, n_dense, nnzcol, (unsigned long long*)(bounds + 6));
+    CBX_HIP(hipMemcpyAsync(&nnz_main, bounds + 6, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  }
   CBX_HIP(hipMemcpyAsync(&nnzC, cpC + nzcB, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipMemcpyAsync(hu, ucnt, sizeof(hu), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1126,6 +1140,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     stats->ms_total = tm.ms(0, 5);
     stats->units_light = hu[0];
     stats->units_heavy = hu[1];
+    stats->nnz_main = nnz_main;
   }
   return CBX_OK;
 }

@@ -88,6 +88,20 @@ void cbx_dcsc_free(cbx_dcsc* A);
 cbx_status cbx_dcsc_to_device(const cbx_dcsc* host_src, cbx_dcsc* dev_dst);
 cbx_status cbx_dcsc_to_host(const cbx_dcsc* dev_src, cbx_dcsc* host_dst);
 
+/* Structural check of a device DCSC: cp consistent, jc strictly increasing
+ * and below n, rows strictly increasing within each column and below m.
+ * CBX_OK or CBX_ERR_INVALID.  (The preconditions of Dcsc, dcsc.h:46-136.) */
+cbx_status cbx_dcsc_check(const cbx_dcsc* X);
+
+/* Copy only the column structure of a device DCSC to host arrays
+ * (cp_host[nzc+1], jc_host[nzc]); either pointer may be NULL. */
+cbx_status cbx_dcsc_structure_to_host(const cbx_dcsc* X, int64_t* cp_host, int64_t* jc_host);
+
+/* Per stored column j of a device DCSC: sum_i mix64(i ^ salt) * bits(X(i,j))
+ * in wrapping uint64 arithmetic, written to host_out[0..nzc).  Linear in X,
+ * so a product's checksums follow from its operands in O(nnz) on the host. */
+cbx_status cbx_dcsc_column_checksums(const cbx_dcsc* X, uint64_t salt, uint64_t* host_out);
+
 /* ------------------------------------------------------------- generators */
 
 enum {
@@ -132,6 +146,7 @@ typedef struct cbx_spgemm_stats {
   int64_t predicted_nnz;  /* learned mode: sum of the predicted column sizes */
   int64_t units_light;    /* row-range units of the numeric pass (small/large) */
   int64_t units_heavy;
+  int64_t nnz_main;       /* outputs written by the row-range-unit kernels */
   double ms_setup;        /* validation, A column map, flop count, binning */
   double ms_symbolic;     /* symbolic pass (two-phase) or prediction (learned) */
   double ms_alloc;        /* output allocation */

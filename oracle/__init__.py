@@ -53,6 +53,9 @@ def _load_oracle() -> C.CDLL:
         L.orc_symbolic.restype = C.c_int32
         L.orc_free.argtypes = [P(cbx_dcsc)]
         L.orc_free.restype = None
+        L.orc_time_spgemm.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, C.c_int64, C.c_int64,
+                                      P(C.c_double), P(C.c_int64), P(C.c_int64)]
+        L.orc_time_spgemm.restype = C.c_int32
         L.orc_max_threads.restype = C.c_int32
         L.orc_set_threads.argtypes = [C.c_int32]
         _orc = L
@@ -74,6 +77,8 @@ def _load_ref() -> C.CDLL:
         L.ref_spgemm.restype = C.c_int32
         L.ref_symbolic.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
         L.ref_symbolic.restype = C.c_int32
+        L.ref_time_hybrid_i64.argtypes = [P(cbx_dcsc), P(cbx_dcsc), P(C.c_double), P(C.c_int64)]
+        L.ref_time_hybrid_i64.restype = C.c_int32
         L.ref_free.argtypes = [P(cbx_dcsc)]
         L.ref_free.restype = None
         _ref = L
@@ -108,6 +113,39 @@ def symbolic(A: Dcsc, B: Dcsc) -> tuple[np.ndarray, np.ndarray]:
     if st != 0:
         raise RuntimeError(f"orc_symbolic failed with status {st}")
     return flop, nnz
+
+
+def time_spgemm(A: Dcsc, B: Dcsc, semiring: str = "plus_times", mode: str = "hybrid",
+                cols: tuple[int, int] | None = None) -> tuple[float, int, int]:
+    """Seconds, nnz and flops of the restated kernel on B's stored columns cols."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    secs, nnz, fl = C.c_double(0), C.c_int64(0), C.c_int64(0)
+    c0, c1 = cols if cols is not None else (0, -1)
+    st = L.orc_time_spgemm(C.byref(sa), C.byref(sb), SEMIRINGS[semiring], MODES[mode], c0, c1, C.byref(secs),
+                           C.byref(nnz), C.byref(fl))
+    if st != 0:
+        raise RuntimeError(f"orc_time_spgemm failed with status {st}")
+    return secs.value, int(nnz.value), int(fl.value)
+
+
+def ref_time_hybrid_i64(A: Dcsc, B: Dcsc) -> tuple[float, int]:
+    """Seconds and nnz of the genuine LocalHybridSpGEMM<PlusTimes<int64>> (oracle/_ref)."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    secs, nnz = C.c_double(0), C.c_int64(0)
+    if L.ref_time_hybrid_i64(C.byref(sa), C.byref(sb), C.byref(secs), C.byref(nnz)) != 0:
+        raise RuntimeError("ref_time_hybrid_i64 failed")
+    return secs.value, int(nnz.value)
+
+
+def column_slice(B: Dcsc, c0: int, c1: int) -> Dcsc:
+    """B restricted to its stored columns [c0, c1) (same shape), like
+    SpDCCols::ColSplit (SpDCCols.cpp:1054) without renumbering."""
+    b0, b1 = int(B.cp[c0]), int(B.cp[c1])
+    return Dcsc(B.m, B.n, B.cp[c0:c1 + 1] - b0, B.jc[c0:c1], B.ir[b0:b1], B.val[b0:b1], B.dtype)
 
 
 def set_threads(n: int) -> None:
@@ -167,3 +205,20 @@ def column_digests(X: Dcsc) -> np.ndarray:
         pos = np.arange(X.nnz, dtype=np.int64) - np.repeat(X.cp[:-1], np.diff(X.cp))
         h = h * (pos.astype(np.uint64) + np.uint64(1))
         return np.add.reduceat(h, X.cp[:-1]) if X.nzc else np.zeros(0, np.uint64)
+
+
+def product_checksums(A: Dcsc, B: Dcsc, salt: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """Closed form of the per-column checksums of C = A*B (plus-times,
+    integer values, wrapping uint64): for B's stored column j,
+    sum_i u(i) C(i,j) = sum_k B(k,j) * w_k with w_k = sum_i u(i) A(i,k),
+    u(i) = mix64(i ^ salt).  Returns (B.jc, checksum per B stored column);
+    columns whose product is empty get 0."""
+    with np.errstate(over="ignore"):
+        u = _mix(A.ir.astype(np.uint64) ^ np.uint64(salt))
+        contrib = u * A.val.astype(np.int64).view(np.uint64)
+        w_col = np.add.reduceat(contrib, A.cp[:-1]) if A.nzc else np.zeros(0, np.uint64)
+        w = np.zeros(A.n, np.uint64)
+        w[A.jc] = w_col
+        t = w[B.ir] * B.val.astype(np.int64).view(np.uint64)
+        cs = np.add.reduceat(t, B.cp[:-1]) if B.nzc else np.zeros(0, np.uint64)
+    return B.jc.copy(), cs

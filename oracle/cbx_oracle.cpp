@@ -325,6 +325,8 @@ using namespace orc;
 
 extern "C" {
 
+void orc_free(cbx_dcsc* C);
+
 // LocalHybridSpGEMM (mode 0) / LocalSpGEMMHash sorted (mode 1) / unsorted
 // (mode 2) on B's stored columns [c0, c1) (c1 < 0 means all).
 cbx_status orc_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semiring, int32_t mode, int64_t c0, int64_t c1,
@@ -351,6 +353,21 @@ cbx_status orc_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop, int
   ColMap cm(*A);
   estimate_flop(*A, *B, cm, 0, B->nzc, flop);
   if (nnz) estimate_nnz_hash(*A, *B, cm, 0, B->nzc, flop, nnz);
+  return CBX_OK;
+}
+
+// Times the restated LocalHybridSpGEMM on B's stored columns [c0, c1) with
+// the current OpenMP thread count; the output is produced, then freed.
+cbx_status orc_time_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semiring, int32_t mode, int64_t c0,
+                           int64_t c1, double* secs, int64_t* nnz, int64_t* flops) {
+  cbx_dcsc C;
+  const double t0 = omp_get_wtime();
+  const cbx_status st = orc_spgemm(A, B, semiring, mode, c0, c1, &C, flops);
+  const double t1 = omp_get_wtime();
+  if (st != CBX_OK) return st;
+  *secs = t1 - t0;
+  *nnz = C.nnz;
+  orc_free(&C);
   return CBX_OK;
 }
 

@@ -7,6 +7,7 @@
 // the product and never runs on a machine without the reference tree at
 // build time.
 #include <mpi.h>
+#include <omp.h>
 
 #include <cstdint>
 #include <cstdlib>
@@ -142,6 +143,24 @@ int ref_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop, int64_t* n
   std::memcpy(nnz, z, sizeof(int64_t) * B->nzc);
   delete[] f;
   delete[] z;
+  delete MA;
+  delete MB;
+  return 0;
+}
+
+// Times combblas::LocalHybridSpGEMM<PlusTimesSRing<int64,int64>,int64>(A, B)
+// alone (operands converted to SpDCCols beforehand; the returned SpTuples is
+// deleted after the clock stops).
+int ref_time_hybrid_i64(const cbx_dcsc* A, const cbx_dcsc* B, double* secs, int64_t* nnz) {
+  ensure_mpi();
+  auto* MA = make_mat<int64_t, int64_t>(*A);
+  auto* MB = make_mat<int64_t, int64_t>(*B);
+  const double t0 = omp_get_wtime();
+  SpTuples<int64_t, int64_t>* T = LocalHybridSpGEMM<PlusTimesSRing<int64_t, int64_t>, int64_t>(*MA, *MB, false, false);
+  const double t1 = omp_get_wtime();
+  *secs = t1 - t0;
+  *nnz = T->getnnz();
+  delete T;
   delete MA;
   delete MB;
   return 0;

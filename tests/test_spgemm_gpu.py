@@ -176,3 +176,24 @@ def test_invalid_input_rejected():
     assert ei.value.status == cb.ERR_INVALID
     with pytest.raises(cb.CbxError):
         gpu_spgemm(A, cb.gen_rmat(9, 4))  # inner dimensions differ
+
+
+@pytest.mark.parametrize("scale,ef", [(18, 16)])
+def test_full_size_linear_checksums(scale, ef):
+    """Large R-MAT product checked without materialising it on the host:
+    device structure check (rows sorted and in range) plus every column's
+    weighted sum against its closed form sum_k (u^T A)_k A(k,j)."""
+    A = cb.gen_rmat(scale, ef)
+    dA = A.to_device()
+    dC, stats = cb.spgemm(dA, dA)
+    dC.check()
+    got = dC.column_checksums(salt=12345)
+    _, jcC = dC.structure()
+    jcB, want = oracle.product_checksums(A, A, salt=12345)
+    pos = np.searchsorted(jcB, jcC)
+    assert np.array_equal(jcB[pos], jcC)
+    assert np.array_equal(got, want[pos])
+    dropped = np.ones(jcB.size, bool)
+    dropped[pos] = False
+    assert not np.any(want[dropped])
+    assert stats["units_light"] + stats["units_heavy"] > 0
