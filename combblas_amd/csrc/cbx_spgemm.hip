@@ -89,6 +89,131 @@ __device__ __forceinline__ int ceil_log2_dev(long long x) {
   return lg;
 }
 
+// ============================================================ unit tier
+// Row-range units.  The symbolic pass builds, per column, the full presence
+// bitmap of its output rows in LDS, counts the rows of every block of kBR
+// rows, and cuts the column into units: runs of consecutive blocks holding
+// at most KMAX outputs and spanning at most RBLK blocks.  The numeric pass
+// runs one workgroup per unit.  A unit's accumulator is compact: the value of
+// row r lives at rank(r) = number of output rows of the unit below r, read
+// from the unit's presence bitmap with per-word prefix counts, so the unit
+// is emitted in row order without sorting and LDS holds only real outputs.
+constexpr int kBR = 4096;        // rows per block (also the tile-pointer granularity)
+constexpr int kBRW = kBR / 32;   // 32-bit bitmap words per block
+
+struct Unit {
+  int32_t col;     // B stored-column position
+  int32_t blk_lo;  // first block
+  int32_t blk_hi;  // one past the last block
+  int32_t cnt;     // outputs in the unit
+  int32_t off;     // offset of the unit's first output within the column
+};
+
+// Per-chunk segment staging for block-level flattening of products.
+template <class T, int NT>
+struct SegStage {
+  int off[NT + 1];
+  int64_t base[NT];
+  T bval[NT];
+};
+
+// Segment of B entry e restricted to P tiles [t_lo, t_hi) (or the whole A
+// column when `full`).
+template <class T, bool VAL>
+__device__ __forceinline__ void load_seg(int64_t e, int64_t b1, const int32_t* __restrict__ bpos,
+                                         const int64_t* __restrict__ cpA, const int32_t* __restrict__ P, int S,
+                                         int t_lo, int t_hi, bool full, const T* __restrict__ valB, int64_t& base,
+                                         int& len, T& bv) {
+  base = 0;
+  len = 0;
+  bv = T{};
+  if (e < b1) {
+    const int32_t p = bpos[e];
+    if (p >= 0) {
+      if (full) {
+        base = cpA[p];
+        len = (int)(cpA[p + 1] - base);
+      } else {
+        const int32_t* pp = P + (int64_t)p * (S + 1);
+        const int a0 = pp[t_lo];
+        base = cpA[p] + a0;
+        len = pp[t_hi] - a0;
+      }
+      if constexpr (VAL) bv = valB[e];
+    }
+  }
+}
+
+// Stages one chunk (at most one segment per thread): the non-empty segments
+// are compacted in order with one packed 64-bit block scan (high half counts
+// segments, low half sums lengths), so staged offsets strictly increase.
+// Returns the number of products; `nseg` receives the number of segments.
+// Contains barriers.
+template <class T, int NT>
+__device__ __forceinline__ int stage_chunk(SegStage<T, NT>& st, long long* wsum, int64_t base, int len, T bv,
+                                           int& nseg) {
+  const long long packed = len > 0 ? ((1ll << 32) | (long long)(unsigned)len) : 0ll;
+  long long tot;
+  const long long ex = block_excl_scan_ll<NT>(packed, wsum, tot);
+  if (len > 0) {
+    const int k = (int)(ex >> 32);
+    st.off[k] = (int)(ex & 0xffffffffll);
+    st.base[k] = base;
+    st.bval[k] = bv;
+  }
+  nseg = (int)(tot >> 32);
+  const int total = (int)(tot & 0xffffffffll);
+  if (threadIdx.x == 0) st.off[nseg] = total;
+  __syncthreads();
+  return total;
+}
+
+// Visits all `tot` staged products.  Each thread takes U consecutive
+// products per round: one binary search for the first, then (segments being
+// non-empty) at most one step forward per product.  Loads are issued for all
+// U products before f runs.  f(idx, bv) gets idx = -1 for lanes past the end.
+// No barriers.
+// Products xb .. xb+U-1 of a staged chunk: A-entry index (-1 past the end)
+// and B value of each.
+template <class T, int NT, int U>
+__device__ __forceinline__ void fill_round(const SegStage<T, NT>& st, int nseg, int tot, int xb, int64_t (&idx)[U],
+                                           T (&bvs)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) idx[u] = -1;
+  if (xb >= tot) return;
+  int lo = 0, hi = nseg;  // largest k with off[k] <= xb (off[nseg] = tot > xb)
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (st.off[mid] <= xb) lo = mid;
+    else hi = mid;
+  }
+  int k = lo;
+  int cur = st.off[k], nxt = st.off[k + 1];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int x = xb + u;
+    if (x < tot) {
+      if (x >= nxt) {
+        ++k;
+        cur = nxt;
+        nxt = st.off[k + 1];
+      }
+      idx[u] = st.base[k] + (x - cur);
+      bvs[u] = st.bval[k];
+    }
+  }
+}
+
+template <class T, int NT, int U, class F>
+__device__ __forceinline__ void visit_staged(const SegStage<T, NT>& st, int nseg, int tot, F&& f, int first_round = 0) {
+  for (int xb = (first_round * NT + (int)threadIdx.x) * U; xb < tot; xb += NT * U) {
+    int64_t idx[U];
+    T bvs[U];
+    fill_round<T, NT, U>(st, nseg, tot, xb, idx, bvs);
+    f(idx, bvs);
+  }
+}
+
 // ------------------------------------------------------------------ setup
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -228,6 +353,7 @@ __global__ __launch_bounds__(NT) void k_sym_hash(const int32_t* __restrict__ ord
                                                  int* __restrict__ err) {
   __shared__ int32_t keys[HMAX];
   __shared__ long long wsum[NT / kWave];
+  __shared__ SegStage<uint8_t, NT> st;
   const int col = order[first + blockIdx.x];
   const int lg = min(ceil_log2_dev(max(2 * flop[col], 64ll)), ceil_log2_dev(HMAX));
   const int H = 1 << lg;
@@ -235,28 +361,30 @@ __global__ __launch_bounds__(NT) void k_sym_hash(const int32_t* __restrict__ ord
   __syncthreads();
   long long cnt = 0;
   const int64_t b0 = cpB[col], b1 = cpB[col + 1];
-  for (int64_t eb = b0 + wave_id() * kWave; eb < b1; eb += NT) {
-    const int64_t e = eb + lane_id();
-    int64_t lo = 0;
-    int len = 0;
-    if (e < b1) {
-      const int32_t p = bpos[e];
-      if (p >= 0) {
-        lo = cpA[p];
-        len = (int)(cpA[p + 1] - lo);
+  for (int64_t eb = b0; eb < b1; eb += NT) {
+    int64_t base;
+    int len, nseg;
+    uint8_t dummy;
+    load_seg<uint8_t, false>(eb + threadIdx.x, b1, bpos, cpA, nullptr, 0, 0, 0, true, nullptr, base, len, dummy);
+    const int tot = stage_chunk<uint8_t, NT>(st, wsum, base, len, 0, nseg);
+    visit_staged<uint8_t, NT, 4>(st, nseg, tot, [&](const int64_t(&idx)[4], const uint8_t(&)[4]) {
+      int32_t rr[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) rr[u] = idx[u] >= 0 ? rowsA[idx[u]] : -1;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int32_t r = rr[u];
+        if (r < 0) continue;
+        uint32_t h = hash_slot(r, lg);
+        int probe = 0;
+        for (; probe < H; ++probe) {  // the table is at least twice the column's flops
+          const int32_t old = atomicCAS(&keys[h], -1, r);
+          if (old == -1) { ++cnt; break; }
+          if (old == r) break;
+          h = (h + 1) & (H - 1);
+        }
+        if (probe == H) atomicOr(err, 1);
       }
-    }
-    wave_visit(lo, len, 0, [&](int64_t idx, int) {
-      const int32_t r = rowsA[idx];
-      uint32_t h = hash_slot(r, lg);
-      int probe = 0;
-      for (; probe < H; ++probe) {  // the table is at least twice the column's flops
-        const int32_t old = atomicCAS(&keys[h], -1, r);
-        if (old == -1) { ++cnt; break; }
-        if (old == r) break;
-        h = (h + 1) & (H - 1);
-      }
-      if (probe == H) atomicOr(err, 1);
     });
   }
   const long long tot = block_sum_ll<NT>(cnt, wsum);
@@ -330,7 +458,8 @@ __global__ __launch_bounds__(NT) void k_num_hash(const int32_t* __restrict__ ord
   using Acc = typename SR::Acc;
   __shared__ int32_t keys[HMAX];
   __shared__ Acc acc[HMAX];
-  __shared__ int wsum[NT / kWave];
+  __shared__ long long wsum[NT / kWave];
+  __shared__ SegStage<T, NT> st;
   const int col = order[first + blockIdx.x];
   const int64_t nz = nnzcol[col];
   const int lg = min(ceil_log2_dev(max(2 * nz, 64ll)), ceil_log2_dev(HMAX));
@@ -341,31 +470,38 @@ __global__ __launch_bounds__(NT) void k_num_hash(const int32_t* __restrict__ ord
   }
   __syncthreads();
   const int64_t b0 = cpB[col], b1 = cpB[col + 1];
-  for (int64_t eb = b0 + wave_id() * kWave; eb < b1; eb += NT) {
-    const int64_t e = eb + lane_id();
-    int64_t lo = 0;
-    int len = 0;
-    T bv{};
-    if (e < b1) {
-      const int32_t p = bpos[e];
-      if (p >= 0) {
-        lo = cpA[p];
-        len = (int)(cpA[p + 1] - lo);
-        bv = valB[e];
+  for (int64_t eb = b0; eb < b1; eb += NT) {
+    int64_t base;
+    int len, nseg;
+    T bv;
+    load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, nullptr, 0, 0, 0, true, valB, base, len, bv);
+    const int tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+    visit_staged<T, NT, 4>(st, nseg, tot, [&](const int64_t(&idx)[4], const T(&bvs)[4]) {
+      int32_t rr[4];
+      T aa[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        rr[u] = -1;
+        if (idx[u] >= 0) {
+          rr[u] = rowsA[idx[u]];
+          aa[u] = valA[idx[u]];
+        }
       }
-    }
-    wave_visit(lo, len, bv, [&](int64_t idx, T b) {
-      const int32_t r = rowsA[idx];
-      const Acc v = SR::mul(valA[idx], b);
-      uint32_t h = hash_slot(r, lg);
-      int probe = 0;
-      for (; probe < H; ++probe) {  // the table is at least twice the column's nnz
-        const int32_t old = atomicCAS(&keys[h], -1, r);
-        if (old == -1 || old == r) break;
-        h = (h + 1) & (H - 1);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int32_t r = rr[u];
+        if (r < 0) continue;
+        const Acc v = SR::mul(aa[u], bvs[u]);
+        uint32_t h = hash_slot(r, lg);
+        int probe = 0;
+        for (; probe < H; ++probe) {  // the table is at least twice the column's nnz
+          const int32_t old = atomicCAS(&keys[h], -1, r);
+          if (old == -1 || old == r) break;
+          h = (h + 1) & (H - 1);
+        }
+        if (probe < H) SR::acc(&acc[h], v);
+        else atomicOr(err, 2);
       }
-      if (probe < H) SR::acc(&acc[h], v);
-      else atomicOr(err, 2);
     });
   }
   __syncthreads();
@@ -375,8 +511,8 @@ __global__ __launch_bounds__(NT) void k_num_hash(const int32_t* __restrict__ ord
     const int i = c0 + threadIdx.x;
     const int32_t k = i < H ? keys[i] : -1;
     const Acc a = i < H ? acc[i] : SR::ident();
-    int tot;
-    const int ex = block_excl_scan<NT>(k != -1 ? 1 : 0, wsum, tot);
+    long long tot;
+    const int ex = (int)block_excl_scan_ll<NT>(k != -1 ? 1 : 0, wsum, tot);
     if (k != -1) {
       keys[base + ex] = k;
       acc[base + ex] = a;
@@ -486,121 +622,6 @@ __global__ __launch_bounds__(NT) void k_num_dense(const int32_t* __restrict__ or
     bits[threadIdx.x] = 0u;
     out += tot;
     __syncthreads();
-  }
-}
-
-// ============================================================ unit tier
-// Row-range units.  The symbolic pass builds, per column, the full presence
-// bitmap of its output rows in LDS, counts the rows of every block of kBR
-// rows, and cuts the column into units: runs of consecutive blocks holding
-// at most KMAX outputs and spanning at most RBLK blocks.  The numeric pass
-// runs one workgroup per unit.  A unit's accumulator is compact: the value of
-// row r lives at rank(r) = number of output rows of the unit below r, read
-// from the unit's presence bitmap with per-word prefix counts, so the unit
-// is emitted in row order without sorting and LDS holds only real outputs.
-constexpr int kBR = 4096;        // rows per block (also the tile-pointer granularity)
-constexpr int kBRW = kBR / 32;   // 32-bit bitmap words per block
-
-struct Unit {
-  int32_t col;     // B stored-column position
-  int32_t blk_lo;  // first block
-  int32_t blk_hi;  // one past the last block
-  int32_t cnt;     // outputs in the unit
-  int32_t off;     // offset of the unit's first output within the column
-};
-
-// Per-chunk segment staging for block-level flattening of products.
-template <class T, int NT>
-struct SegStage {
-  int off[NT + 1];
-  int64_t base[NT];
-  T bval[NT];
-};
-
-// Segment of B entry e restricted to P tiles [t_lo, t_hi) (or the whole A
-// column when `full`).
-template <class T, bool VAL>
-__device__ __forceinline__ void load_seg(int64_t e, int64_t b1, const int32_t* __restrict__ bpos,
-                                         const int64_t* __restrict__ cpA, const int32_t* __restrict__ P, int S,
-                                         int t_lo, int t_hi, bool full, const T* __restrict__ valB, int64_t& base,
-                                         int& len, T& bv) {
-  base = 0;
-  len = 0;
-  bv = T{};
-  if (e < b1) {
-    const int32_t p = bpos[e];
-    if (p >= 0) {
-      if (full) {
-        base = cpA[p];
-        len = (int)(cpA[p + 1] - base);
-      } else {
-        const int32_t* pp = P + (int64_t)p * (S + 1);
-        const int a0 = pp[t_lo];
-        base = cpA[p] + a0;
-        len = pp[t_hi] - a0;
-      }
-      if constexpr (VAL) bv = valB[e];
-    }
-  }
-}
-
-// Stages one chunk (at most one segment per thread): the non-empty segments
-// are compacted in order with one packed 64-bit block scan (high half counts
-// segments, low half sums lengths), so staged offsets strictly increase.
-// Returns the number of products; `nseg` receives the number of segments.
-// Contains barriers.
-template <class T, int NT>
-__device__ __forceinline__ int stage_chunk(SegStage<T, NT>& st, long long* wsum, int64_t base, int len, T bv,
-                                           int& nseg) {
-  const long long packed = len > 0 ? ((1ll << 32) | (long long)(unsigned)len) : 0ll;
-  long long tot;
-  const long long ex = block_excl_scan_ll<NT>(packed, wsum, tot);
-  if (len > 0) {
-    const int k = (int)(ex >> 32);
-    st.off[k] = (int)(ex & 0xffffffffll);
-    st.base[k] = base;
-    st.bval[k] = bv;
-  }
-  nseg = (int)(tot >> 32);
-  const int total = (int)(tot & 0xffffffffll);
-  if (threadIdx.x == 0) st.off[nseg] = total;
-  __syncthreads();
-  return total;
-}
-
-// Visits all `tot` staged products.  Each thread takes U consecutive
-// products per round: one binary search for the first, then (segments being
-// non-empty) at most one step forward per product.  Loads are issued for all
-// U products before f runs.  f(idx, bv) gets idx = -1 for lanes past the end.
-// No barriers.
-template <class T, int NT, int U, class F>
-__device__ __forceinline__ void visit_staged(const SegStage<T, NT>& st, int nseg, int tot, F&& f) {
-  for (int xb = threadIdx.x * U; xb < tot; xb += NT * U) {
-    int lo = 0, hi = nseg;  // largest k with off[k] <= xb (off[nseg] = tot > xb)
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (st.off[mid] <= xb) lo = mid;
-      else hi = mid;
-    }
-    int k = lo;
-    int cur = st.off[k], nxt = st.off[k + 1];
-    int64_t idx[U];
-    T bvs[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int x = xb + u;
-      idx[u] = -1;
-      if (x < tot) {
-        if (x >= nxt) {
-          ++k;
-          cur = nxt;
-          nxt = st.off[k + 1];
-        }
-        idx[u] = st.base[k] + (x - cur);
-        bvs[u] = st.bval[k];
-      }
-    }
-    f(idx, bvs);
   }
 }
 
@@ -717,15 +738,18 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
   if (threadIdx.x == 0) nnzcol[col] = colcnt;
 }
 
-// Numeric pass of the unit tier: one workgroup per unit.
-template <int K, class T, int NT, int U, int KMAX, int RBLK>
+// Numeric pass of the unit tier: one workgroup per unit.  Sweep 1 sets the
+// unit's presence bitmap, a block scan turns it into per-word ranks, sweep 2
+// accumulates every product at its row's rank.  When the column's B entries
+// fit one chunk, the first CR rounds of products (row, product value) stay in
+// registers between the sweeps, so sweep 2 re-reads nothing for them.
+template <int K, class T, int NT, int U, int KMAX, int RBLK, int CR>
 __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units, const int64_t* __restrict__ cpB,
                                                   const int32_t* __restrict__ bpos, const T* __restrict__ valB,
                                                   const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                                                   const T* __restrict__ valA, const int32_t* __restrict__ P, int S,
-                                                  int64_t m, const int64_t* __restrict__ cpC,
-                                                  int64_t* __restrict__ irC, T* __restrict__ valC,
-                                                  int* __restrict__ err) {
+                                                  const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
+                                                  T* __restrict__ valC, int* __restrict__ err) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int NW32 = RBLK * kBRW;   // bitmap words
@@ -751,23 +775,80 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
   for (int i = threadIdx.x; i < cnt; i += NT) vals[i] = SR::ident();
   const int nw32 = (t_hi - t_lo) * kBRW;
   for (int i = threadIdx.x; i < nw32; i += NT) bits[i] = 0u;
-  // sweep 1: presence bitmap of the unit's rows
+
+  auto set_bits = [&](const int64_t(&idx)[U], const T(&)[U]) {
+    int32_t r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+  };
+  auto rank_of = [&](int32_t r) {
+    const int w = r >> 6, b = r & 63;
+    const uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+    return (int)pre[w] + __popcll(word & ((1ull << b) - 1ull));
+  };
+  auto accumulate = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
+    int32_t r[U];
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      r[u] = -1;
+      if (idx[u] >= 0) {
+        r[u] = rowsA[idx[u]] - row0;
+        a[u] = valA[idx[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r[u] < 0) continue;
+      const int rank = rank_of(r[u]);
+      if (rank < cnt) SR::acc(&vals[rank], SR::mul(a[u], bvs[u]));
+    }
+  };
+
+  int32_t crow[CR][U];
+  Acc cprod[CR][U];
   int tot = 0, nseg = 0;
-  for (int64_t eb = b0; eb < b1; eb += NT) {
+  if (single_chunk) {
     int64_t base;
     int len;
     T bv;
-    load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+    load_seg<T, true>(b0 + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
     tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
-    visit_staged<T, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const T(&)[U]) {
-      int32_t r[U];
+    // sweep 1, cached rounds
 #pragma unroll
-      for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
+    for (int rr = 0; rr < CR; ++rr) {
+      int64_t idx[U];
+      T bvs[U];
+      fill_round<T, NT, U>(st, nseg, tot, (rr * NT + (int)threadIdx.x) * U, idx, bvs);
+      T a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        crow[rr][u] = -1;
+        if (idx[u] >= 0) {
+          crow[rr][u] = rowsA[idx[u]] - row0;
+          a[u] = valA[idx[u]];
+        }
+      }
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
-    });
-    if (!single_chunk) __syncthreads();
+        if (crow[rr][u] >= 0) {
+          atomicOr(&bits[crow[rr][u] >> 5], 1u << (crow[rr][u] & 31));
+          cprod[rr][u] = SR::mul(a[u], bvs[u]);
+        }
+    }
+    visit_staged<T, NT, U>(st, nseg, tot, set_bits, CR);
+  } else {
+    for (int64_t eb = b0; eb < b1; eb += NT) {
+      int64_t base;
+      int len;
+      T bv;
+      load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+      tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+      visit_staged<T, NT, U>(st, nseg, tot, set_bits);
+    }
   }
   __syncthreads();
   // exclusive prefix popcount per 64-bit word
@@ -793,28 +874,16 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
   }
   __syncthreads();
   // sweep 2: accumulate values at their ranks
-  auto accumulate = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
-    int32_t r[U];
-    T a[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      r[u] = -1;
-      if (idx[u] >= 0) {
-        r[u] = rowsA[idx[u]] - row0;
-        a[u] = valA[idx[u]];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (r[u] < 0) continue;
-      const int w = r[u] >> 6, b = r[u] & 63;
-      const uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
-      const int rank = pre[w] + __popcll(word & ((1ull << b) - 1ull));
-      if (rank < cnt) SR::acc(&vals[rank], SR::mul(a[u], bvs[u]));
-    }
-  };
   if (single_chunk) {
-    visit_staged<T, NT, U>(st, nseg, tot, accumulate);
+#pragma unroll
+    for (int rr = 0; rr < CR; ++rr)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (crow[rr][u] >= 0) {
+          const int rank = rank_of(crow[rr][u]);
+          if (rank < cnt) SR::acc(&vals[rank], cprod[rr][u]);
+        }
+    visit_staged<T, NT, U>(st, nseg, tot, accumulate, CR);
   } else {
     for (int64_t eb = b0; eb < b1; eb += NT) {
       int64_t base;
@@ -824,7 +893,6 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
       int ns2;
       const int t2 = stage_chunk<T, NT>(st, wsum, base, len, bv, ns2);
       visit_staged<T, NT, U>(st, ns2, t2, accumulate);
-      __syncthreads();
     }
   }
   __syncthreads();
@@ -1087,11 +1155,11 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   // --- numeric
   if (use_units) {
     if (hu[1])
-      k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
-          units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, m, cpC, C->ir, (T*)C->val, err);
+      k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+          units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err);
     if (hu[0])
-      k_num_units<K, T, kLightNT, 4, kLightK, kLightRB><<<(unsigned)hu[0], kLightNT, 0, s>>>(
-          units_l, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, m, cpC, C->ir, (T*)C->val, err);
+      k_num_units<K, T, kLightNT, 4, kLightK, kLightRB, 2><<<(unsigned)hu[0], kLightNT, 0, s>>>(
+          units_l, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err);
   } else if (n_dense) {
     k_num_dense<K, T, kDenseNT><<<(unsigned)n_dense, kDenseNT, 0, s>>>(order_s, 0, B->cp, bpos, valB, A->cp, rowsA,
                                                                      valA, P, S, cpC, C->ir, (T*)C->val, err);
