@@ -64,6 +64,19 @@ class cbx_spgemm_stats(C.Structure):
     ]
 
 
+class cbx_mlp_train_opts(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("pad", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
+MLP_FEATURES, MLP_HIDDEN = 6, 64
+MLP_PARAMS = 64 * 6 + 64 + 64 * 64 + 64 + 64 + 1 + 12
+
+
+class cbx_bfs_stats(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("pad", C.c_int32), ("reached", C.c_int64), ("ms", C.c_double)]
+
+
 _lib_handle = None
 
 
@@ -91,8 +104,28 @@ def lib() -> C.CDLL:
         L.cbx_dcsc_structure_to_host.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p]
         L.cbx_dcsc_structure_to_host.restype = C.c_int32
         L.cbx_dcsc_column_checksums.argtypes = [P(cbx_dcsc), C.c_uint64, C.c_void_p]
+        L.cbx_malloc.argtypes = [P(C.c_void_p), C.c_size_t]
+        L.cbx_free.argtypes = [C.c_void_p]
+        L.cbx_free.restype = None
+        L.cbx_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int32]
+        L.cbx_spmspv.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                 C.c_void_p, P(C.c_int32), C.c_void_p]
+        L.cbx_bfs_plan_create.argtypes = [P(cbx_dcsc), P(C.c_void_p)]
+        L.cbx_bfs_run.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, P(cbx_bfs_stats)]
+        L.cbx_bfs_plan_destroy.argtypes = [C.c_void_p]
+        L.cbx_bfs_plan_destroy.restype = None
+        L.cbx_spgemm_symbolic.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
+        L.cbx_spgemm_features.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
+        L.cbx_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.cbx_mlp_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, P(C.c_float)]
+        L.cbx_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, P(cbx_mlp_train_opts),
+                                    C.c_void_p]
+        for name in ("cbx_spgemm_symbolic", "cbx_spgemm_features", "cbx_mlp_forward", "cbx_mlp_gradient",
+                     "cbx_mlp_train"):
+            getattr(L, name).restype = C.c_int32
         for name in ("cbx_dcsc_to_device", "cbx_dcsc_to_host", "cbx_gen_rmat", "cbx_spgemm", "cbx_dcsc_check",
-                     "cbx_dcsc_column_checksums"):
+                     "cbx_dcsc_column_checksums", "cbx_malloc", "cbx_memcpy", "cbx_spmspv", "cbx_bfs_plan_create",
+                     "cbx_bfs_run"):
             getattr(L, name).restype = C.c_int32
         _lib_handle = L
     return _lib_handle
@@ -316,3 +349,142 @@ def LocalSpGEMMHash(A: Dcsc, B: Dcsc, semiring: str = "plus_times", sort: bool =
     dA, dB = A.to_device(), B.to_device()
     dC, _ = spgemm(dA, dB, semiring, "two_phase" if sort else "unsorted")
     return dC.to_host()
+
+
+# ------------------------------------------------------------------ vectors
+class DeviceArray:
+    """A device buffer from the library's cache holding a 1-D numpy-typed array."""
+
+    def __init__(self, n: int, dtype):
+        self.n, self.dtype = int(n), np.dtype(dtype)
+        p = C.c_void_p()
+        _check(lib().cbx_malloc(C.byref(p), max(self.n * self.dtype.itemsize, 8)))
+        self.ptr = p.value
+
+    @staticmethod
+    def from_host(a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.size, a.dtype)
+        if a.size:
+            _check(lib().cbx_memcpy(d.ptr, a.ctypes.data, a.nbytes, 0))
+        return d
+
+    def to_host(self, n: int | None = None) -> np.ndarray:
+        n = self.n if n is None else n
+        out = np.empty(n, self.dtype)
+        if n:
+            _check(lib().cbx_memcpy(out.ctypes.data, self.ptr, n * self.dtype.itemsize, 1))
+        return out
+
+    def release(self) -> None:
+        if self.ptr and _lib_handle is not None:
+            _lib_handle.cbx_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+_XDT = {np.dtype(np.int64): "int64", np.dtype(np.float64): "float64", np.dtype(np.float32): "float32"}
+
+
+def spmspv(A: DeviceDcsc, xind: np.ndarray, xval: np.ndarray, semiring: str = "select_max") -> tuple[np.ndarray, np.ndarray]:
+    """y = A (x) x for a sparse x (SpMXSpV, SpImpl.cpp:55-213); returns (yind, yval) on the host."""
+    xind = np.ascontiguousarray(xind, np.int32)
+    xval = np.ascontiguousarray(xval)
+    dt = _XDT[xval.dtype]
+    dx, dv = DeviceArray.from_host(xind), DeviceArray.from_host(xval)
+    dyi, dyv = DeviceArray(A.m, np.int32), DeviceArray(A.m, xval.dtype)
+    ny = C.c_int32(0)
+    _check(lib().cbx_spmspv(C.byref(A.s), dx.ptr, dv.ptr, xind.size, SEMIRINGS[semiring], DTYPES[dt], dyi.ptr,
+                            dyv.ptr, C.byref(ny), None))
+    return dyi.to_host(ny.value), dyv.to_host(ny.value)
+
+
+class BfsPlan:
+    """Top-down BFS plan over a device adjacency matrix (square, symmetric)."""
+
+    def __init__(self, A: DeviceDcsc):
+        h = C.c_void_p()
+        _check(lib().cbx_bfs_plan_create(C.byref(A.s), C.byref(h)))
+        self.h = h.value
+        self.n = A.n
+        self.parents = DeviceArray(A.n, np.int64)
+
+    def run(self, root: int, copy: bool = True) -> tuple[np.ndarray | None, dict]:
+        st = cbx_bfs_stats()
+        _check(lib().cbx_bfs_run(self.h, int(root), self.parents.ptr, C.byref(st)))
+        stats = {"levels": st.levels, "reached": st.reached, "ms": st.ms}
+        return (self.parents.to_host() if copy else None), stats
+
+    def release(self) -> None:
+        if self.h and _lib_handle is not None:
+            _lib_handle.cbx_bfs_plan_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def bfs_roots(A: Dcsc, count: int, seed: int = 4242) -> np.ndarray:
+    """`count` distinct BFS roots drawn uniformly (seeded) among the vertices
+    of degree > 0 (the reference draws its roots among non-isolated vertices,
+    TopDownBFS.cpp:380-411)."""
+    deg = np.zeros(A.n, np.int64)
+    deg[A.jc] = np.diff(A.cp)
+    cand = np.nonzero(deg > 0)[0]
+    rng = np.random.default_rng(seed)
+    return rng.choice(cand, size=min(count, cand.size), replace=False)
+
+
+# ------------------------------------------------------- learned predictor
+def spgemm_symbolic(A: DeviceDcsc, B: DeviceDcsc) -> np.ndarray:
+    """Exact nnz of C(:,j) for every stored column j of B (the training labels)."""
+    d = DeviceArray(max(B.nzc, 1), np.int64)
+    _check(lib().cbx_spgemm_symbolic(C.byref(A.s), C.byref(B.s), d.ptr, None))
+    return d.to_host(B.nzc)
+
+
+def spgemm_features(A: DeviceDcsc, B: DeviceDcsc) -> np.ndarray:
+    """float32[nzc_B, 6] predictor features (see cbx_spgemm_features)."""
+    d = DeviceArray(max(B.nzc * MLP_FEATURES, 1), np.float32)
+    _check(lib().cbx_spgemm_features(C.byref(A.s), C.byref(B.s), d.ptr, None))
+    return d.to_host(B.nzc * MLP_FEATURES).reshape(B.nzc, MLP_FEATURES)
+
+
+def mlp_forward(params: np.ndarray, X: np.ndarray) -> np.ndarray:
+    """Predicted log2(1 + nnz) for feature rows X (on the device)."""
+    X = np.ascontiguousarray(X, np.float32)
+    dp, dx = DeviceArray.from_host(np.ascontiguousarray(params, np.float32)), DeviceArray.from_host(X)
+    do = DeviceArray(max(len(X), 1), np.float32)
+    _check(lib().cbx_mlp_forward(dp.ptr, dx.ptr, len(X), do.ptr, None))
+    return do.to_host(len(X))
+
+
+def mlp_gradient(params: np.ndarray, X: np.ndarray, y: np.ndarray) -> tuple[np.ndarray, float]:
+    """Full-batch MSE gradient and loss from the device kernels."""
+    dp = DeviceArray.from_host(np.ascontiguousarray(params, np.float32))
+    dx = DeviceArray.from_host(np.ascontiguousarray(X, np.float32))
+    dy = DeviceArray.from_host(np.ascontiguousarray(y, np.float32))
+    dg = DeviceArray(MLP_PARAMS, np.float32)
+    loss = C.c_float(0)
+    _check(lib().cbx_mlp_gradient(dp.ptr, dx.ptr, dy.ptr, len(X), dg.ptr, C.byref(loss)))
+    return dg.to_host(), float(loss.value)
+
+
+def mlp_train(params: np.ndarray, X: np.ndarray, y: np.ndarray, steps: int, batch: int, lr: float,
+              seed: int = 7) -> tuple[np.ndarray, np.ndarray]:
+    """Adam training on the device; returns (trained params, per-step loss)."""
+    dp = DeviceArray.from_host(np.ascontiguousarray(params, np.float32))
+    dx = DeviceArray.from_host(np.ascontiguousarray(X, np.float32))
+    dy = DeviceArray.from_host(np.ascontiguousarray(y, np.float32))
+    curve = np.zeros(steps, np.float32)
+    o = cbx_mlp_train_opts(steps, batch, lr, 0, seed)
+    _check(lib().cbx_mlp_train(dp.ptr, dx.ptr, dy.ptr, len(X), C.byref(o), curve.ctypes.data))
+    return dp.to_host(), curve

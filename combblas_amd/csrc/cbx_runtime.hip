@@ -201,6 +201,24 @@ static cbx_status copy_dcsc(const cbx_dcsc* src, cbx_dcsc* dst, cbx_location to)
   return CBX_OK;
 }
 
+extern "C" cbx_status cbx_malloc(void** ptr, size_t bytes) {
+  CBX_RETURN_IF(!ptr, CBX_ERR_INVALID, "cbx_malloc: null argument");
+  cbx_status st = init_device_once();
+  if (st) return st;
+  return dev_alloc(ptr, bytes);
+}
+
+extern "C" void cbx_free(void* ptr) { dev_free(ptr); }
+
+extern "C" cbx_status cbx_memcpy(void* dst, const void* src, size_t bytes, int32_t kind) {
+  CBX_RETURN_IF(kind < 0 || kind > 2, CBX_ERR_INVALID, "cbx_memcpy: kind must be 0, 1 or 2");
+  if (bytes == 0) return CBX_OK;
+  CBX_RETURN_IF(!dst || !src, CBX_ERR_INVALID, "cbx_memcpy: null pointer");
+  const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+  CBX_HIP(hipMemcpy(dst, src, bytes, k[kind]));
+  return CBX_OK;
+}
+
 extern "C" cbx_status cbx_dcsc_to_device(const cbx_dcsc* host_src, cbx_dcsc* dev_dst) {
   CBX_RETURN_IF(!dev_dst, CBX_ERR_INVALID, "cbx_dcsc_to_device: dst is null");
   cbx_status st = check_dcsc(host_src, "src", CBX_HOST);
@@ -234,4 +252,28 @@ extern "C" cbx_status cbx_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semir
   if ((st = init_device_once())) return st;
   SpgemmConfig cfg;
   return spgemm_dispatch(A, B, semiring, mode, cfg, C, stats, (hipStream_t)stream);
+}
+
+static cbx_status check_pair(const cbx_dcsc* A, const cbx_dcsc* B, const char* who) {
+  cbx_status st;
+  if ((st = check_dcsc(A, "A", CBX_DEVICE))) return st;
+  if ((st = check_dcsc(B, "B", CBX_DEVICE))) return st;
+  CBX_RETURN_IF(A->n != B->m, CBX_ERR_INVALID, "%s: inner dimensions differ", who);
+  CBX_RETURN_IF(A->nnz >= (1ll << 31) || B->nnz >= (1ll << 31), CBX_ERR_LIMIT,
+                "%s: operands must hold fewer than 2^31 nonzeros each", who);
+  return init_device_once();
+}
+
+extern "C" cbx_status cbx_spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, void* stream) {
+  cbx_status st = check_pair(A, B, "cbx_spgemm_symbolic");
+  if (st) return st;
+  CBX_RETURN_IF(!nnz, CBX_ERR_INVALID, "cbx_spgemm_symbolic: null output");
+  return spgemm_symbolic(A, B, nnz, (hipStream_t)stream);
+}
+
+extern "C" cbx_status cbx_spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, void* stream) {
+  cbx_status st = check_pair(A, B, "cbx_spgemm_features");
+  if (st) return st;
+  CBX_RETURN_IF(!feats, CBX_ERR_INVALID, "cbx_spgemm_features: null output");
+  return spgemm_features(A, B, feats, (hipStream_t)stream);
 }

@@ -22,6 +22,9 @@ cbx_status dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
 void release_cache();
 
+cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, hipStream_t s);
+cbx_status spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, hipStream_t s);
+
 cbx_status spgemm_dispatch(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring sr, cbx_spgemm_mode mode,
                            const SpgemmConfig& cfg, cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s);
 

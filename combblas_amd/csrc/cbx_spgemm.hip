@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "cbx_device.h"
+#include "cbx_flatten.h"
 #include "cbx_semiring.h"
 #include "cbx_spgemm.h"
 
@@ -109,14 +110,6 @@ struct Unit {
   int32_t off;     // offset of the unit's first output within the column
 };
 
-// Per-chunk segment staging for block-level flattening of products.
-template <class T, int NT>
-struct SegStage {
-  int off[NT + 1];
-  int64_t base[NT];
-  T bval[NT];
-};
-
 // Segment of B entry e restricted to P tiles [t_lo, t_hi) (or the whole A
 // column when `full`).
 template <class T, bool VAL>
@@ -141,76 +134,6 @@ __device__ __forceinline__ void load_seg(int64_t e, int64_t b1, const int32_t* _
       }
       if constexpr (VAL) bv = valB[e];
     }
-  }
-}
-
-// Stages one chunk (at most one segment per thread): the non-empty segments
-// are compacted in order with one packed 64-bit block scan (high half counts
-// segments, low half sums lengths), so staged offsets strictly increase.
-// Returns the number of products; `nseg` receives the number of segments.
-// Contains barriers.
-template <class T, int NT>
-__device__ __forceinline__ int stage_chunk(SegStage<T, NT>& st, long long* wsum, int64_t base, int len, T bv,
-                                           int& nseg) {
-  const long long packed = len > 0 ? ((1ll << 32) | (long long)(unsigned)len) : 0ll;
-  long long tot;
-  const long long ex = block_excl_scan_ll<NT>(packed, wsum, tot);
-  if (len > 0) {
-    const int k = (int)(ex >> 32);
-    st.off[k] = (int)(ex & 0xffffffffll);
-    st.base[k] = base;
-    st.bval[k] = bv;
-  }
-  nseg = (int)(tot >> 32);
-  const int total = (int)(tot & 0xffffffffll);
-  if (threadIdx.x == 0) st.off[nseg] = total;
-  __syncthreads();
-  return total;
-}
-
-// Visits all `tot` staged products.  Each thread takes U consecutive
-// products per round: one binary search for the first, then (segments being
-// non-empty) at most one step forward per product.  Loads are issued for all
-// U products before f runs.  f(idx, bv) gets idx = -1 for lanes past the end.
-// No barriers.
-// Products xb .. xb+U-1 of a staged chunk: A-entry index (-1 past the end)
-// and B value of each.
-template <class T, int NT, int U>
-__device__ __forceinline__ void fill_round(const SegStage<T, NT>& st, int nseg, int tot, int xb, int64_t (&idx)[U],
-                                           T (&bvs)[U]) {
-#pragma unroll
-  for (int u = 0; u < U; ++u) idx[u] = -1;
-  if (xb >= tot) return;
-  int lo = 0, hi = nseg;  // largest k with off[k] <= xb (off[nseg] = tot > xb)
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (st.off[mid] <= xb) lo = mid;
-    else hi = mid;
-  }
-  int k = lo;
-  int cur = st.off[k], nxt = st.off[k + 1];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int x = xb + u;
-    if (x < tot) {
-      if (x >= nxt) {
-        ++k;
-        cur = nxt;
-        nxt = st.off[k + 1];
-      }
-      idx[u] = st.base[k] + (x - cur);
-      bvs[u] = st.bval[k];
-    }
-  }
-}
-
-template <class T, int NT, int U, class F>
-__device__ __forceinline__ void visit_staged(const SegStage<T, NT>& st, int nseg, int tot, F&& f, int first_round = 0) {
-  for (int xb = (first_round * NT + (int)threadIdx.x) * U; xb < tot; xb += NT * U) {
-    int64_t idx[U];
-    T bvs[U];
-    fill_round<T, NT, U>(st, nseg, tot, xb, idx, bvs);
-    f(idx, bvs);
   }
 }
 
@@ -999,7 +922,8 @@ constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outp
 
 template <int K, class T>
 static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_mode mode, const SpgemmConfig& cfg,
-                               cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s) {
+                               cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s,
+                               int64_t* symbolic_out = nullptr) {
   const int64_t m = A->m, n = B->n;
   const int64_t nzcA = A->nzc, nzcB = B->nzc, nnzA = A->nnz, nnzB = B->nnz;
   Timer tm(stats != nullptr);
@@ -1013,6 +937,11 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (stats) std::memset(stats, 0, sizeof(*stats));
 
   if (nnzA == 0 || nnzB == 0 || nzcA == 0 || nzcB == 0) {  // mtSpGEMM.h:224-227
+    if (symbolic_out) {
+      if (nzcB) CBX_HIP(hipMemsetAsync(symbolic_out, 0, sizeof(int64_t) * nzcB, s));
+      CBX_HIP(hipStreamSynchronize(s));
+      return CBX_OK;
+    }
     CBX_TRY(dev_alloc((void**)&C->cp, sizeof(int64_t)));
     CBX_HIP(hipMemsetAsync(C->cp, 0, sizeof(int64_t), s));
     CBX_HIP(hipStreamSynchronize(s));
@@ -1141,8 +1070,10 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_HIP(hipMemcpyAsync(hu, ucnt, sizeof(hu), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   tm.mark(2, s);
+  if (symbolic_out) CBX_HIP(hipMemcpyAsync(symbolic_out, nnzcol, sizeof(int64_t) * nzcB, hipMemcpyDeviceToDevice, s));
   CBX_HIP(hipStreamSynchronize(s));
   if (herr) return set_error(CBX_ERR_HIP, "cbx_spgemm: symbolic pass failed (flag %d)", herr);
+  if (symbolic_out) return CBX_OK;
 
   // --- output allocation (cached: repeated products reuse the arrays)
   CBX_TRY(dev_alloc((void**)&C->cp, sizeof(int64_t) * (nzcB + 1)));
@@ -1211,6 +1142,83 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     stats->nnz_main = nnz_main;
   }
   return CBX_OK;
+}
+
+// Predictor features per stored column of B (one wave per column); see
+// cbx_spgemm_features in cbx.h for the list.
+__global__ void k_features(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
+                           const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA, int64_t m,
+                           float* __restrict__ feats) {
+  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  if (col >= nzcB) return;
+  const int lane = lane_id();
+  long long f = 0, mx = 0, lo = m, hi = -1;
+  for (int64_t e = cpB[col] + lane; e < cpB[col + 1]; e += kWave) {
+    const int32_t p = bpos[e];
+    if (p < 0) continue;
+    const int64_t a0 = cpA[p], a1 = cpA[p + 1];
+    if (a1 == a0) continue;
+    const long long len = a1 - a0;
+    f += len;
+    mx = len > mx ? len : mx;
+    const long long r0 = rowsA[a0], r1 = rowsA[a1 - 1];
+    lo = r0 < lo ? r0 : lo;
+    hi = r1 > hi ? r1 : hi;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    f += __shfl_xor(f, d, kWave);
+    const long long m2 = __shfl_xor(mx, d, kWave);
+    mx = m2 > mx ? m2 : mx;
+    const long long l2 = __shfl_xor(lo, d, kWave);
+    lo = l2 < lo ? l2 : lo;
+    const long long h2 = __shfl_xor(hi, d, kWave);
+    hi = h2 > hi ? h2 : hi;
+  }
+  if (lane != 0) return;
+  const double b = (double)(cpB[col + 1] - cpB[col]);
+  const double span = f > 0 ? (double)(hi - lo + 1) : 0.0;
+  const double est = span > 0 ? span * (1.0 - exp(-(double)f / span)) : 0.0;
+  float* o = feats + col * CBX_MLP_FEATURES;
+  o[0] = (float)log2(1.0 + b);
+  o[1] = (float)log2(1.0 + (double)f);
+  o[2] = (float)log2(1.0 + (double)mx);
+  o[3] = (float)log2(1.0 + (double)f / (b > 0 ? b : 1.0));
+  o[4] = (float)(span / (double)(m > 0 ? m : 1));
+  o[5] = (float)log2(1.0 + est);
+}
+
+cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, hipStream_t s) {
+  const int64_t nzcA = A->nzc, nzcB = B->nzc, nnzA = A->nnz, nnzB = B->nnz;
+  if (nzcB == 0) return CBX_OK;
+  DevBuf b_colmap, b_rows, b_bpos, b_err;
+  int32_t *colmap, *rowsA, *bpos;
+  int* err;
+  CBX_TRY(dalloc(b_colmap, A->n, s, &colmap));
+  CBX_TRY(dalloc(b_rows, nnzA, s, &rowsA));
+  CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
+  CBX_TRY(dalloc(b_err, 1, s, &err));
+  CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  k_validate<<<nblk(nzcA * kWave, 256), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
+  k_validate<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
+  int herr = 0;
+  CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  if (herr) return set_error(CBX_ERR_INVALID, "cbx_spgemm_features: operand is not a valid DCSC");
+  k_fill_i32<<<nblk(A->n, 256), 256, 0, s>>>(colmap, A->n, -1);
+  if (nzcA) k_colmap<<<nblk(nzcA, 256), 256, 0, s>>>(A->jc, nzcA, colmap);
+  if (nnzA) k_rows32<<<nblk(nnzA, 256), 256, 0, s>>>(A->ir, nnzA, rowsA);
+  if (nnzB) k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
+  k_features<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, A->m, feats);
+  CBX_HIP(hipGetLastError());
+  CBX_HIP(hipStreamSynchronize(s));
+  return CBX_OK;
+}
+
+cbx_status spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, hipStream_t s) {
+  cbx_dcsc C;
+  SpgemmConfig cfg;
+  return spgemm_typed<kPlusTimes, int64_t>(A, B, CBX_SPGEMM_TWO_PHASE, cfg, &C, nullptr, s, nnz);
 }
 
 cbx_status spgemm_dispatch(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring sr, cbx_spgemm_mode mode,

@@ -1,0 +1,506 @@
+// Sparse matrix times sparse vector (SpMSpV) and top-down BFS for gfx950.
+//
+// cbx_spmspv replaces the local y = A*x of CombBLAS's SpMV with a sparse
+// vector: SpImpl<SR,IT,NUM,IVT,OVT>::SpMXSpV (SpImpl.cpp:55-158) and its
+// boolean-matrix form SpImpl<SR,IT,bool,IVT,OVT>::SpMXSpV (SpImpl.cpp:167-213),
+// reached through generic_gespmv (Friends.h:448-482).  Output indices are
+// ascending, an entry exists wherever some A(i,k) x_k term exists, and a
+// boolean A contributes x_k itself (the boolean specialisation never calls
+// multiply, SpImpl.cpp:183-188).  The GPU form is a dense accumulator over the
+// m rows with a presence bitmap; products are block-flattened across the
+// frontier's columns, and the output is compacted in row order by a scan of
+// the bitmap's word counts.
+//
+// The BFS plan runs the level loop of TopDownBFS.cpp:442-448 with
+// SelectMaxSRing<bool,int64> (parent = largest frontier neighbour):
+//   fringe.setNumToInd(); fringe = SpMV(A, fringe); fringe = EWiseMult(fringe,
+//   parents, exclude=true, -1); parents.Set(fringe);
+// fused per level into two kernels: expand (every frontier vertex k offers k to
+// each unvisited neighbour with an atomic max; the first offer appends the
+// neighbour to the next frontier) and finalize (parents = offers; mark
+// visited).  Each level is one captured hipGraph (two graphs alternate the
+// frontier buffers); the host reads only the next frontier's size.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+
+#include "cbx_device.h"
+#include "cbx_flatten.h"
+#include "cbx_semiring.h"
+#include "cbx_spgemm.h"
+
+namespace cbx {
+
+// ----------------------------------------------- global-memory accumulation
+template <int K, class T>
+struct GAcc;
+template <>
+struct GAcc<kPlusTimes, int64_t> {
+  static __device__ void acc(int64_t* p, int64_t v) { atomicAdd((unsigned long long*)p, (unsigned long long)v); }
+};
+template <>
+struct GAcc<kPlusTimes, double> {
+  static __device__ void acc(double* p, double v) { atomicAdd(p, v); }
+};
+template <>
+struct GAcc<kPlusTimes, float> {
+  static __device__ void acc(float* p, float v) { atomicAdd(p, v); }
+};
+template <>
+struct GAcc<kMinPlus, int64_t> {
+  static __device__ void acc(int64_t* p, int64_t v) { atomicMin((long long*)p, (long long)v); }
+};
+template <>
+struct GAcc<kSelectMax, int64_t> {
+  static __device__ void acc(int64_t* p, int64_t v) { atomicMax((long long*)p, (long long)v); }
+};
+template <class F, class U>
+__device__ void gcas_minmax(F* p, F v, bool is_min) {
+  U* up = reinterpret_cast<U*>(p);
+  U old = __hip_atomic_load(up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    const F cur = __builtin_bit_cast(F, old);
+    if (!(is_min ? (v < cur) : (cur < v))) return;
+    if (__hip_atomic_compare_exchange_strong(up, &old, __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+template <>
+struct GAcc<kMinPlus, double> {
+  static __device__ void acc(double* p, double v) { gcas_minmax<double, unsigned long long>(p, v, true); }
+};
+template <>
+struct GAcc<kMinPlus, float> {
+  static __device__ void acc(float* p, float v) { gcas_minmax<float, unsigned int>(p, v, true); }
+};
+template <>
+struct GAcc<kSelectMax, double> {
+  static __device__ void acc(double* p, double v) { gcas_minmax<double, unsigned long long>(p, v, false); }
+};
+template <>
+struct GAcc<kSelectMax, float> {
+  static __device__ void acc(float* p, float v) { gcas_minmax<float, unsigned int>(p, v, false); }
+};
+
+namespace {
+
+__global__ void k_sv_colmap(const int64_t* __restrict__ jc, int64_t nzc, int64_t n, int32_t* __restrict__ colmap) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) colmap[i] = -1;
+}
+__global__ void k_sv_colmap_scatter(const int64_t* __restrict__ jc, int64_t nzc, int32_t* __restrict__ colmap) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < nzc) colmap[jc[p]] = (int32_t)p;
+}
+
+template <class T>
+__global__ void k_sv_init(T* __restrict__ acc, int64_t m, T ident, uint32_t* __restrict__ bits, int64_t nw) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < m) acc[i] = ident;
+  if (i < nw) bits[i] = 0u;
+}
+
+// x entries must be valid column indices, ascending and unique.
+__global__ void k_sv_check_x(const int32_t* __restrict__ xind, int32_t nx, int64_t n, int* __restrict__ bad) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nx) return;
+  const int32_t k = xind[t];
+  if (k < 0 || k >= n || (t + 1 < nx && xind[t + 1] <= k)) atomicOr(bad, 1);
+}
+
+// y_acc[i] (+)= A(i,k) (x) x_k for k in x; presence bit per row.
+template <int K, class T, bool BOOLA, int NT, int U>
+__global__ __launch_bounds__(NT) void k_sv_expand(const int64_t* __restrict__ cpA, const int64_t* __restrict__ irA,
+                                                  const void* __restrict__ valA_, const int32_t* __restrict__ colmap,
+                                                  const int32_t* __restrict__ xind, const T* __restrict__ xval,
+                                                  int32_t nx, T* __restrict__ acc, uint32_t* __restrict__ bits) {
+  using SR = Sem<K, T>;
+  __shared__ SegStage<T, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  const T* valA = (const T*)valA_;
+  for (int64_t c0 = (int64_t)blockIdx.x * NT; c0 < nx; c0 += (int64_t)gridDim.x * NT) {
+    const int64_t t = c0 + threadIdx.x;
+    int64_t base = 0;
+    int len = 0;
+    T xv{};
+    if (t < nx) {
+      const int32_t p = colmap[xind[t]];
+      if (p >= 0) {
+        base = cpA[p];
+        len = (int)(cpA[p + 1] - base);
+        xv = xval[t];
+      }
+    }
+    int nseg;
+    const int tot = stage_chunk<T, NT>(st, wsum, base, len, xv, nseg);
+    visit_staged<T, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const T(&xs)[U]) {
+      int64_t r[U];
+      T a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        r[u] = -1;
+        if (idx[u] >= 0) {
+          r[u] = irA[idx[u]];
+          if constexpr (!BOOLA) a[u] = valA[idx[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (r[u] < 0) continue;
+        T v;
+        if constexpr (BOOLA) v = xs[u];
+        else v = (T)SR::out(SR::mul(a[u], xs[u]));
+        GAcc<K, T>::acc(&acc[r[u]], v);
+        atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+      }
+    });
+  }
+}
+
+__global__ void k_sv_wordcnt(const uint32_t* __restrict__ bits, int64_t nw, int32_t* __restrict__ cnt) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w < nw) cnt[w] = __popc(bits[w]);
+}
+
+template <class T>
+__global__ void k_sv_emit(const uint32_t* __restrict__ bits, const int32_t* __restrict__ off, int64_t nw,
+                          const T* __restrict__ acc, int32_t* __restrict__ yind, T* __restrict__ yval) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  uint32_t b = bits[w];
+  int32_t q = off[w];
+  while (b) {
+    const int j = __ffs(b) - 1;
+    b &= b - 1;
+    const int32_t i = (int32_t)(w * 32 + j);
+    yind[q] = i;
+    yval[q] = acc[i];
+    ++q;
+  }
+}
+
+inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int K, class T>
+T ident_of() {
+  if (K == kPlusTimes) return std::is_floating_point<T>::value ? (T)-0.0 : (T)0;
+  if (K == kMinPlus) return std::numeric_limits<T>::max();
+  return std::numeric_limits<T>::lowest();
+}
+
+template <int K, class T>
+cbx_status spmspv_typed(const cbx_dcsc* A, bool boolA, const int32_t* xind, const T* xval, int32_t nx, int32_t* yind,
+                        T* yval, int32_t* ny, hipStream_t s) {
+  const int64_t m = A->m, n = A->n, nw = m / 32 + 1;
+  *ny = 0;
+  if (nx == 0 || A->nnz == 0) return CBX_OK;
+  int32_t *colmap = nullptr, *cnt = nullptr, *off = nullptr;
+  T* acc = nullptr;
+  uint32_t* bits = nullptr;
+  int* bad = nullptr;
+  void* tmp = nullptr;
+  size_t tb = 0;
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, off, (int)(nw + 1), s));
+  auto cleanup = [&]() {
+    for (void* p : {(void*)colmap, (void*)cnt, (void*)off, (void*)acc, (void*)bits, (void*)bad, tmp})
+      if (p) (void)hipFreeAsync(p, s);
+  };
+  if (hipMallocAsync(&colmap, sizeof(int32_t) * std::max<int64_t>(n, 1), s) != hipSuccess ||
+      hipMallocAsync(&cnt, sizeof(int32_t) * (nw + 1), s) != hipSuccess ||
+      hipMallocAsync(&off, sizeof(int32_t) * (nw + 1), s) != hipSuccess ||
+      hipMallocAsync(&acc, sizeof(T) * std::max<int64_t>(m, 1), s) != hipSuccess ||
+      hipMallocAsync(&bits, sizeof(uint32_t) * nw, s) != hipSuccess ||
+      hipMallocAsync(&bad, sizeof(int), s) != hipSuccess || hipMallocAsync(&tmp, std::max<size_t>(tb, 16), s) != hipSuccess) {
+    cleanup();
+    return set_error(CBX_ERR_NOMEM, "cbx_spmspv: workspace allocation failed");
+  }
+  int hbad = 0;
+  CBX_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  k_sv_check_x<<<nb(nx, 256), 256, 0, s>>>(xind, nx, n, bad);
+  CBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  if (hbad) {
+    cleanup();
+    return set_error(CBX_ERR_INVALID, "cbx_spmspv: x indices must be ascending, unique and below ncols");
+  }
+  k_sv_colmap<<<nb(n, 256), 256, 0, s>>>(A->jc, A->nzc, n, colmap);
+  k_sv_colmap_scatter<<<nb(A->nzc, 256), 256, 0, s>>>(A->jc, A->nzc, colmap);
+  k_sv_init<T><<<nb(std::max(m, nw), 256), 256, 0, s>>>(acc, m, ident_of<K, T>(), bits, nw);
+  const unsigned g = std::min<unsigned>(nb(nx, 256), 2048u);
+  if (boolA)
+    k_sv_expand<K, T, true, 256, 4><<<g, 256, 0, s>>>(A->cp, A->ir, A->val, colmap, xind, xval, nx, acc, bits);
+  else
+    k_sv_expand<K, T, false, 256, 4><<<g, 256, 0, s>>>(A->cp, A->ir, A->val, colmap, xind, xval, nx, acc, bits);
+  k_sv_wordcnt<<<nb(nw, 256), 256, 0, s>>>(bits, nw, cnt);
+  CBX_HIP(hipMemsetAsync(cnt + nw, 0, sizeof(int32_t), s));
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, off, (int)(nw + 1), s));
+  k_sv_emit<T><<<nb(nw, 256), 256, 0, s>>>(bits, off, nw, acc, yind, yval);
+  CBX_HIP(hipMemcpyAsync(ny, off + nw, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipGetLastError());
+  CBX_HIP(hipStreamSynchronize(s));
+  cleanup();
+  CBX_HIP(hipStreamSynchronize(s));
+  return CBX_OK;
+}
+
+}  // namespace
+
+// ====================================================================== BFS
+namespace {
+
+__global__ void k_bfs_dense_cp(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc, int64_t nzc,
+                               int64_t n, int64_t* __restrict__ beg, int64_t* __restrict__ end) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p < nzc) {
+    beg[jc[p]] = cp[p];
+    end[jc[p]] = cp[p + 1];
+  }
+}
+
+__global__ void k_bfs_rows32(const int64_t* __restrict__ ir, int64_t nnz, int32_t* __restrict__ rows) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nnz) rows[i] = (int32_t)ir[i];
+}
+
+__global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __restrict__ parents,
+                            uint32_t* __restrict__ visited, int64_t nw, int32_t* __restrict__ f0,
+                            int32_t* __restrict__ counts, int64_t root) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) {
+    cand[i] = -1;
+    parents[i] = -1;
+  }
+  if (i < nw) visited[i] = 0u;
+  if (i == 0) {
+    f0[0] = (int32_t)root;
+    counts[0] = 1;
+    counts[1] = 0;
+  }
+}
+
+__global__ void k_bfs_zero(int32_t* __restrict__ c) {
+  if (threadIdx.x == 0) *c = 0;
+}
+
+// One level's SpMV(A, fringe) with SelectMax<bool,int64> fused with the
+// EWiseMult(exclude) filter: each frontier vertex k offers k to its unvisited
+// neighbours; cand[i] keeps the largest offer; the first offer appends i.
+template <int NT, int U>
+__global__ __launch_bounds__(NT) void k_bfs_expand(const int64_t* __restrict__ beg, const int64_t* __restrict__ end,
+                                                   const int32_t* __restrict__ rows, const int32_t* __restrict__ fin,
+                                                   const int32_t* __restrict__ fin_count,
+                                                   const uint32_t* __restrict__ visited, long long* __restrict__ cand,
+                                                   int32_t* __restrict__ fout, int32_t* __restrict__ fout_count) {
+  __shared__ SegStage<int32_t, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  const int nf = *fin_count;
+  for (int c0 = blockIdx.x * NT; c0 < nf; c0 += gridDim.x * NT) {
+    const int t = c0 + threadIdx.x;
+    int64_t base = 0;
+    int len = 0;
+    int32_t k = 0;
+    if (t < nf) {
+      k = fin[t];
+      base = beg[k];
+      len = (int)(end[k] - base);
+    }
+    int nseg;
+    const int tot = stage_chunk<int32_t, NT>(st, wsum, base, len, k, nseg);
+    visit_staged<int32_t, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const int32_t(&ks)[U]) {
+      int32_t r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rows[idx[u]] : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        bool push = false;
+        if (r[u] >= 0 && !((visited[r[u] >> 5] >> (r[u] & 31)) & 1u)) {
+          const long long old = atomicMax(&cand[r[u]], (long long)ks[u]);
+          push = old == -1;
+        }
+        // one atomic per wave for the appends of this step
+        const uint64_t mask = __ballot(push);
+        if (mask) {
+          const int leader = __ffsll((unsigned long long)mask) - 1;
+          int basei = 0;
+          if (lane_id() == leader) basei = atomicAdd(fout_count, __popcll(mask));
+          basei = __shfl(basei, leader, kWave);
+          if (push) {
+            const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+            fout[basei + __popcll(below)] = r[u];
+          }
+        }
+      }
+    });
+  }
+}
+
+// parents.Set(fringe) (FullyDistVec.cpp:1036) and the visited mark.
+__global__ void k_bfs_finalize(const int32_t* __restrict__ f, const int32_t* __restrict__ count,
+                               const long long* __restrict__ cand, int64_t* __restrict__ parents,
+                               uint32_t* __restrict__ visited) {
+  const int nf = *count;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x) {
+    const int32_t i = f[t];
+    parents[i] = cand[i];
+    atomicOr(&visited[i >> 5], 1u << (i & 31));
+  }
+}
+
+}  // namespace
+
+}  // namespace cbx
+
+using namespace cbx;
+
+struct cbx_bfs_plan_s {
+  int64_t n = 0, nnz = 0, nw = 0;
+  hipStream_t stream = nullptr;
+  int64_t *beg = nullptr, *end = nullptr, *parents = nullptr;
+  int32_t *rows = nullptr, *f[2] = {nullptr, nullptr}, *counts = nullptr;
+  long long* cand = nullptr;
+  uint32_t* visited = nullptr;
+  int32_t* host_count = nullptr;  // pinned
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+static void bfs_plan_free(cbx_bfs_plan_s* p) {
+  if (!p) return;
+  for (int i = 0; i < 2; ++i) {
+    if (p->exec[i]) (void)hipGraphExecDestroy(p->exec[i]);
+    if (p->graph[i]) (void)hipGraphDestroy(p->graph[i]);
+    if (p->f[i]) (void)hipFree(p->f[i]);
+  }
+  for (void* q : {(void*)p->beg, (void*)p->end, (void*)p->parents, (void*)p->rows, (void*)p->counts, (void*)p->cand,
+                  (void*)p->visited})
+    if (q) (void)hipFree(q);
+  if (p->host_count) (void)hipHostFree(p->host_count);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+constexpr int kBfsNT = 256;
+constexpr unsigned kBfsGrid = 2048;
+
+extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** out) {
+  CBX_RETURN_IF(!A || !out, CBX_ERR_INVALID, "cbx_bfs_plan_create: null argument");
+  *out = nullptr;
+  CBX_RETURN_IF(A->location != CBX_DEVICE, CBX_ERR_INVALID, "cbx_bfs_plan_create: expected a device matrix");
+  CBX_RETURN_IF(A->m != A->n, CBX_ERR_INVALID, "cbx_bfs_plan_create: the adjacency matrix must be square");
+  CBX_RETURN_IF(A->n >= (1ll << 31) || A->nnz >= (1ll << 31), CBX_ERR_LIMIT,
+                "cbx_bfs_plan_create: n and nnz must be below 2^31");
+  cbx_status st = cbx_dcsc_check(A);
+  if (st != CBX_OK) return st;
+  auto* p = new cbx_bfs_plan_s();
+  p->n = A->n;
+  p->nnz = A->nnz;
+  p->nw = A->n / 32 + 1;
+  auto fail = [&](const char* what) {
+    bfs_plan_free(p);
+    return set_error(CBX_ERR_HIP, "cbx_bfs_plan_create: %s", what);
+  };
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+  const int64_t n = p->n;
+  if (hipMalloc(&p->beg, 8 * std::max<int64_t>(n, 1)) || hipMalloc(&p->end, 8 * std::max<int64_t>(n, 1)) ||
+      hipMalloc(&p->parents, 8 * std::max<int64_t>(n, 1)) || hipMalloc(&p->rows, 4 * std::max<int64_t>(p->nnz, 1)) ||
+      hipMalloc(&p->f[0], 4 * std::max<int64_t>(n, 1)) || hipMalloc(&p->f[1], 4 * std::max<int64_t>(n, 1)) ||
+      hipMalloc(&p->counts, 16) || hipMalloc(&p->cand, 8 * std::max<int64_t>(n, 1)) ||
+      hipMalloc(&p->visited, 4 * p->nw) || hipHostMalloc(&p->host_count, 16, hipHostMallocDefault))
+    return fail("allocation");
+  hipStream_t s = p->stream;
+  if (hipMemsetAsync(p->beg, 0, 8 * n, s) || hipMemsetAsync(p->end, 0, 8 * n, s)) return fail("memset");
+  k_bfs_dense_cp<<<nb(A->nzc, 256), 256, 0, s>>>(A->cp, A->jc, A->nzc, n, p->beg, p->end);
+  k_bfs_rows32<<<nb(A->nnz, 256), 256, 0, s>>>(A->ir, A->nnz, p->rows);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail("setup kernels");
+  // one graph per frontier parity: clear the next count, expand, finalize,
+  // copy the next count to pinned host memory
+  for (int par = 0; par < 2; ++par) {
+    int32_t* cin = p->counts + par;
+    int32_t* cout = p->counts + (1 - par);
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail("capture begin");
+    k_bfs_zero<<<1, 64, 0, s>>>(cout);
+    k_bfs_expand<kBfsNT, 4><<<kBfsGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->f[par], cin, p->visited, p->cand,
+                                                       p->f[1 - par], cout);
+    k_bfs_finalize<<<1024, 256, 0, s>>>(p->f[1 - par], cout, p->cand, p->parents, p->visited);
+    (void)hipMemcpyAsync(p->host_count, cout, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (hipStreamEndCapture(s, &p->graph[par]) != hipSuccess) return fail("capture end");
+    if (hipGraphInstantiate(&p->exec[par], p->graph[par], nullptr, nullptr, 0) != hipSuccess)
+      return fail("graph instantiate");
+  }
+  *out = p;
+  return CBX_OK;
+}
+
+extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* parents_dev, cbx_bfs_stats* stats) {
+  CBX_RETURN_IF(!p || !parents_dev, CBX_ERR_INVALID, "cbx_bfs_run: null argument");
+  CBX_RETURN_IF(root < 0 || root >= p->n, CBX_ERR_INVALID, "cbx_bfs_run: root %lld out of range", (long long)root);
+  hipStream_t s = p->stream;
+  hipEvent_t e0, e1;
+  CBX_HIP(hipEventCreate(&e0));
+  CBX_HIP(hipEventCreate(&e1));
+  CBX_HIP(hipEventRecord(e0, s));
+  k_bfs_reset<<<nb(std::max(p->n, p->nw), 256), 256, 0, s>>>(p->n, p->cand, p->parents, p->visited, p->nw, p->f[0],
+                                                             p->counts, root);
+  int levels = 0;
+  int64_t reached = 0;
+  for (int par = 0;; par ^= 1) {
+    CBX_HIP(hipGraphLaunch(p->exec[par], s));
+    CBX_HIP(hipStreamSynchronize(s));
+    ++levels;
+    const int32_t next = *(volatile int32_t*)p->host_count;
+    reached += next;
+    if (next == 0 || levels > p->n + 1) break;
+  }
+  CBX_HIP(hipMemcpyAsync(parents_dev, p->parents, sizeof(int64_t) * p->n, hipMemcpyDeviceToDevice, s));
+  CBX_HIP(hipEventRecord(e1, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (stats) {
+    stats->levels = levels;
+    stats->reached = reached;
+    stats->ms = ms;
+  }
+  return CBX_OK;
+}
+
+extern "C" void cbx_bfs_plan_destroy(cbx_bfs_plan_s* p) { bfs_plan_free(p); }
+
+extern "C" cbx_status cbx_spmspv(const cbx_dcsc* A, const int32_t* xind, const void* xval, int32_t nx,
+                                 cbx_semiring semiring, cbx_dtype xdtype, int32_t* yind, void* yval, int32_t* ny,
+                                 void* stream) {
+  CBX_RETURN_IF(!A || !ny || (nx > 0 && (!xind || !xval)) || !yind || !yval, CBX_ERR_INVALID,
+                "cbx_spmspv: null argument");
+  CBX_RETURN_IF(A->location != CBX_DEVICE, CBX_ERR_INVALID, "cbx_spmspv: expected a device matrix");
+  CBX_RETURN_IF(A->m >= (1ll << 31) || A->n >= (1ll << 31), CBX_ERR_LIMIT, "cbx_spmspv: dimensions must be below 2^31");
+  CBX_RETURN_IF(nx < 0 || nx > A->n, CBX_ERR_INVALID, "cbx_spmspv: bad vector length");
+  const bool boolA = A->dtype == CBX_BOOL;
+  CBX_RETURN_IF(!boolA && A->dtype != xdtype, CBX_ERR_UNSUPPORTED,
+                "cbx_spmspv: A must be boolean or share the vector's dtype");
+  cbx_status st = cbx_dcsc_check(A);  // the kernels index by A's rows and columns
+  if (st != CBX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  switch (semiring * 8 + xdtype) {
+#define CBX_SV(SRK, SRE, T, DT)                                                                                     \
+  case SRE * 8 + DT:                                                                                                \
+    return spmspv_typed<SRK, T>(A, boolA, xind, (const T*)xval, nx, yind, (T*)yval, ny, s);
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, int64_t, CBX_I64)
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, double, CBX_F64)
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, float, CBX_F32)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, int64_t, CBX_I64)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, double, CBX_F64)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, float, CBX_F32)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, int64_t, CBX_I64)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, double, CBX_F64)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, float, CBX_F32)
+#undef CBX_SV
+  }
+  return set_error(CBX_ERR_UNSUPPORTED, "cbx_spmspv: semiring %d with vector dtype %d is not provided", (int)semiring,
+                   (int)xdtype);
+}

@@ -1,0 +1,118 @@
+"""Per-column nnz predictor for the learned single-pass SpGEMM.
+
+Training data are (features, exact nnz) pairs that the two-phase path's own
+symbolic pass produces on generated R-MAT products A*A: 32 matrices, 8 at
+each scale 16..19, edge factor 16, seeds 1000+i.  The network (6 -> 64 -> 64
+-> 1, ReLU, FP32) is trained on the GPU by the library's own kernels
+(cbx_mlp_train) and predicts log2(1 + nnz_j).  Trained parameters are kept
+under combblas_amd/models/ with a JSON record of how they were made.
+
+    python -m combblas_amd.predictor train     # on a GPU machine
+"""
+from __future__ import annotations
+
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+import combblas_amd as cb
+
+MODEL_DIR = Path(__file__).resolve().parent / "models"
+MODEL_PATH = MODEL_DIR / "nnz_mlp_rmat.f32"
+META_PATH = MODEL_DIR / "nnz_mlp_rmat.json"
+
+TRAIN_SCALES = (16, 17, 18, 19)
+PER_SCALE = 8
+EDGE_FACTOR = 16
+
+
+def training_matrix(i: int) -> tuple[int, int]:
+    """(scale, seed) of training matrix i (0..31)."""
+    return TRAIN_SCALES[i // PER_SCALE], 1000 + i
+
+
+def pairs_for(A: cb.Dcsc) -> tuple[np.ndarray, np.ndarray]:
+    """(features float32[n,6], log2(1+nnz) float32[n]) for the columns of A*A."""
+    dA = A.to_device()
+    X = cb.spgemm_features(dA, dA)
+    nnz = cb.spgemm_symbolic(dA, dA)
+    keep = nnz > 0
+    return X[keep], np.log2(1.0 + nnz[keep]).astype(np.float32)
+
+
+def collect(count: int = len(TRAIN_SCALES) * PER_SCALE, log=print) -> tuple[np.ndarray, np.ndarray]:
+    Xs, ys = [], []
+    for i in range(count):
+        scale, seed = training_matrix(i)
+        A = cb.gen_rmat(scale, EDGE_FACTOR, seed=seed)
+        X, y = pairs_for(A)
+        Xs.append(X)
+        ys.append(y)
+        log(f"matrix {i:2d}: scale {scale} seed {seed}: {len(y)} columns")
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def init_params(X: np.ndarray, seed: int = 1) -> np.ndarray:
+    """He-initialised weights, zero biases, feature mean / std from X."""
+    F, H = cb.MLP_FEATURES, cb.MLP_HIDDEN
+    rng = np.random.default_rng(seed)
+    sd = X.std(0)
+    sd[sd < 1e-6] = 1.0
+    parts = [rng.normal(0, np.sqrt(2 / F), (H, F)), np.zeros(H), rng.normal(0, np.sqrt(2 / H), (H, H)), np.zeros(H),
+             rng.normal(0, np.sqrt(1 / H), H), np.zeros(1), X.mean(0), sd]
+    return np.concatenate([np.asarray(p, np.float64).reshape(-1) for p in parts]).astype(np.float32)
+
+
+def relative_error(params: np.ndarray, X: np.ndarray, y: np.ndarray) -> dict:
+    """Prediction quality on (X, y): errors of nnz = 2^y - 1."""
+    pred = cb.mlp_forward(params, X)
+    true_nnz = np.exp2(y.astype(np.float64)) - 1
+    pred_nnz = np.exp2(pred.astype(np.float64)) - 1
+    rel = (pred_nnz - true_nnz) / np.maximum(true_nnz, 1)
+    return {"mean_abs_rel": float(np.mean(np.abs(rel))), "median_abs_rel": float(np.median(np.abs(rel))),
+            "p95_abs_rel": float(np.quantile(np.abs(rel), 0.95)),
+            "total_rel": float((pred_nnz.sum() - true_nnz.sum()) / true_nnz.sum()),
+            "under_predicted_frac": float(np.mean(pred_nnz < true_nnz))}
+
+
+def train(steps: int = 4000, batch: int = 4096, lr: float = 2e-3, seed: int = 7, log=print) -> dict:
+    t0 = time.time()
+    X, y = collect(log=log)
+    t1 = time.time()
+    P0 = init_params(X)
+    P, curve = cb.mlp_train(P0, X, y, steps=steps, batch=batch, lr=lr, seed=seed)
+    t2 = time.time()
+    q_train = relative_error(P, X, y)
+    A = cb.gen_rmat(20, EDGE_FACTOR)  # held out: the s20 workload itself
+    Xh, yh = pairs_for(A)
+    q_s20 = relative_error(P, Xh, yh)
+    MODEL_DIR.mkdir(exist_ok=True)
+    P.astype(np.float32).tofile(MODEL_PATH)
+    meta = {"architecture": "6-64-64-1 ReLU, FP32, target log2(1+nnz_j), MSE, Adam",
+            "training_matrices": [dict(zip(("scale", "seed"), training_matrix(i))) for i in range(32)],
+            "edge_factor": EDGE_FACTOR, "samples": int(len(y)), "steps": steps, "batch": batch, "lr": lr,
+            "seed": seed, "loss_first": float(curve[:50].mean()), "loss_last": float(curve[-50:].mean()),
+            "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "train_quality": q_train,
+            "heldout_rmat_s20_ef16": q_s20}
+    META_PATH.write_text(json.dumps(meta, indent=2) + "\n")
+    log(json.dumps(meta, indent=2))
+    return meta
+
+
+def load() -> np.ndarray:
+    if not MODEL_PATH.exists():
+        raise FileNotFoundError(f"{MODEL_PATH} missing: train it with `python -m combblas_amd.predictor train`")
+    P = np.fromfile(MODEL_PATH, np.float32)
+    if P.size != cb.MLP_PARAMS:
+        raise ValueError(f"{MODEL_PATH}: expected {cb.MLP_PARAMS} floats, found {P.size}")
+    return P
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "train":
+        train()
+    else:
+        print(__doc__)

@@ -83,6 +83,13 @@ cbx_status cbx_device_synchronize(void);
  * free(), device arrays with hipFree), then zero the struct.  NULL-safe. */
 void cbx_dcsc_free(cbx_dcsc* A);
 
+/* Raw device buffers from the library's caching allocator (for vectors passed
+ * to cbx_spmspv / cbx_bfs_run), and copies: kind 0 host->device, 1
+ * device->host, 2 device->device. */
+cbx_status cbx_malloc(void** ptr, size_t bytes);
+void cbx_free(void* ptr);
+cbx_status cbx_memcpy(void* dst, const void* src, size_t bytes, int32_t kind);
+
 /* Copy a matrix between host and device.  `dst` receives newly allocated
  * arrays owned by the caller (release with cbx_dcsc_free). */
 cbx_status cbx_dcsc_to_device(const cbx_dcsc* host_src, cbx_dcsc* dev_dst);
@@ -169,6 +176,86 @@ void cbx_release_cached_memory(void);
  * respect to the host (it returns when C is complete). */
 cbx_status cbx_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring semiring,
                       cbx_spgemm_mode mode, cbx_dcsc* C, cbx_spgemm_stats* stats, void* stream);
+
+/* -------------------------------------------------- learned nnz predictor */
+
+/* Symbolic column counts: nnz of C(:,j) for every stored column j of B
+ * (estimateNNZ_Hash, mtSpGEMM.h:806-933), written to device int64[nzc_B].
+ * These are the predictor's training labels. */
+cbx_status cbx_spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, void* stream);
+
+/* Cheap per-column features of C = A*B for every stored column j of B,
+ * device float[nzc_B][6]:
+ *   log2(1+nnz(B(:,j))), log2(1+flop_j), log2(1+max_k nnz(A(:,k))),
+ *   log2(1+flop_j/nnz(B(:,j))), row span of the contributing A columns / m,
+ *   log2(1 + span*(1-exp(-flop_j/span)))   (k ranges over B(:,j)). */
+cbx_status cbx_spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, void* stream);
+
+/* The predictor: a 6 -> 64 -> 64 -> 1 ReLU MLP in FP32 predicting
+ * log2(1 + nnz_j) from standardised features.  Parameters (device float
+ * array of CBX_MLP_PARAMS): W1[64][6] b1[64] W2[64][64] b2[64] W3[64] b3
+ * mu[6] sd[6] (feature mean and standard deviation used for scaling). */
+#define CBX_MLP_FEATURES 6
+#define CBX_MLP_HIDDEN 64
+#define CBX_MLP_PARAMS (64 * 6 + 64 + 64 * 64 + 64 + 64 + 1 + 12)
+
+typedef struct cbx_mlp_train_opts {
+  int32_t steps;  /* Adam steps */
+  int32_t batch;  /* samples per step, drawn with replacement */
+  float lr;       /* Adam learning rate */
+  int32_t pad;
+  uint64_t seed;
+} cbx_mlp_train_opts;
+
+/* out[i] = predicted log2(1 + nnz) for features X[i][6] (device arrays). */
+cbx_status cbx_mlp_forward(const float* params, const float* X, int64_t n, float* out, void* stream);
+/* Mean-squared-error gradient over all n samples (device arrays; grad gets
+ * CBX_MLP_PARAMS floats, the last 12 zero) and the loss (host). */
+cbx_status cbx_mlp_gradient(const float* params, const float* X, const float* y, int32_t n, float* grad, float* loss);
+/* Adam training on the device with the library's own kernels; loss_curve
+ * (host, optional) receives each step's mini-batch loss. */
+cbx_status cbx_mlp_train(float* params, const float* X, const float* y, int64_t n, const cbx_mlp_train_opts* opts,
+                         float* loss_curve);
+
+/* ----------------------------------------------------------------- SpMSpV */
+
+/* y = A (x) x on `semiring` with a sparse vector x (SpImpl::SpMXSpV,
+ * SpImpl.cpp:55-158 and SpImpl.cpp:167-213, via generic_gespmv,
+ * Friends.h:448-482).  A: device DCSC (m x n), dtype CBX_BOOL or `xdtype`.
+ * x: nx device indices (ascending, unique, below n) with device values of
+ * `xdtype` (CBX_I64, CBX_F64 or CBX_F32).  y: device arrays of capacity m;
+ * *ny (host) receives the number of entries, indices ascending.  An entry
+ * exists wherever some A(i,k) x_k term exists.  A boolean A contributes x_k
+ * itself (the reference's boolean specialisation never calls multiply), so
+ * SELECT_MAX on a boolean A is SelectMaxSRing<bool,T> (Semirings.h:193-210). */
+cbx_status cbx_spmspv(const cbx_dcsc* A, const int32_t* xind, const void* xval, int32_t nx, cbx_semiring semiring,
+                      cbx_dtype xdtype, int32_t* yind, void* yval, int32_t* ny, void* stream);
+
+/* -------------------------------------------------------------------- BFS */
+
+/* Top-down BFS of TopDownBFS.cpp:442-448 on one GPU:
+ *   fringe.setNumToInd(); fringe = SpMV<SelectMaxSRing<bool,int64>>(A, fringe);
+ *   fringe = EWiseMult(fringe, parents, true, -1); parents.Set(fringe);
+ * until the fringe is empty.  parents[v] is the largest-index frontier
+ * neighbour through which v was first reached; the root's own entry is set
+ * when a later level reaches it back (as in the reference), vertices never
+ * reached stay -1.  A plan holds the adjacency matrix in the layout the
+ * kernels read plus one captured hipGraph per frontier parity; it is built
+ * once and run for many roots, like the reference's prepared matrix. */
+typedef struct cbx_bfs_plan_s* cbx_bfs_plan;
+
+typedef struct cbx_bfs_stats {
+  int32_t levels;   /* SpMV iterations (TopDownBFS's "Number iterations") */
+  int32_t pad;
+  int64_t reached;  /* vertices that received a parent */
+  double ms;        /* device time of the whole BFS */
+} cbx_bfs_stats;
+
+/* A: device DCSC, square, symmetric pattern (values ignored). */
+cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan* plan);
+/* parents: device int64[n], overwritten. */
+cbx_status cbx_bfs_run(cbx_bfs_plan plan, int64_t root, int64_t* parents, cbx_bfs_stats* stats);
+void cbx_bfs_plan_destroy(cbx_bfs_plan plan);
 
 #ifdef __cplusplus
 }

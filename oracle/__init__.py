@@ -56,6 +56,11 @@ def _load_oracle() -> C.CDLL:
         L.orc_time_spgemm.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, C.c_int64, C.c_int64,
                                       P(C.c_double), P(C.c_int64), P(C.c_int64)]
         L.orc_time_spgemm.restype = C.c_int32
+        L.orc_spmspv.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                 C.c_void_p]
+        L.orc_spmspv.restype = C.c_int32
+        L.orc_bfs.argtypes = [P(cbx_dcsc), C.c_int64, C.c_void_p]
+        L.orc_bfs.restype = C.c_int32
         L.orc_max_threads.restype = C.c_int32
         L.orc_set_threads.argtypes = [C.c_int32]
         _orc = L
@@ -79,6 +84,11 @@ def _load_ref() -> C.CDLL:
         L.ref_symbolic.restype = C.c_int32
         L.ref_time_hybrid_i64.argtypes = [P(cbx_dcsc), P(cbx_dcsc), P(C.c_double), P(C.c_int64)]
         L.ref_time_hybrid_i64.restype = C.c_int32
+        for nm in ("ref_spmspv_selectmax_bool", "ref_spmspv_plustimes_i64"):
+            getattr(L, nm).argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+            getattr(L, nm).restype = C.c_int32
+        L.ref_bfs.argtypes = [P(cbx_dcsc), C.c_int64, C.c_void_p]
+        L.ref_bfs.restype = C.c_int32
         L.ref_free.argtypes = [P(cbx_dcsc)]
         L.ref_free.restype = None
         _ref = L
@@ -222,3 +232,50 @@ def product_checksums(A: Dcsc, B: Dcsc, salt: int = 0) -> tuple[np.ndarray, np.n
         t = w[B.ir] * B.val.astype(np.int64).view(np.uint64)
         cs = np.add.reduceat(t, B.cp[:-1]) if B.nzc else np.zeros(0, np.uint64)
     return B.jc.copy(), cs
+
+
+_XDT = {np.dtype(np.int64): 0, np.dtype(np.float64): 1, np.dtype(np.float32): 2}
+
+
+def spmspv(A: Dcsc, xind: np.ndarray, xval: np.ndarray, semiring: str = "select_max") -> tuple[np.ndarray, np.ndarray]:
+    """Oracle y = A x (restated SpMXSpV); boolean A uses the boolean form."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    xind = np.ascontiguousarray(xind, np.int32)
+    xval = np.ascontiguousarray(xval)
+    yi = np.zeros(A.m, np.int32)
+    yv = np.zeros(A.m, xval.dtype)
+    ny = L.orc_spmspv(C.byref(sa), xind.ctypes.data, xval.ctypes.data, xind.size, SEMIRINGS[semiring],
+                      _XDT[xval.dtype], yi.ctypes.data, yv.ctypes.data)
+    return yi[:ny], yv[:ny]
+
+
+def bfs(A: Dcsc, root: int) -> tuple[np.ndarray, int]:
+    """Oracle top-down BFS (TopDownBFS.cpp:442-448): (parents, iterations)."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    parents = np.zeros(A.n, np.int64)
+    it = L.orc_bfs(C.byref(sa), int(root), parents.ctypes.data)
+    return parents, int(it)
+
+
+def ref_spmspv(A: Dcsc, xind: np.ndarray, xval: np.ndarray, semiring: str) -> tuple[np.ndarray, np.ndarray]:
+    """Genuine generic_gespmv: SelectMax on a boolean A, or PlusTimes<int64>."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    xind = np.ascontiguousarray(xind, np.int32)
+    xval = np.ascontiguousarray(xval, np.int64)
+    yi = np.zeros(A.m, np.int32)
+    yv = np.zeros(A.m, np.int64)
+    fn = L.ref_spmspv_selectmax_bool if semiring == "select_max" else L.ref_spmspv_plustimes_i64
+    ny = fn(C.byref(sa), xind.ctypes.data, xval.ctypes.data, xind.size, yi.ctypes.data, yv.ctypes.data)
+    return yi[:ny], yv[:ny]
+
+
+def ref_bfs(A: Dcsc, root: int) -> tuple[np.ndarray, int]:
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    parents = np.zeros(A.n, np.int64)
+    it = L.ref_bfs(C.byref(sa), int(root), parents.ctypes.data)
+    return parents, int(it)
+

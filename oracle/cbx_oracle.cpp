@@ -26,6 +26,7 @@
 #include <cstring>
 #include <limits>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -319,6 +320,74 @@ static cbx_status dispatch_dtype(const cbx_dcsc& A, const cbx_dcsc& B, int64_t c
   }
 }
 
+// ------------------------------------------------------------ SpMSpV
+// SpImpl<SR,IT,NUM,IVT,OVT>::SpMXSpV, heap form for sizeof(NUM) <= sizeof(OVT)
+// (SpImpl.cpp:110-157): k-way merge of the columns of A selected by x with a
+// HeapEntry min-heap; equal rows fold with SR::add(previous, new).
+template <class SR, class T>
+static void spmxspv_heap(const cbx_dcsc& A, const ColMap& cm, const int32_t* xind, const T* xval, int32_t nx,
+                         std::vector<int32_t>& yind, std::vector<T>& yval) {
+  const T* Av = (const T*)A.val;
+  std::vector<std::pair<int64_t, int64_t>> colinds(nx);
+  for (int32_t j = 0; j < nx; ++j) colinds[j] = cm.range(A, xind[j]);
+  std::vector<HeapItem<T>> heap(nx);
+  int64_t hsize = 0;
+  for (int32_t j = 0; j < nx; ++j)
+    if (colinds[j].first != colinds[j].second)
+      heap[hsize++] = HeapItem<T>{A.ir[colinds[j].first], j, Av[colinds[j].first]};
+  std::make_heap(heap.data(), heap.data() + hsize, heap_less<T>);
+  while (hsize > 0) {
+    std::pop_heap(heap.data(), heap.data() + hsize, heap_less<T>);
+    HeapItem<T>& top = heap[hsize - 1];
+    const int64_t locv = top.runr;
+    const T mrhs = SR::multiply(top.num, xval[locv]);
+    if (!yind.empty() && yind.back() == top.key) yval.back() = SR::add(yval.back(), mrhs);
+    else {
+      yind.push_back((int32_t)top.key);
+      yval.push_back(mrhs);
+    }
+    if (++colinds[locv].first != colinds[locv].second) {
+      top.key = A.ir[colinds[locv].first];
+      top.num = Av[colinds[locv].first];
+      std::push_heap(heap.data(), heap.data() + hsize, heap_less<T>);
+    } else {
+      --hsize;
+    }
+  }
+}
+
+// SpImpl<SR,IT,bool,IVT,OVT>::SpMXSpV (SpImpl.cpp:167-213): a boolean A
+// contributes x_k itself; per row the contributions fold with SR::add in
+// ascending-row order.  The reference orders equal rows through a sequence
+// heap; add is commutative for every supported semiring, and min/max fold
+// exactly, so a dense accumulator reproduces it (FP plus-times up to order).
+template <class SR, class T>
+static void spmxspv_bool(const cbx_dcsc& A, const ColMap& cm, const int32_t* xind, const T* xval, int32_t nx,
+                         std::vector<int32_t>& yind, std::vector<T>& yval, std::vector<T>& spa,
+                         std::vector<uint8_t>& seen, std::vector<int32_t>& touched) {
+  touched.clear();
+  for (int32_t j = 0; j < nx; ++j) {
+    auto r = cm.range(A, xind[j]);
+    for (int64_t q = r.first; q < r.second; ++q) {
+      const int32_t i = (int32_t)A.ir[q];
+      if (!seen[i]) {
+        seen[i] = 1;
+        spa[i] = xval[j];
+        touched.push_back(i);
+      } else {
+        spa[i] = SR::add(spa[i], xval[j]);
+      }
+    }
+  }
+  std::sort(touched.begin(), touched.end());
+  yind.assign(touched.begin(), touched.end());
+  yval.resize(touched.size());
+  for (size_t t = 0; t < touched.size(); ++t) {
+    yval[t] = spa[touched[t]];
+    seen[touched[t]] = 0;
+  }
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -369,6 +438,73 @@ cbx_status orc_time_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semirin
   *nnz = C.nnz;
   orc_free(&C);
   return CBX_OK;
+}
+
+// y = A x (SpMXSpV).  A boolean A -> boolean form; else heap form.  Host
+// arrays; yind/yval must hold A.m entries; returns the number of entries.
+int32_t orc_spmspv(const cbx_dcsc* A, const int32_t* xind, const void* xval, int32_t nx, int32_t semiring,
+                   int32_t xdtype, int32_t* yind, void* yval) {
+  ColMap cm(*A);
+  std::vector<int32_t> yi;
+  int32_t ny = -1;
+  auto run = [&](auto* tag, auto sr) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    using SR = decltype(sr);
+    std::vector<T> yv;
+    if (A->dtype == CBX_BOOL) {
+      std::vector<T> spa(A->m);
+      std::vector<uint8_t> seen(A->m, 0);
+      std::vector<int32_t> touched;
+      spmxspv_bool<SR, T>(*A, cm, xind, (const T*)xval, nx, yi, yv, spa, seen, touched);
+    } else {
+      spmxspv_heap<SR, T>(*A, cm, xind, (const T*)xval, nx, yi, yv);
+    }
+    std::copy(yi.begin(), yi.end(), yind);
+    std::copy(yv.begin(), yv.end(), (T*)yval);
+    ny = (int32_t)yi.size();
+  };
+  switch (semiring * 8 + xdtype) {
+    case CBX_SR_PLUS_TIMES * 8 + CBX_I64: run((int64_t*)nullptr, PlusTimes<int64_t>()); break;
+    case CBX_SR_PLUS_TIMES * 8 + CBX_F64: run((double*)nullptr, PlusTimes<double>()); break;
+    case CBX_SR_PLUS_TIMES * 8 + CBX_F32: run((float*)nullptr, PlusTimes<float>()); break;
+    case CBX_SR_MIN_PLUS * 8 + CBX_I64: run((int64_t*)nullptr, MinPlus<int64_t>()); break;
+    case CBX_SR_MIN_PLUS * 8 + CBX_F64: run((double*)nullptr, MinPlus<double>()); break;
+    case CBX_SR_MIN_PLUS * 8 + CBX_F32: run((float*)nullptr, MinPlus<float>()); break;
+    case CBX_SR_SELECT_MAX * 8 + CBX_I64: run((int64_t*)nullptr, SelectMax<int64_t>()); break;
+    case CBX_SR_SELECT_MAX * 8 + CBX_F64: run((double*)nullptr, SelectMax<double>()); break;
+    case CBX_SR_SELECT_MAX * 8 + CBX_F32: run((float*)nullptr, SelectMax<float>()); break;
+  }
+  return ny;
+}
+
+// Top-down BFS, the level loop of TopDownBFS.cpp:442-448 on one process:
+// fringe.setNumToInd(); fringe = SpMV<SelectMaxSRing<bool,int64>>(A, fringe);
+// fringe = EWiseMult(fringe, parents, true, -1) (ParFriends.h:2700, keep
+// entries whose parent is -1); parents.Set(fringe) (FullyDistVec.cpp:1036).
+// Returns the number of iterations; parents (host int64[n]) are overwritten.
+int32_t orc_bfs(const cbx_dcsc* A, int64_t root, int64_t* parents) {
+  const int64_t n = A->n;
+  std::fill(parents, parents + n, (int64_t)-1);
+  ColMap cm(*A);
+  std::vector<int32_t> find(1, (int32_t)root), yind, kept;
+  std::vector<int64_t> fnum, ynum, spa(n);
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<int32_t> touched;
+  int32_t iterations = 0;
+  while (!find.empty()) {
+    fnum.assign(find.begin(), find.end());  // setNumToInd
+    spmxspv_bool<SelectMax<int64_t>, int64_t>(*A, cm, find.data(), fnum.data(), (int32_t)find.size(), yind, ynum,
+                                              spa, seen, touched);
+    kept.clear();
+    for (size_t t = 0; t < yind.size(); ++t)
+      if (parents[yind[t]] == -1) {  // EWiseMult(..., exclude=true, -1)
+        kept.push_back(yind[t]);
+        parents[yind[t]] = ynum[t];  // Set
+      }
+    find.swap(kept);
+    ++iterations;
+  }
+  return iterations;
 }
 
 void orc_free(cbx_dcsc* C) {

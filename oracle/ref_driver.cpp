@@ -166,6 +166,69 @@ int ref_time_hybrid_i64(const cbx_dcsc* A, const cbx_dcsc* B, double* secs, int6
   return 0;
 }
 
+// generic_gespmv<SelectMaxSRing<bool,int64_t>> (Friends.h:448-462) on a
+// boolean A: the reference's SpMXSpV with its sequence heap.
+int32_t ref_spmspv_selectmax_bool(const cbx_dcsc* A, const int32_t* xind, const int64_t* xval, int32_t nx,
+                                  int32_t* yind, int64_t* yval) {
+  ensure_mpi();
+  auto* MA = make_mat<bool, uint8_t>(*A);
+  std::vector<int32_t> indy;
+  std::vector<int64_t> numy;
+  PreAllocatedSPA<int64_t> spa;
+  generic_gespmv<SelectMaxSRing<bool, int64_t>>(*MA, xind, xval, nx, indy, numy, spa);
+  std::copy(indy.begin(), indy.end(), yind);
+  std::copy(numy.begin(), numy.end(), yval);
+  delete MA;
+  return (int32_t)indy.size();
+}
+
+// generic_gespmv<PlusTimesSRing<int64,int64>> on an int64 A (heap form).
+int32_t ref_spmspv_plustimes_i64(const cbx_dcsc* A, const int32_t* xind, const int64_t* xval, int32_t nx,
+                                 int32_t* yind, int64_t* yval) {
+  ensure_mpi();
+  auto* MA = make_mat<int64_t, int64_t>(*A);
+  std::vector<int32_t> indy;
+  std::vector<int64_t> numy;
+  PreAllocatedSPA<int64_t> spa;
+  generic_gespmv<PlusTimesSRing<int64_t, int64_t>>(*MA, xind, xval, nx, indy, numy, spa);
+  std::copy(indy.begin(), indy.end(), yind);
+  std::copy(numy.begin(), numy.end(), yval);
+  delete MA;
+  return (int32_t)indy.size();
+}
+
+// The TopDownBFS level loop (TopDownBFS.cpp:442-448) on one process with the
+// reference's local SpMSpV; the vector glue restates FullyDistSpVec's
+// setNumToInd (FullyDistSpVec.h:250-259), EWiseMult with exclude
+// (ParFriends.h:2700-2760) and FullyDistVec::Set (FullyDistVec.cpp:1036).
+int32_t ref_bfs(const cbx_dcsc* A, int64_t root, int64_t* parents) {
+  ensure_mpi();
+  auto* MA = make_mat<bool, uint8_t>(*A);
+  const int64_t n = A->n;
+  std::fill(parents, parents + n, (int64_t)-1);
+  std::vector<int32_t> ind(1, (int32_t)root), indy;
+  std::vector<int64_t> num, numy;
+  PreAllocatedSPA<int64_t> spa;
+  int32_t it = 0;
+  while (!ind.empty()) {
+    num.assign(ind.begin(), ind.end());
+    indy.clear();
+    numy.clear();
+    generic_gespmv<SelectMaxSRing<bool, int64_t>>(*MA, ind.data(), num.data(), (int32_t)ind.size(), indy, numy, spa);
+    ind.clear();
+    num.clear();
+    for (size_t i = 0; i < indy.size(); ++i)
+      if (parents[indy[i]] == -1) {
+        ind.push_back(indy[i]);
+        num.push_back(numy[i]);
+      }
+    for (size_t i = 0; i < ind.size(); ++i) parents[ind[i]] = num[i];
+    ++it;
+  }
+  delete MA;
+  return it;
+}
+
 void ref_free(cbx_dcsc* C) {
   std::free(C->cp);
   std::free(C->jc);

@@ -1,0 +1,81 @@
+"""GPU SpMSpV (cbx_spmspv) and BFS (cbx_bfs_*) against the oracle: integer
+semirings and BFS parents bit-exact; FP plus-times within the a-priori bound."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import combblas_amd as cb
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def graph(scale, ef=16):
+    return cb.gen_rmat(scale, ef, symmetrize=True, no_self_loops=True, pattern=True).astype("bool")
+
+
+@pytest.mark.parametrize("semiring", ["select_max", "plus_times", "min_plus"])
+@pytest.mark.parametrize("adtype", ["bool", "int64", "float64"])
+def test_spmspv_matches_oracle(semiring, adtype):
+    rng = np.random.default_rng(17)
+    G = graph(12, 8)
+    A = G.astype(adtype)
+    if adtype == "int64":
+        A.val = rng.integers(-9, 10, A.nnz).astype(np.int64)
+    elif adtype == "float64":
+        A.val = rng.uniform(0.5, 2.0, A.nnz)
+    xdt = np.float64 if adtype == "float64" else np.int64
+    dA = A.to_device()
+    for frac in (0.0005, 0.02, 0.3):
+        x = np.sort(rng.choice(A.n, max(1, int(frac * A.n)), replace=False)).astype(np.int32)
+        xv = (rng.uniform(0.5, 2.0, x.size) if xdt == np.float64 else rng.integers(-50, 50, x.size)).astype(xdt)
+        gi, gv = cb.spmspv(dA, x, xv, semiring)
+        oi, ov = oracle.spmspv(A, x, xv, semiring)
+        assert np.array_equal(gi, oi)
+        if xdt == np.float64 and semiring == "plus_times":
+            assert np.allclose(gv, ov, rtol=1e-12, atol=0)
+        else:
+            assert np.array_equal(gv, ov)
+
+
+def test_spmspv_empty_and_invalid():
+    A = graph(8, 4)
+    dA = A.to_device()
+    yi, yv = cb.spmspv(dA, np.zeros(0, np.int32), np.zeros(0, np.int64))
+    assert yi.size == 0
+    with pytest.raises(cb.CbxError):
+        cb.spmspv(dA, np.array([5, 3], np.int32), np.array([1, 2], np.int64))  # not ascending
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_bfs_parents_bit_exact(scale):
+    A = graph(scale)
+    plan = cb.BfsPlan(A.to_device())
+    for root in cb.bfs_roots(A, 8, seed=scale):
+        got, st = plan.run(int(root))
+        want, it = oracle.bfs(A, int(root))
+        assert st["levels"] == it
+        assert st["reached"] == int((want >= 0).sum())
+        assert np.array_equal(got, want)
+
+
+def test_bfs_isolated_root_and_tiny_graph():
+    A = cb.Dcsc.from_coo(6, 6, [0, 1, 1, 2], [1, 0, 2, 1], [1, 1, 1, 1], "bool")
+    plan = cb.BfsPlan(A.to_device())
+    got, st = plan.run(4)  # isolated
+    assert st["levels"] == 1 and np.all(got == -1)
+    got, st = plan.run(0)
+    want, it = oracle.bfs(A, 0)
+    assert np.array_equal(got, want) and st["levels"] == it
+
+
+def test_bfs_rmat_s20_ef16_config():
+    """BFS workload: symmetrized unweighted R-MAT s20 ef16, self-loops removed;
+    4 of the 64 seeded roots checked against the oracle bit for bit."""
+    A = graph(20, 16)
+    plan = cb.BfsPlan(A.to_device())
+    for root in cb.bfs_roots(A, 64, seed=4242)[:4]:
+        got, st = plan.run(int(root))
+        want, it = oracle.bfs(A, int(root))
+        assert st["levels"] == it and np.array_equal(got, want)

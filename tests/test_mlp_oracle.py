@@ -1,0 +1,19 @@
+"""The FP64 MLP oracle's analytic gradient against central finite differences."""
+import numpy as np
+
+from oracle import mlp
+
+
+def test_oracle_gradient_finite_differences():
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(40, 6))
+    y = rng.normal(size=40)
+    P = mlp.init_params(X).astype(np.float64)
+    _, g = mlp.loss_and_grad(P, X, y)
+    ntrain = 64 * 6 + 64 + 64 * 64 + 64 + 64 + 1
+    for i in rng.choice(ntrain, 40, replace=False):
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += 1e-6
+        Pm[i] -= 1e-6
+        fd = (mlp.loss_and_grad(Pp, X, y)[0] - mlp.loss_and_grad(Pm, X, y)[0]) / 2e-6
+        assert abs(fd - g[i]) <= 1e-4 * (abs(fd) + 1e-6)
