@@ -57,7 +57,7 @@ class cbx_dcsc(C.Structure):
 class cbx_spgemm_stats(C.Structure):
     _fields_ = [
         ("flops", C.c_int64), ("nnz", C.c_int64), ("nzc", C.c_int64),
-        ("overflow_cols", C.c_int64), ("predicted_nnz", C.c_int64),
+        ("overflow_cols", C.c_int64), ("predicted_nnz", C.c_int64), ("learned_fallback", C.c_int64),
         ("units_light", C.c_int64), ("units_heavy", C.c_int64), ("nnz_main", C.c_int64),
         ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_alloc", C.c_double),
         ("ms_numeric", C.c_double), ("ms_main_kernel", C.c_double), ("ms_total", C.c_double),
@@ -115,6 +115,8 @@ def lib() -> C.CDLL:
         L.cbx_bfs_plan_destroy.argtypes = [C.c_void_p]
         L.cbx_bfs_plan_destroy.restype = None
         L.cbx_spgemm_symbolic.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
+        L.cbx_spgemm_set_predictor.argtypes = [C.c_void_p]
+        L.cbx_spgemm_set_predictor.restype = C.c_int32
         L.cbx_spgemm_features.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
         L.cbx_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.cbx_mlp_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, P(C.c_float)]
@@ -488,3 +490,14 @@ def mlp_train(params: np.ndarray, X: np.ndarray, y: np.ndarray, steps: int, batc
     o = cbx_mlp_train_opts(steps, batch, lr, 0, seed)
     _check(lib().cbx_mlp_train(dp.ptr, dx.ptr, dy.ptr, len(X), C.byref(o), curve.ctypes.data))
     return dp.to_host(), curve
+
+
+def set_predictor(params: np.ndarray | None) -> None:
+    """Install (or remove, with None) the nnz predictor used by mode="learned"."""
+    if params is None:
+        _check(lib().cbx_spgemm_set_predictor(None))
+        return
+    p = np.ascontiguousarray(params, np.float32)
+    if p.size != MLP_PARAMS:
+        raise ValueError(f"expected {MLP_PARAMS} parameters, got {p.size}")
+    _check(lib().cbx_spgemm_set_predictor(p.ctypes.data))

@@ -209,6 +209,11 @@ void grad_step(const float* P, const float* X, const float* Y, const int64_t* id
 }
 
 }  // namespace
+
+void mlp_infer_launch(const float* params, const float* X, int64_t n, float* out, hipStream_t s) {
+  if (n > 0) k_mlp_infer<<<gb(n, 256), 256, 0, s>>>(params, X, n, out);
+}
+
 }  // namespace cbx
 
 using namespace cbx;

@@ -248,9 +248,11 @@ extern "C" cbx_status cbx_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semir
                 "cbx_spgemm: operands must hold fewer than 2^31 nonzeros each");
   CBX_RETURN_IF(mode != CBX_SPGEMM_TWO_PHASE && mode != CBX_SPGEMM_UNSORTED && mode != CBX_SPGEMM_LEARNED,
                 CBX_ERR_INVALID, "cbx_spgemm: unknown mode %d", (int)mode);
-  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED, CBX_ERR_UNSUPPORTED, "cbx_spgemm: learned mode is not available yet");
   if ((st = init_device_once())) return st;
   SpgemmConfig cfg;
+  cfg.predictor = predictor_params();
+  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor, CBX_ERR_INVALID,
+                "cbx_spgemm: learned mode needs a predictor (cbx_spgemm_set_predictor)");
   return spgemm_dispatch(A, B, semiring, mode, cfg, C, stats, (hipStream_t)stream);
 }
 
@@ -276,4 +278,26 @@ extern "C" cbx_status cbx_spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, 
   if (st) return st;
   CBX_RETURN_IF(!feats, CBX_ERR_INVALID, "cbx_spgemm_features: null output");
   return spgemm_features(A, B, feats, (hipStream_t)stream);
+}
+
+static float* g_predictor = nullptr;
+static std::mutex g_predictor_mu;
+
+const float* cbx::predictor_params() {
+  std::lock_guard<std::mutex> g(g_predictor_mu);
+  return g_predictor;
+}
+
+extern "C" cbx_status cbx_spgemm_set_predictor(const float* params) {
+  std::lock_guard<std::mutex> g(g_predictor_mu);
+  if (!params) {
+    if (g_predictor) (void)hipFree(g_predictor);
+    g_predictor = nullptr;
+    return CBX_OK;
+  }
+  cbx_status st = init_device_once();
+  if (st) return st;
+  if (!g_predictor) CBX_HIP(hipMalloc(&g_predictor, sizeof(float) * CBX_MLP_PARAMS));
+  CBX_HIP(hipMemcpy(g_predictor, params, sizeof(float) * CBX_MLP_PARAMS, hipMemcpyHostToDevice));
+  return CBX_OK;
 }

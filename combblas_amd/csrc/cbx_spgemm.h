@@ -15,7 +15,17 @@ struct SpgemmConfig {
   int64_t small_flop_max = 4096;
   // Unit tier: columns whose output exceeds heavy_nnz use the large units.
   int64_t heavy_nnz = 65536;
+  // Learned mode: device parameters of the nnz predictor (null = not loaded),
+  // unit size target as a fraction of the value window, and the headroom of
+  // the output capacity over the predicted total.
+  const float* predictor = nullptr;
+  float learned_target = 0.5f;
+  float capacity_margin = 0.15f;
 };
+
+// Launches the predictor's inference kernel (cbx_mlp.hip).
+void mlp_infer_launch(const float* params, const float* X, int64_t n, float* out, hipStream_t s);
+const float* predictor_params();
 
 // Cached device allocations for arrays handed to callers (see cbx_runtime.hip).
 cbx_status dev_alloc(void** p, size_t bytes);

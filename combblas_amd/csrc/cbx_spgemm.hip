@@ -833,6 +833,357 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
   }
 }
 
+// Predictor features per stored column of B (one wave per column); see
+// cbx_spgemm_features in cbx.h for the list.
+__global__ void k_features(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
+                           const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA, int64_t m,
+                           float* __restrict__ feats) {
+  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  if (col >= nzcB) return;
+  const int lane = lane_id();
+  long long f = 0, mx = 0, lo = m, hi = -1;
+  for (int64_t e = cpB[col] + lane; e < cpB[col + 1]; e += kWave) {
+    const int32_t p = bpos[e];
+    if (p < 0) continue;
+    const int64_t a0 = cpA[p], a1 = cpA[p + 1];
+    if (a1 == a0) continue;
+    const long long len = a1 - a0;
+    f += len;
+    mx = len > mx ? len : mx;
+    const long long r0 = rowsA[a0], r1 = rowsA[a1 - 1];
+    lo = r0 < lo ? r0 : lo;
+    hi = r1 > hi ? r1 : hi;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    f += __shfl_xor(f, d, kWave);
+    const long long m2 = __shfl_xor(mx, d, kWave);
+    mx = m2 > mx ? m2 : mx;
+    const long long l2 = __shfl_xor(lo, d, kWave);
+    lo = l2 < lo ? l2 : lo;
+    const long long h2 = __shfl_xor(hi, d, kWave);
+    hi = h2 > hi ? h2 : hi;
+  }
+  if (lane != 0) return;
+  const double b = (double)(cpB[col + 1] - cpB[col]);
+  const double span = f > 0 ? (double)(hi - lo + 1) : 0.0;
+  const double est = span > 0 ? span * (1.0 - exp(-(double)f / span)) : 0.0;
+  float* o = feats + col * CBX_MLP_FEATURES;
+  o[0] = (float)log2(1.0 + b);
+  o[1] = (float)log2(1.0 + (double)f);
+  o[2] = (float)log2(1.0 + (double)mx);
+  o[3] = (float)log2(1.0 + (double)f / (b > 0 ? b : 1.0));
+  o[4] = (float)(span / (double)(m > 0 ? m : 1));
+  o[5] = (float)log2(1.0 + est);
+}
+
+// ============================================================ learned tier
+// Single pass: units are cut from predicted column sizes, and each unit's
+// exact size (from its presence bitmap) is published to a decoupled
+// look-back chain that yields its output offset before its values are
+// accumulated and written.
+struct UnitL {
+  int32_t col;
+  int32_t blk_lo;
+  int32_t blk_hi;
+};
+
+// Row histogram of A over kBR-row blocks (the prior that spreads a
+// predicted column size over the blocks).
+__global__ void k_row_hist(const int32_t* __restrict__ rowsA, int64_t nnzA, int S, unsigned long long* __restrict__ hist) {
+  extern __shared__ unsigned int lh[];
+  for (int b = threadIdx.x; b < S; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnzA; i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&lh[rowsA[i] / kBR], 1u);
+  __syncthreads();
+  for (int b = threadIdx.x; b < S; b += blockDim.x)
+    if (lh[b]) atomicAdd(&hist[b], (unsigned long long)lh[b]);
+}
+
+// Lowest and highest output row block a column can touch (from the first and
+// last row of every contributing A column).  One wave per column.
+__global__ void k_col_blocks(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
+                             const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+                             int32_t* __restrict__ blo, int32_t* __restrict__ bhi) {
+  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  if (col >= nzcB) return;
+  int lo = 0x7fffffff, hi = -1;
+  for (int64_t e = cpB[col] + lane_id(); e < cpB[col + 1]; e += kWave) {
+    const int32_t p = bpos[e];
+    if (p < 0) continue;
+    const int64_t a0 = cpA[p], a1 = cpA[p + 1];
+    if (a1 == a0) continue;
+    lo = min(lo, rowsA[a0] / kBR);
+    hi = max(hi, rowsA[a1 - 1] / kBR);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    lo = min(lo, __shfl_xor(lo, d, kWave));
+    hi = max(hi, __shfl_xor(hi, d, kWave));
+  }
+  if (lane_id() == 0) {
+    blo[col] = lo;
+    bhi[col] = hi;
+  }
+}
+
+// Greedy cut of a column into units from expected block counts
+// pred * hist[b] / nnzA: a unit closes when its expected size would pass
+// `target` or its span would pass rblk blocks.  Pass 1 counts (out == null),
+// pass 2 writes at `base`.
+__device__ __forceinline__ int cut_units(int col, float pred, int lo, int hi, const float* __restrict__ w,
+                                         float target, int rblk, UnitL* out) {
+  int n = 0, start = lo;
+  float acc = 0.f;
+  for (int b = lo; b <= hi; ++b) {
+    const float e = pred * w[b];
+    if (b > start && (acc + e > target || b - start >= rblk)) {
+      if (out) out[n] = UnitL{col, start, b};
+      ++n;
+      start = b;
+      acc = 0.f;
+    }
+    acc += e;
+  }
+  if (out) out[n] = UnitL{col, start, hi + 1};
+  return n + 1;
+}
+
+__global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, int64_t small_max,
+                                const float* __restrict__ logpred, const int32_t* __restrict__ blo,
+                                const int32_t* __restrict__ bhi, const float* __restrict__ w, float target, int rblk,
+                                int64_t m, const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
+                                UnitL* __restrict__ units, float* __restrict__ pred_nnz) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= nzcB) return;
+  if (flop[j] <= small_max || bhi[j] < 0) {
+    if (!units) {
+      ucount[j] = 0;
+      pred_nnz[j] = 0.f;
+    }
+    return;
+  }
+  float pred = exp2f(logpred[j]) - 1.f;
+  const float cap = (float)min<int64_t>(flop[j], m);
+  pred = fminf(fmaxf(pred, 1.f), cap);
+  if (!units) {
+    ucount[j] = cut_units((int)j, pred, blo[j], bhi[j], w, target, rblk, nullptr);
+    pred_nnz[j] = pred;
+  } else {
+    cut_units((int)j, pred, blo[j], bhi[j], w, target, rblk, units + ubase[j]);
+  }
+}
+
+constexpr unsigned long long kAggFlag = 1ull << 62, kInclFlag = 2ull << 62, kValMask = (1ull << 62) - 1;
+
+// One workgroup per unit, units taken in order through a ticket counter so
+// that every predecessor of a unit is already resident when it waits on it.
+template <int K, class T, int NT, int U, int KMAX, int RBLK, int CR>
+__global__ __launch_bounds__(NT) void k_num_learned(
+    const UnitL* __restrict__ units, int* __restrict__ ticket, unsigned long long* __restrict__ status,
+    const int64_t* __restrict__ hcs, const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+    const T* __restrict__ valB, const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+    const T* __restrict__ valA, const int32_t* __restrict__ P, int S, int64_t cap, int64_t* __restrict__ irC,
+    T* __restrict__ valC, unsigned long long* __restrict__ col_total, int* __restrict__ ovf_units,
+    int* __restrict__ err) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int NW32 = RBLK * kBRW;
+  constexpr int NW64 = NW32 / 2;
+  constexpr int WPT = (NW64 + NT - 1) / NT;
+  __shared__ Acc vals[KMAX];
+  __shared__ uint32_t bits[NW32];
+  __shared__ int32_t pre[NW64];
+  __shared__ SegStage<T, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  __shared__ int s_t;
+  __shared__ long long s_excl;
+  if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int t = s_t;
+  const UnitL un = units[t];
+  const int col = un.col, t_lo = un.blk_lo, t_hi = un.blk_hi;
+  const int32_t row0 = t_lo * kBR;
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  const bool single_chunk = (b1 - b0) <= NT;
+  const int nw32 = min(t_hi - t_lo, RBLK) * kBRW;
+  for (int i = threadIdx.x; i < nw32; i += NT) bits[i] = 0u;
+
+  auto set_bits = [&](const int64_t(&idx)[U], const T(&)[U]) {
+    int32_t r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+  };
+  auto rank_of = [&](int32_t r) {
+    const int w = r >> 6, b = r & 63;
+    const uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+    return (int)pre[w] + __popcll(word & ((1ull << b) - 1ull));
+  };
+  int q0 = 0;  // first rank of the current value window
+  auto accumulate = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
+    int32_t r[U];
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      r[u] = -1;
+      if (idx[u] >= 0) {
+        r[u] = rowsA[idx[u]] - row0;
+        a[u] = valA[idx[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r[u] < 0) continue;
+      const int q = rank_of(r[u]) - q0;
+      if (q >= 0 && q < KMAX) SR::acc(&vals[q], SR::mul(a[u], bvs[u]));
+    }
+  };
+
+  int32_t crow[CR][U];
+  Acc cprod[CR][U];
+  int tot = 0, nseg = 0;
+  const bool valid = (t_hi - t_lo) <= RBLK;
+  if (!valid && threadIdx.x == 0) atomicOr(err, 64);
+  if (single_chunk) {
+    int64_t base;
+    int len;
+    T bv;
+    load_seg<T, true>(b0 + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+    if (!valid) len = 0;
+    tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+#pragma unroll
+    for (int rr = 0; rr < CR; ++rr) {
+      int64_t idx[U];
+      T bvs[U];
+      fill_round<T, NT, U>(st, nseg, tot, (rr * NT + (int)threadIdx.x) * U, idx, bvs);
+      T a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        crow[rr][u] = -1;
+        if (idx[u] >= 0) {
+          crow[rr][u] = rowsA[idx[u]] - row0;
+          a[u] = valA[idx[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (crow[rr][u] >= 0) {
+          atomicOr(&bits[crow[rr][u] >> 5], 1u << (crow[rr][u] & 31));
+          cprod[rr][u] = SR::mul(a[u], bvs[u]);
+        }
+    }
+    visit_staged<T, NT, U>(st, nseg, tot, set_bits, CR);
+  } else {
+    for (int64_t eb = b0; eb < b1; eb += NT) {
+      int64_t base;
+      int len;
+      T bv;
+      load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+      if (!valid) len = 0;
+      tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+      visit_staged<T, NT, U>(st, nseg, tot, set_bits);
+    }
+  }
+  __syncthreads();
+  const int nw64 = nw32 / 2;
+  int cnt;
+  {
+    int local[WPT];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+      const int w = threadIdx.x * WPT + q;
+      const int c = w < nw64 ? __popc(bits[2 * w]) + __popc(bits[2 * w + 1]) : 0;
+      local[q] = sum;
+      sum += c;
+    }
+    long long total;
+    const int ex = (int)block_excl_scan_ll<NT>(sum, wsum, total);
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+      const int w = threadIdx.x * WPT + q;
+      if (w < nw64) pre[w] = ex + local[q];
+    }
+    cnt = (int)total;
+  }
+  // publish the size, then look back for the offset (bounded wait)
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&status[t], kAggFlag | (unsigned long long)cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    long long excl = 0;
+    long long spins = 0;
+    for (int i = t - 1; i >= 0;) {
+      const unsigned long long v = __hip_atomic_load(&status[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      if ((v >> 62) == 0) {
+        if (++spins > (1ll << 22)) {  // a predecessor never published: give up, flag it
+          atomicOr(err, 256);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        continue;
+      }
+      excl += (long long)(v & kValMask);
+      if ((v >> 62) == 2) break;
+      --i;
+    }
+    __hip_atomic_store(&status[t], kInclFlag | (unsigned long long)(excl + cnt), __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    s_excl = excl;
+    if (cnt) atomicAdd(&col_total[col], (unsigned long long)cnt);
+    if (cnt > KMAX) atomicAdd(ovf_units, 1);
+  }
+  __syncthreads();
+  const int64_t out = s_excl + hcs[col];
+  const bool can_write = out + cnt <= cap;
+  if (!can_write && threadIdx.x == 0) atomicOr(err, 512);
+  // values, in windows of KMAX ranks (one window unless the unit overflowed)
+  for (q0 = 0; q0 < cnt; q0 += KMAX) {
+    const int wn = min(KMAX, cnt - q0);
+    for (int i = threadIdx.x; i < wn; i += NT) vals[i] = SR::ident();
+    __syncthreads();
+    if (single_chunk) {
+#pragma unroll
+      for (int rr = 0; rr < CR; ++rr)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (crow[rr][u] >= 0) {
+            const int q = rank_of(crow[rr][u]) - q0;
+            if (q >= 0 && q < KMAX) SR::acc(&vals[q], cprod[rr][u]);
+          }
+      visit_staged<T, NT, U>(st, nseg, tot, accumulate, CR);
+    } else {
+      for (int64_t eb = b0; eb < b1; eb += NT) {
+        int64_t base;
+        int len;
+        T bv;
+        load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+        if (!valid) len = 0;
+        int ns2;
+        const int t2 = stage_chunk<T, NT>(st, wsum, base, len, bv, ns2);
+        visit_staged<T, NT, U>(st, ns2, t2, accumulate);
+      }
+    }
+    __syncthreads();
+    if (can_write)
+      for (int i = threadIdx.x; i < wn; i += NT) valC[out + q0 + i] = SR::out(vals[i]);
+    __syncthreads();
+  }
+  if (can_write)
+    for (int w = threadIdx.x; w < nw64; w += NT) {
+      uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+      int q = pre[w];
+      while (word) {
+        const int b = __ffsll((unsigned long long)word) - 1;
+        word &= word - 1;
+        irC[out + q] = (int64_t)row0 + w * 64 + b;
+        ++q;
+      }
+    }
+}
+
 // Tile pointers at kBR granularity, one wave per stored A column:
 // P[p*(S+1)+t] = #entries of column p with row < t*kBR (binary search).
 __global__ void k_ptable_wave(const int64_t* __restrict__ cpA, int64_t nzcA, const int32_t* __restrict__ rowsA,
@@ -853,6 +1204,27 @@ __global__ void k_ptable_wave(const int64_t* __restrict__ cpA, int64_t nzcA, con
     }
     out[t] = lo;
   }
+}
+
+// w[b] = hist[b] / nnzA
+__global__ void k_hist_to_w(const unsigned long long* __restrict__ hist, int S, double inv, float* __restrict__ w) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < S) w[b] = (float)((double)hist[b] * inv);
+}
+
+__global__ void k_add_col_totals(int64_t* __restrict__ nnzcol, const unsigned long long* __restrict__ col_total,
+                                 int64_t n) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j < n) nnzcol[j] += (int64_t)col_total[j];
+}
+
+__global__ void k_sum_f32(const float* __restrict__ v, int64_t n, double* __restrict__ out) {
+  double a = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a += v[i];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) a += __shfl_xor(a, d, kWave);
+  if (lane_id() == 0) atomicAdd(out, a);
 }
 
 // ------------------------------------------------------------------ driver
@@ -917,8 +1289,104 @@ constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass 
 constexpr int kSymUnitNT = 1024;
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 32;    // <= 4096 outputs, <= 128K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
+constexpr int kLearnNT = 512, kLearnK = 4096, kLearnRB = 32;    // learned units: value window, span
 
 }  // namespace
+
+struct LearnedInfo {
+  int64_t nnzC = 0, units = 0, overflow_units = 0, hash_total = 0;
+  double pred_sum = 0;
+  bool over_capacity = false;
+};
+
+// Learned single pass for the unit-tier columns (see the learned tier above).
+// On entry nnzcol holds the exact sizes of the hash-tier columns (0 for the
+// rest).  On success C->ir / C->val are allocated and the unit-tier outputs
+// written, nnzcol holds every column's size and cpC its exclusive scan.
+template <int K, class T>
+static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const SpgemmConfig& cfg, hipStream_t s,
+                                  const int32_t* rowsA, const int32_t* bpos, const int64_t* flop, const int32_t* P,
+                                  int S, int64_t* nnzcol, int64_t* cpC, void* tmp, size_t tmp_bytes, cbx_dcsc* C,
+                                  int* err, LearnedInfo& info) {
+  const int64_t nzcB = B->nzc, m = A->m;
+  DevBuf b_feat, b_lp, b_hist, b_w, b_blo, b_bhi, b_uc, b_ub, b_pn, b_units, b_status, b_tick, b_ct, b_ovf, b_hcs, b_sum;
+  float *feats, *logpred, *w, *pred_nnz;
+  unsigned long long *hist, *status, *col_total;
+  int32_t *blo, *bhi;
+  int64_t *ucount, *ubase, *hcs;
+  int *ticket, *ovf;
+  double* dsum;
+  UnitL* units;
+  CBX_TRY(dalloc(b_feat, nzcB * CBX_MLP_FEATURES, s, &feats));
+  CBX_TRY(dalloc(b_lp, nzcB, s, &logpred));
+  CBX_TRY(dalloc(b_hist, S, s, &hist));
+  CBX_TRY(dalloc(b_w, S, s, &w));
+  CBX_TRY(dalloc(b_blo, nzcB, s, &blo));
+  CBX_TRY(dalloc(b_bhi, nzcB, s, &bhi));
+  CBX_TRY(dalloc(b_uc, nzcB + 1, s, &ucount));
+  CBX_TRY(dalloc(b_ub, nzcB + 1, s, &ubase));
+  CBX_TRY(dalloc(b_pn, nzcB, s, &pred_nnz));
+  CBX_TRY(dalloc(b_hcs, nzcB + 1, s, &hcs));
+  CBX_TRY(dalloc(b_sum, 2, s, &dsum));
+  CBX_TRY(dalloc(b_tick, 2, s, &ticket));
+  CBX_TRY(dalloc(b_ovf, 2, s, &ovf));
+  CBX_TRY(dalloc(b_ct, nzcB, s, &col_total));
+  // prediction
+  k_features<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, m, feats);
+  mlp_infer_launch(cfg.predictor, feats, nzcB, logpred, s);
+  // block prior and per-column block range
+  CBX_HIP(hipMemsetAsync(hist, 0, sizeof(unsigned long long) * S, s));
+  k_row_hist<<<256, 256, sizeof(unsigned int) * S, s>>>(rowsA, A->nnz, S, hist);
+  k_hist_to_w<<<nblk(S, 256), 256, 0, s>>>(hist, S, 1.0 / (double)std::max<int64_t>(A->nnz, 1), w);
+  k_col_blocks<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, blo, bhi);
+  // units in column order: count, scan, write
+  const float target = cfg.learned_target * (float)kLearnK;
+  k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
+                                                  kLearnRB, m, nullptr, ucount, nullptr, pred_nnz);
+  CBX_HIP(hipMemsetAsync(ucount + nzcB, 0, sizeof(int64_t), s));
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ucount, ubase, (int)(nzcB + 1), s));
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
+  CBX_HIP(hipMemsetAsync(dsum, 0, sizeof(double), s));
+  k_sum_f32<<<256, 256, 0, s>>>(pred_nnz, nzcB, dsum);
+  int64_t nunits = 0, hash_total = 0;
+  double pred_sum = 0;
+  CBX_HIP(hipMemcpyAsync(&nunits, ubase + nzcB, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemcpyAsync(&hash_total, hcs + nzcB, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemcpyAsync(&pred_sum, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  info.units = nunits;
+  info.pred_sum = pred_sum;
+  info.hash_total = hash_total;
+  CBX_TRY(dalloc(b_units, std::max<int64_t>(nunits, 1), s, &units));
+  CBX_TRY(dalloc(b_status, std::max<int64_t>(nunits, 1), s, &status));
+  if (nunits)
+    k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
+                                                    kLearnRB, m, ubase, nullptr, units, nullptr);
+  // output capacity from the predictions (plus the exact hash-tier sizes)
+  const int64_t cap = hash_total + (int64_t)(pred_sum * (1.0 + cfg.capacity_margin)) + (1 << 20);
+  CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * cap));
+  CBX_TRY(dev_alloc(&C->val, sizeof(T) * cap));
+  CBX_HIP(hipMemsetAsync(status, 0, sizeof(unsigned long long) * std::max<int64_t>(nunits, 1), s));
+  CBX_HIP(hipMemsetAsync(ticket, 0, sizeof(int) * 2, s));
+  CBX_HIP(hipMemsetAsync(ovf, 0, sizeof(int) * 2, s));
+  CBX_HIP(hipMemsetAsync(col_total, 0, sizeof(unsigned long long) * nzcB, s));
+  if (nunits)
+    k_num_learned<K, T, kLearnNT, 4, kLearnK, kLearnRB, 2><<<(unsigned)nunits, kLearnNT, 0, s>>>(
+        units, ticket, status, hcs, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cap, C->ir,
+        (T*)C->val, col_total, ovf, err);
+  CBX_HIP(hipGetLastError());
+  k_add_col_totals<<<nblk(nzcB, 256), 256, 0, s>>>(nnzcol, col_total, nzcB);
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
+  int herr = 0, hovf = 0;
+  CBX_HIP(hipMemcpyAsync(&info.nnzC, cpC + nzcB, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  info.overflow_units = hovf;
+  info.over_capacity = (herr & 512) != 0 || info.nnzC > cap;
+  if (herr & ~512) return set_error(CBX_ERR_HIP, "cbx_spgemm: learned pass inconsistency (flag %d)", herr);
+  return CBX_OK;
+}
 
 template <int K, class T>
 static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_mode mode, const SpgemmConfig& cfg,
@@ -1040,6 +1508,38 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const int64_t total_flop = hb[5];
   tm.mark(1, s);
 
+  // --- learned single pass (unit-tier columns), with the exact hash-tier
+  //     symbolic for the small columns
+  bool learned = mode == CBX_SPGEMM_LEARNED && use_units && n_dense > 0 && cfg.predictor != nullptr;
+  bool learned_done = false;
+  LearnedInfo linfo;
+  int64_t nnzC = 0, nnz_main = 0;
+  int hu[2] = {0, 0}, herr = 0;
+  if (learned) {
+    if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+    if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+    if (n_h256) k_sym_hash<256, kHashNT><<<(unsigned)n_h256, kHashNT, 0, s>>>(order_s, hb[2], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+    tm.mark(2, s);
+    tm.mark(3, s);
+    CBX_TRY((learned_numeric<K, T>(A, B, cfg, s, rowsA, bpos, flop, P, S, nnzcol, cpC, tmp, tmp_bytes, C, err, linfo)));
+    if (linfo.over_capacity) {  // predictions too small overall: redo exactly
+      dev_free(C->ir);
+      dev_free(C->val);
+      C->ir = nullptr;
+      C->val = nullptr;
+      CBX_HIP(hipMemsetAsync(nnzcol, 0, sizeof(int64_t) * (nzcB + 1), s));
+      CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+    } else {
+      learned_done = true;
+      nnzC = linfo.nnzC;
+      nnz_main = linfo.nnzC - linfo.hash_total;
+      C->nnz = nnzC;
+      CBX_TRY(dev_alloc((void**)&C->cp, sizeof(int64_t) * (nzcB + 1)));
+      CBX_TRY(dev_alloc((void**)&C->jc, sizeof(int64_t) * std::max<int64_t>(nzcB, 1)));
+      tm.mark(4, s);
+    }
+  }
+  if (!learned_done) {
   // --- symbolic
   int64_t cap_l = 0, cap_h = 0;
   if (use_units && n_dense) {
@@ -1059,8 +1559,6 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (n_h256) k_sym_hash<256, kHashNT><<<(unsigned)n_h256, kHashNT, 0, s>>>(order_s, hb[2], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
-  int64_t nnzC = 0, nnz_main = 0;
-  int hu[2] = {0, 0}, herr = 0;
   if (n_dense) {
     CBX_HIP(hipMemsetAsync(bounds + 6, 0, sizeof(int64_t), s));
     k_sum_ordered<<<nblk(n_dense, 256), 256, 0, s>>>(order_s, n_dense, nnzcol, (unsigned long long*)(bounds + 6));
@@ -1096,6 +1594,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
                                                                      valA, P, S, cpC, C->ir, (T*)C->val, err);
   }
   tm.mark(4, s);
+  }  // !learned_done
   const bool sort = mode != CBX_SPGEMM_UNSORTED;
 #define CBX_LAUNCH_HASH(HM, NCOL, FIRST)                                                                          \
   if (NCOL) {                                                                                                     \
@@ -1137,55 +1636,14 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     stats->ms_numeric = tm.ms(3, 5);
     stats->ms_main_kernel = tm.ms(3, 4);
     stats->ms_total = tm.ms(0, 5);
-    stats->units_light = hu[0];
-    stats->units_heavy = hu[1];
+    stats->units_light = learned_done ? linfo.units : hu[0];
+    stats->units_heavy = learned_done ? 0 : hu[1];
     stats->nnz_main = nnz_main;
+    stats->overflow_cols = linfo.overflow_units;
+    stats->predicted_nnz = (int64_t)linfo.pred_sum;
+    stats->learned_fallback = learned && !learned_done;
   }
   return CBX_OK;
-}
-
-// Predictor features per stored column of B (one wave per column); see
-// cbx_spgemm_features in cbx.h for the list.
-__global__ void k_features(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
-                           const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA, int64_t m,
-                           float* __restrict__ feats) {
-  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
-  if (col >= nzcB) return;
-  const int lane = lane_id();
-  long long f = 0, mx = 0, lo = m, hi = -1;
-  for (int64_t e = cpB[col] + lane; e < cpB[col + 1]; e += kWave) {
-    const int32_t p = bpos[e];
-    if (p < 0) continue;
-    const int64_t a0 = cpA[p], a1 = cpA[p + 1];
-    if (a1 == a0) continue;
-    const long long len = a1 - a0;
-    f += len;
-    mx = len > mx ? len : mx;
-    const long long r0 = rowsA[a0], r1 = rowsA[a1 - 1];
-    lo = r0 < lo ? r0 : lo;
-    hi = r1 > hi ? r1 : hi;
-  }
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    f += __shfl_xor(f, d, kWave);
-    const long long m2 = __shfl_xor(mx, d, kWave);
-    mx = m2 > mx ? m2 : mx;
-    const long long l2 = __shfl_xor(lo, d, kWave);
-    lo = l2 < lo ? l2 : lo;
-    const long long h2 = __shfl_xor(hi, d, kWave);
-    hi = h2 > hi ? h2 : hi;
-  }
-  if (lane != 0) return;
-  const double b = (double)(cpB[col + 1] - cpB[col]);
-  const double span = f > 0 ? (double)(hi - lo + 1) : 0.0;
-  const double est = span > 0 ? span * (1.0 - exp(-(double)f / span)) : 0.0;
-  float* o = feats + col * CBX_MLP_FEATURES;
-  o[0] = (float)log2(1.0 + b);
-  o[1] = (float)log2(1.0 + (double)f);
-  o[2] = (float)log2(1.0 + (double)mx);
-  o[3] = (float)log2(1.0 + (double)f / (b > 0 ? b : 1.0));
-  o[4] = (float)(span / (double)(m > 0 ? m : 1));
-  o[5] = (float)log2(1.0 + est);
 }
 
 cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, hipStream_t s) {

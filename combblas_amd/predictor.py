@@ -78,7 +78,8 @@ def relative_error(params: np.ndarray, X: np.ndarray, y: np.ndarray) -> dict:
             "under_predicted_frac": float(np.mean(pred_nnz < true_nnz))}
 
 
-def train(steps: int = 4000, batch: int = 4096, lr: float = 2e-3, seed: int = 7, log=print) -> dict:
+def train(steps: int = 4000, batch: int = 4096, lr: float = 2e-3, seed: int = 7, log=print,
+          out_dir: Path | None = None) -> dict:
     t0 = time.time()
     X, y = collect(log=log)
     t1 = time.time()
@@ -89,15 +90,16 @@ def train(steps: int = 4000, batch: int = 4096, lr: float = 2e-3, seed: int = 7,
     A = cb.gen_rmat(20, EDGE_FACTOR)  # held out: the s20 workload itself
     Xh, yh = pairs_for(A)
     q_s20 = relative_error(P, Xh, yh)
-    MODEL_DIR.mkdir(exist_ok=True)
-    P.astype(np.float32).tofile(MODEL_PATH)
+    out_dir = Path(out_dir) if out_dir else MODEL_DIR
+    out_dir.mkdir(parents=True, exist_ok=True)
+    P.astype(np.float32).tofile(out_dir / MODEL_PATH.name)
     meta = {"architecture": "6-64-64-1 ReLU, FP32, target log2(1+nnz_j), MSE, Adam",
             "training_matrices": [dict(zip(("scale", "seed"), training_matrix(i))) for i in range(32)],
             "edge_factor": EDGE_FACTOR, "samples": int(len(y)), "steps": steps, "batch": batch, "lr": lr,
             "seed": seed, "loss_first": float(curve[:50].mean()), "loss_last": float(curve[-50:].mean()),
             "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "train_quality": q_train,
             "heldout_rmat_s20_ef16": q_s20}
-    META_PATH.write_text(json.dumps(meta, indent=2) + "\n")
+    (out_dir / META_PATH.name).write_text(json.dumps(meta, indent=2) + "\n")
     log(json.dumps(meta, indent=2))
     return meta
 
@@ -113,6 +115,6 @@ def load() -> np.ndarray:
 
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "train":
-        train()
+        train(out_dir=Path(sys.argv[2]) if len(sys.argv) > 2 else None)
     else:
         print(__doc__)

@@ -137,9 +137,12 @@ enum {
   /* LocalSpGEMMHash(..., sort=false) (mtSpGEMM.h:463-656): same entries, rows
    * within a column in no particular order. */
   CBX_SPGEMM_UNSORTED = 1,
-  /* Learned single pass: a per-column nnz predictor replaces the symbolic
-   * pass; mispredicted columns are recomputed exactly.  Output identical to
-   * CBX_SPGEMM_TWO_PHASE. */
+  /* Learned single pass: the installed nnz predictor (cbx_spgemm_set_predictor)
+   * replaces the symbolic pass of the large columns; each row-range unit's
+   * exact size comes from its own presence bitmap and its output offset from
+   * a decoupled look-back, units larger than predicted are re-run exactly,
+   * and an exceeded output capacity falls back to the two-phase path.
+   * Output identical to CBX_SPGEMM_TWO_PHASE. */
   CBX_SPGEMM_LEARNED = 2
 };
 
@@ -149,8 +152,11 @@ typedef struct cbx_spgemm_stats {
   int64_t flops;          /* multiply-adds, sum over columns of estimateFLOP */
   int64_t nnz;            /* nnz of the product */
   int64_t nzc;            /* non-empty output columns */
-  int64_t overflow_cols;  /* learned mode: columns re-run on the exact path */
-  int64_t predicted_nnz;  /* learned mode: sum of the predicted column sizes */
+  int64_t overflow_cols;  /* learned mode: units whose exact size exceeded the value window and were
+                             re-run exactly in windows of ranks (caught by a device counter) */
+  int64_t predicted_nnz;  /* learned mode: sum of the predicted sizes of the unit-tier columns */
+  int64_t learned_fallback; /* learned mode: 1 if the predicted output capacity was exceeded and the
+                               product was recomputed by the exact two-phase path */
   int64_t units_light;    /* row-range units of the numeric pass (small/large) */
   int64_t units_heavy;
   int64_t nnz_main;       /* outputs written by the row-range-unit kernels */
@@ -161,6 +167,10 @@ typedef struct cbx_spgemm_stats {
   double ms_main_kernel;  /* the row-range-unit numeric kernels alone */
   double ms_total;
 } cbx_spgemm_stats;
+
+/* Installs the nnz predictor used by CBX_SPGEMM_LEARNED (host array of
+ * CBX_MLP_PARAMS floats, copied to the device), or removes it (NULL). */
+cbx_status cbx_spgemm_set_predictor(const float* params);
 
 /* Return the device memory that freed matrices left in the library's cache
  * (cbx_dcsc_free keeps device blocks for reuse by later calls). */

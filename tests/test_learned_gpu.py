@@ -1,0 +1,101 @@
+"""Learned single-pass SpGEMM (CBX_SPGEMM_LEARNED) must equal the exact result
+whatever the predictor says.  Constant predictors steer every path: units far
+larger than predicted (re-run exactly in rank windows, counted), a predicted
+total too small for the output (fallback to the two-phase path), and
+predictions that fit; the trained model (if present) must need no fallback."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import combblas_amd as cb
+import oracle
+from combblas_amd import predictor
+from tests.helpers import bit_exact, same_pattern
+
+pytestmark = pytest.mark.gpu
+B3 = 64 * 6 + 64 + 64 * 64 + 64 + 64  # index of the output bias
+
+
+def constant_predictor(log2_nnz: float) -> np.ndarray:
+    P = np.zeros(cb.MLP_PARAMS, np.float32)
+    P[B3] = log2_nnz
+    P[-6:] = 1.0  # feature scale
+    return P
+
+
+@pytest.fixture(autouse=True)
+def _reset_predictor():
+    yield
+    cb.set_predictor(None)
+
+
+def learned(A, B, semiring="plus_times"):
+    dC, st = cb.spgemm(A.to_device(), B.to_device(), semiring, "learned")
+    return dC.to_host(), st
+
+
+def test_learned_requires_predictor():
+    cb.set_predictor(None)
+    A = cb.gen_rmat(10, 8)
+    with pytest.raises(cb.CbxError) as ei:
+        learned(A, A)
+    assert ei.value.status == cb.ERR_INVALID
+
+
+@pytest.mark.parametrize("log2_nnz,expect", [(1.0, "fallback"), (11.0, "overflow"), (20.0, "fits")])
+def test_learned_constant_predictors_exact(log2_nnz, expect):
+    A = cb.gen_rmat(14, 8)
+    cb.set_predictor(constant_predictor(log2_nnz))
+    got, st = learned(A, A)
+    want, _ = oracle.spgemm(A, A)
+    bit_exact(got, want)
+    if expect == "fallback":
+        assert st["learned_fallback"] == 1
+    else:
+        assert st["learned_fallback"] == 0 and st["units_light"] > 0
+    if expect == "overflow":
+        assert st["overflow_cols"] > 0
+
+
+@pytest.mark.parametrize("semiring,dtype", [("plus_times", "int64"), ("plus_times", "bool"), ("min_plus", "int64"),
+                                            ("select_max", "int64"), ("plus_times", "float64")])
+def test_learned_semirings(semiring, dtype):
+    rng = np.random.default_rng(2)
+    A = cb.gen_rmat(13, 8).astype(dtype)
+    if dtype == "int64":
+        A.val = rng.integers(-9, 10, A.nnz).astype(np.int64)
+    cb.set_predictor(constant_predictor(13.0))
+    got, st = learned(A, A, semiring)
+    assert st["learned_fallback"] == 0
+    want, _ = oracle.spgemm(A, A, semiring)
+    if dtype == "float64":
+        same_pattern(got, want)
+        assert np.allclose(got.val, want.val, rtol=1e-12, atol=0)
+    else:
+        bit_exact(got, want)
+
+
+@pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
+def test_learned_trained_model_s16():
+    A = cb.gen_rmat(16, 16)
+    cb.set_predictor(predictor.load())
+    got, st = learned(A, A)
+    assert st["learned_fallback"] == 0
+    want, _ = oracle.spgemm(A, A)
+    bit_exact(got, want)
+
+
+@pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
+def test_learned_trained_model_s18_checksums():
+    A = cb.gen_rmat(18, 16)
+    cb.set_predictor(predictor.load())
+    dA = A.to_device()
+    dC, st = cb.spgemm(dA, dA, "plus_times", "learned")
+    assert st["learned_fallback"] == 0
+    dC.check()
+    got = dC.column_checksums(salt=99)
+    _, jcC = dC.structure()
+    jcB, want = oracle.product_checksums(A, A, salt=99)
+    pos = np.searchsorted(jcB, jcC)
+    assert np.array_equal(jcB[pos], jcC) and np.array_equal(got, want[pos])
