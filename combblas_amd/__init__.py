@@ -77,6 +77,12 @@ class cbx_bfs_stats(C.Structure):
     _fields_ = [("levels", C.c_int32), ("pad", C.c_int32), ("reached", C.c_int64), ("ms", C.c_double)]
 
 
+class cbx_mcl_stats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("flops", "nnz_product", "nnz", "phases", "recovered_cols", "selected_cols",
+                                         "reselected_cols", "overflow_cols", "learned_fallback")] + \
+               [(k, C.c_double) for k in ("ms_product", "ms_prune", "ms_total")]
+
+
 _lib_handle = None
 
 
@@ -98,6 +104,15 @@ def lib() -> C.CDLL:
         L.cbx_dcsc_to_host.argtypes = [P(cbx_dcsc), P(cbx_dcsc)]
         L.cbx_gen_rmat.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                    C.c_uint32, P(cbx_dcsc)]
+        L.cbx_gen_powerlaw_stochastic.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_int64, C.c_uint64,
+                                                  P(cbx_dcsc)]
+        L.cbx_gen_powerlaw_stochastic.restype = C.c_int32
+        L.cbx_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc),
+                                    P(cbx_mcl_stats), C.c_void_p]
+        L.cbx_mcl_prune.restype = C.c_int32
+        L.cbx_mcl_expand.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double,
+                                     C.c_int64, C.c_int32, P(cbx_dcsc), P(cbx_mcl_stats), C.c_void_p]
+        L.cbx_mcl_expand.restype = C.c_int32
         L.cbx_spgemm.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, P(cbx_dcsc),
                                  P(cbx_spgemm_stats), C.c_void_p]
         L.cbx_dcsc_check.argtypes = [P(cbx_dcsc)]
@@ -324,6 +339,18 @@ def gen_rmat(scale: int, edgefactor: int, a: float = 0.57, b: float = 0.19, c: f
     return Dcsc.from_struct(s, lib().cbx_dcsc_free)
 
 
+def gen_powerlaw_stochastic(n: int, mean: float = 50.0, alpha: float = 2.1, dmax: int | None = None,
+                            seed: int = 0x3C1) -> Dcsc:
+    """Column-stochastic power-law matrix (configuration model, log-normal weights) as host DCSC float32.
+
+    dmax defaults to the structural cut-off sqrt(n * mean) of an uncorrelated configuration model."""
+    if dmax is None:
+        dmax = max(int(mean) + 1, int((n * mean) ** 0.5))
+    s = cbx_dcsc()
+    _check(lib().cbx_gen_powerlaw_stochastic(n, mean, alpha, dmax, seed, C.byref(s)))
+    return Dcsc.from_struct(s, lib().cbx_dcsc_free)
+
+
 # -------------------------------------------------------------------- SpGEMM
 def spgemm(A: DeviceDcsc, B: DeviceDcsc, semiring: str = "plus_times", mode: str = "two_phase",
            stream: int | None = None) -> tuple[DeviceDcsc, dict]:
@@ -334,6 +361,27 @@ def spgemm(A: DeviceDcsc, B: DeviceDcsc, semiring: str = "plus_times", mode: str
                             C.byref(st), stream))
     stats = {k: getattr(st, k) for k, _ in cbx_spgemm_stats._fields_}
     return DeviceDcsc(out), stats
+
+
+def mcl_prune(C_: DeviceDcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, pct: float = 0.9,
+              stream: int | None = None) -> tuple[DeviceDcsc, dict]:
+    """MCLPruneRecoverySelect (ParFriends.h:185-345) on a device matrix; returns (pruned copy, stats)."""
+    out = cbx_dcsc()
+    st = cbx_mcl_stats()
+    _check(lib().cbx_mcl_prune(C.byref(C_.s), thr, select, recover, pct, C.byref(out), C.byref(st), stream))
+    return DeviceDcsc(out), {k: getattr(st, k) for k, _ in cbx_mcl_stats._fields_}
+
+
+def mcl_expand(A: DeviceDcsc, B: DeviceDcsc | None = None, thr: float = 1e-4, select: int = 1000, recover: int = 0,
+               pct: float = 0.9, phases: int = 0, mode: str = "two_phase",
+               stream: int | None = None) -> tuple[DeviceDcsc, dict]:
+    """One MCL expansion: A*B (B defaults to A) in column phases, each pruned as cbx_mcl_prune."""
+    out = cbx_dcsc()
+    st = cbx_mcl_stats()
+    B = A if B is None else B
+    _check(lib().cbx_mcl_expand(C.byref(A.s), C.byref(B.s), thr, select, recover, pct, phases, MODES[mode],
+                                C.byref(out), C.byref(st), stream))
+    return DeviceDcsc(out), {k: getattr(st, k) for k, _ in cbx_mcl_stats._fields_}
 
 
 def LocalHybridSpGEMM(A: Dcsc, B: Dcsc, semiring: str = "plus_times") -> Dcsc:

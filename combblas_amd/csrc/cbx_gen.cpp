@@ -135,3 +135,126 @@ extern "C" cbx_status cbx_gen_rmat(int32_t scale, int32_t edgefactor, double a, 
   }
   return triples_to_dcsc(t, n, n, pattern, out);
 }
+
+// ----------------------------------------------------------------------------
+// Power-law, column-stochastic matrix (the input of an MCL expansion step,
+// Applications/MCL.cpp:573).  Vertex i gets degree
+//   d_i = min(dmax, floor(xmin * u_i^(-1/(alpha-1)))),   u_i uniform in (0,1],
+// with xmin found by bisection so that the sample mean of d equals `mean`.
+// Edges come from the directed configuration model: column j holds d_j
+// out-stubs, the row stubs are the same multiset (vertex i appears d_i times)
+// in a seeded random order, and the e-th column stub is paired with the e-th
+// row stub.  Every edge carries a log-normal(0,1) weight; duplicate (row,col)
+// pairs add their weights, and each column is divided by its sum (in double)
+// before rounding to float, so columns sum to 1 up to float rounding.
+namespace cbx {
+
+struct WTriple { uint64_t key; double val; };
+
+static double degree_of(double xmin, double alpha, double u, int64_t dmax) {
+  double d = std::floor(xmin * std::pow(u, -1.0 / (alpha - 1.0)));
+  return d > (double)dmax ? (double)dmax : d;
+}
+
+}  // namespace cbx
+
+extern "C" cbx_status cbx_gen_powerlaw_stochastic(int64_t n, double mean, double alpha, int64_t dmax,
+                                                  uint64_t seed, cbx_dcsc* out) {
+  if (!out) return set_error(CBX_ERR_INVALID, "cbx_gen_powerlaw_stochastic: out is null");
+  if (n < 1 || n > ((int64_t)1 << 31) - 1)
+    return set_error(CBX_ERR_INVALID, "cbx_gen_powerlaw_stochastic: n must be in [1, 2^31)");
+  if (!(alpha > 1.0) || !(mean >= 1.0) || dmax < 1 || (double)dmax < mean)
+    return set_error(CBX_ERR_INVALID, "cbx_gen_powerlaw_stochastic: need alpha > 1, mean >= 1, dmax >= mean");
+  std::vector<double> u(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    EdgeRng r(seed ^ 0x5DE6A11ULL, (uint64_t)i);
+    u[i] = 1.0 - r.uniform();  // (0,1]
+  }
+  auto sample_mean = [&](double xmin) {
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int64_t i = 0; i < n; ++i) s += degree_of(xmin, alpha, u[i], dmax);
+    return s / (double)n;
+  };
+  double lo = 1.0, hi = mean + 1.0;
+  if (sample_mean(lo) > mean) hi = lo;  // even xmin = 1 overshoots: use it
+  else {
+    while (sample_mean(hi) < mean && hi < 1e12) hi *= 2;
+    for (int it = 0; it < 80; ++it) {
+      double mid = 0.5 * (lo + hi);
+      (sample_mean(mid) < mean ? lo : hi) = mid;
+    }
+  }
+  const double xmin = hi;
+  std::vector<int64_t> off(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t d = (int64_t)degree_of(xmin, alpha, u[i], dmax);
+    off[i + 1] = off[i] + (d < 1 ? 1 : d);
+  }
+  const int64_t nst = off[n];
+  std::vector<WTriple> t;
+  std::vector<std::pair<uint64_t, uint32_t>> rows;
+  try {
+    t.resize(nst);
+    rows.resize(nst);
+  } catch (...) {
+    return set_error(CBX_ERR_NOMEM, "cbx_gen_powerlaw_stochastic: host allocation failed");
+  }
+  // row stubs in a seeded random order: sort (random key, vertex)
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t s = off[i]; s < off[i + 1]; ++s)
+      rows[s] = {splitmix(seed ^ splitmix((uint64_t)s + 0x1234567ULL)), (uint32_t)i};
+  __gnu_parallel::sort(rows.begin(), rows.end());
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t j = 0; j < n; ++j)
+    for (int64_t s = off[j]; s < off[j + 1]; ++s) {
+      EdgeRng r(seed ^ 0x10A0A1ULL, (uint64_t)s);
+      // Box-Muller: one standard normal from two uniforms
+      double u1 = 1.0 - r.uniform(), u2 = r.uniform();
+      double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+      t[s] = WTriple{((uint64_t)j << 32) | rows[s].second, std::exp(z)};
+    }
+  std::vector<std::pair<uint64_t, uint32_t>>().swap(rows);
+  __gnu_parallel::sort(t.begin(), t.end(), [](const WTriple& x, const WTriple& y) { return x.key < y.key; });
+  size_t w = 0;
+  for (size_t i = 0; i < t.size(); ++i) {
+    if (w > 0 && t[w - 1].key == t[i].key) t[w - 1].val += t[i].val;
+    else t[w++] = t[i];
+  }
+  t.resize(w);
+  const int64_t nnz = (int64_t)w;
+  int64_t nzc = 0;
+  for (size_t i = 0; i < w; ++i)
+    if (i == 0 || (t[i].key >> 32) != (t[i - 1].key >> 32)) ++nzc;
+  std::memset(out, 0, sizeof(*out));
+  out->m = n; out->n = n; out->nnz = nnz; out->nzc = nzc;
+  out->dtype = CBX_F32; out->location = CBX_HOST;
+  out->cp = (int64_t*)std::malloc(sizeof(int64_t) * (nzc + 1));
+  out->jc = (int64_t*)std::malloc(sizeof(int64_t) * (nzc > 0 ? nzc : 1));
+  out->ir = (int64_t*)std::malloc(sizeof(int64_t) * (nnz > 0 ? nnz : 1));
+  out->val = std::malloc(sizeof(float) * (nnz > 0 ? nnz : 1));
+  if (!out->cp || !out->jc || !out->ir || !out->val) {
+    cbx_dcsc_free(out);
+    return set_error(CBX_ERR_NOMEM, "cbx_gen_powerlaw_stochastic: host allocation failed");
+  }
+  int64_t c = 0;
+  for (int64_t i = 0; i < nnz; ++i) {
+    if (i == 0 || (t[i].key >> 32) != (t[i - 1].key >> 32)) {
+      out->jc[c] = (int64_t)(t[i].key >> 32);
+      out->cp[c] = i;
+      ++c;
+    }
+    out->ir[i] = (int64_t)(t[i].key & 0xffffffffULL);
+  }
+  out->cp[nzc] = nnz;
+  float* val = (float*)out->val;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t q = 0; q < nzc; ++q) {
+    double s = 0;
+    for (int64_t p = out->cp[q]; p < out->cp[q + 1]; ++p) s += t[p].val;
+    for (int64_t p = out->cp[q]; p < out->cp[q + 1]; ++p) val[p] = (float)(t[p].val / s);
+  }
+  return CBX_OK;
+}

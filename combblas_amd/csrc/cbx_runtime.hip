@@ -14,7 +14,7 @@
 
 namespace cbx {
 
-static size_t dtype_size(cbx_dtype t) {
+size_t dtype_size(cbx_dtype t) {
   switch (t) {
     case CBX_I64: return 8;
     case CBX_F64: return 8;
@@ -26,7 +26,7 @@ static size_t dtype_size(cbx_dtype t) {
 
 // Keep freed stream-ordered allocations in the device pool so repeated calls
 // do not go back to the driver.
-static cbx_status init_device_once() {
+cbx_status init_device_once() {
   static std::once_flag flag;
   static cbx_status result = CBX_OK;
   std::call_once(flag, [] {
@@ -45,7 +45,7 @@ static cbx_status init_device_once() {
 }
 
 // Structural checks shared by all entry points that take a DCSC.
-static cbx_status check_dcsc(const cbx_dcsc* X, const char* name, cbx_location loc) {
+cbx_status check_dcsc(const cbx_dcsc* X, const char* name, cbx_location loc) {
   CBX_RETURN_IF(!X, CBX_ERR_INVALID, "%s is null", name);
   CBX_RETURN_IF(X->location != loc, CBX_ERR_INVALID, "%s: expected %s memory", name,
                 loc == CBX_DEVICE ? "device" : "host");

@@ -32,6 +32,13 @@ cbx_status dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
 void release_cache();
 
+// Shared front-door helpers (cbx_runtime.hip).
+size_t dtype_size(cbx_dtype t);
+cbx_status init_device_once();
+cbx_status check_dcsc(const cbx_dcsc* X, const char* name, cbx_location loc);
+
+// flop_j of every stored column of B (estimateFLOP), copied to host memory.
+cbx_status spgemm_flops(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop_host, hipStream_t s);
 cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, hipStream_t s);
 cbx_status spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, hipStream_t s);
 

@@ -1673,6 +1673,25 @@ cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, h
   return CBX_OK;
 }
 
+cbx_status spgemm_flops(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop_host, hipStream_t s) {
+  const int64_t nzcA = A->nzc, nzcB = B->nzc, nnzB = B->nnz;
+  if (nzcB == 0) return CBX_OK;
+  DevBuf b_colmap, b_bpos, b_flop;
+  int32_t *colmap, *bpos;
+  int64_t* flop;
+  CBX_TRY(dalloc(b_colmap, A->n, s, &colmap));
+  CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
+  CBX_TRY(dalloc(b_flop, nzcB, s, &flop));
+  k_fill_i32<<<nblk(A->n, 256), 256, 0, s>>>(colmap, A->n, -1);
+  if (nzcA) k_colmap<<<nblk(nzcA, 256), 256, 0, s>>>(A->jc, nzcA, colmap);
+  if (nnzB) k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
+  k_flop<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, flop);
+  CBX_HIP(hipGetLastError());
+  CBX_HIP(hipMemcpyAsync(flop_host, flop, sizeof(int64_t) * nzcB, hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  return CBX_OK;
+}
+
 cbx_status spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, hipStream_t s) {
   cbx_dcsc C;
   SpgemmConfig cfg;

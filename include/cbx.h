@@ -126,6 +126,17 @@ enum {
 cbx_status cbx_gen_rmat(int32_t scale, int32_t edgefactor, double a, double b, double c,
                         uint64_t seed, uint32_t flags, cbx_dcsc* out);
 
+/* Column-stochastic power-law matrix, n x n, dtype CBX_F32 (the input of an
+ * MCL expansion step, Applications/MCL.cpp:573, which reads its graph from a
+ * file instead).  Degrees d_i = min(dmax, floor(xmin * u^(-1/(alpha-1)))) with
+ * xmin chosen so that the mean degree is `mean` (every vertex has degree >= 1);
+ * edges from the directed configuration model (column j has d_j entries, row i
+ * appears d_i times over all columns, stubs paired in seeded random order);
+ * log-normal(0,1) weights, duplicate pairs summed, columns normalised to sum
+ * 1.  Deterministic for a given seed, independent of the thread count. */
+cbx_status cbx_gen_powerlaw_stochastic(int64_t n, double mean, double alpha, int64_t dmax, uint64_t seed,
+                                       cbx_dcsc* out);
+
 /* ----------------------------------------------------------------- SpGEMM */
 
 /* Modes of the local product. */
@@ -266,6 +277,49 @@ cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan* plan);
 /* parents: device int64[n], overwritten. */
 cbx_status cbx_bfs_run(cbx_bfs_plan plan, int64_t root, int64_t* parents, cbx_bfs_stats* stats);
 void cbx_bfs_plan_destroy(cbx_bfs_plan plan);
+
+/* ------------------------------------------------------------ MCL expansion */
+
+typedef struct cbx_mcl_stats {
+  int64_t flops;           /* multiply-adds of the product */
+  int64_t nnz_product;     /* entries of the product before the epilogue */
+  int64_t nnz;             /* entries kept */
+  int64_t phases;          /* column phases the product ran in */
+  int64_t recovered_cols;  /* columns whose cut came from recovery */
+  int64_t selected_cols;   /* columns whose cut came from selection */
+  int64_t reselected_cols; /* selected columns then recovered (recovery after selection) */
+  int64_t overflow_cols;   /* learned mode, summed over phases (see cbx_spgemm_stats) */
+  int64_t learned_fallback;
+  double ms_product, ms_prune, ms_total;
+} cbx_mcl_stats;
+
+/* MCLPruneRecoverySelect(A, hardThreshold, selectNum, recoverNum, recoverPct,
+ * kselectVersion) (ParFriends.h:185-345) on one process's block, for a
+ * device DCSC of float32 or float64 values.  Per stored column: entries >
+ * hard_threshold are counted and summed; a column with fewer than recover_num
+ * of them, more entries in all, and a sum below recover_pct is cut at its
+ * recover_num-th largest entry (Kselect, SpParMat.cpp:1413-1690: the
+ * smallest entry if it has fewer); otherwise a column with more than
+ * select_num of them is cut at its select_num-th largest entry, and then at
+ * its recover_num-th largest if fewer than recover_num entries survive that
+ * cut and their sum is below recover_pct; other columns are cut at
+ * hard_threshold.  Entries with !(v < cut) are kept (PruneColumn with
+ * std::less: ties at the cut are kept, and so are entries equal to the
+ * threshold); emptied columns are dropped.  Column statistics are summed in
+ * FP64 (the reference sums sequentially in the value type).  `out` receives
+ * a new device DCSC (release with cbx_dcsc_free); `C` is not modified. */
+cbx_status cbx_mcl_prune(const cbx_dcsc* C, double hard_threshold, int64_t select_num, int64_t recover_num,
+                         double recover_pct, cbx_dcsc* out, cbx_mcl_stats* stats, void* stream);
+
+/* One MCL expansion A*B followed by cbx_mcl_prune, with B's stored columns
+ * split into `phases` flop-balanced column phases (MemEfficientSpGEMM,
+ * ParFriends.h:452-780: ColSplit, per-phase product and prune,
+ * ColConcatenate) so the unpruned product never has to fit at once;
+ * phases <= 0 picks the count from free device memory.  mode:
+ * CBX_SPGEMM_TWO_PHASE or CBX_SPGEMM_LEARNED (plus-times semiring). */
+cbx_status cbx_mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, double hard_threshold, int64_t select_num,
+                          int64_t recover_num, double recover_pct, int64_t phases, cbx_spgemm_mode mode,
+                          cbx_dcsc* out, cbx_mcl_stats* stats, void* stream);
 
 #ifdef __cplusplus
 }

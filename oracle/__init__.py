@@ -61,6 +61,9 @@ def _load_oracle() -> C.CDLL:
         L.orc_spmspv.restype = C.c_int32
         L.orc_bfs.argtypes = [P(cbx_dcsc), C.c_int64, C.c_void_p]
         L.orc_bfs.restype = C.c_int32
+        L.orc_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc),
+                                    C.c_void_p]
+        L.orc_mcl_prune.restype = C.c_int32
         L.orc_max_threads.restype = C.c_int32
         L.orc_set_threads.argtypes = [C.c_int32]
         _orc = L
@@ -89,6 +92,8 @@ def _load_ref() -> C.CDLL:
             getattr(L, nm).restype = C.c_int32
         L.ref_bfs.argtypes = [P(cbx_dcsc), C.c_int64, C.c_void_p]
         L.ref_bfs.restype = C.c_int32
+        L.ref_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc)]
+        L.ref_mcl_prune.restype = C.c_int32
         L.ref_free.argtypes = [P(cbx_dcsc)]
         L.ref_free.restype = None
         _ref = L
@@ -186,6 +191,32 @@ def ref_symbolic(A: Dcsc, B: Dcsc) -> tuple[np.ndarray, np.ndarray]:
     nnz = np.zeros(B.nzc, np.int64)
     L.ref_symbolic(C.byref(sa), C.byref(sb), flop.ctypes.data, nnz.ctypes.data)
     return flop, nnz
+
+
+def mcl_prune(A: Dcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, pct: float = 0.9,
+              return_cut: bool = False):
+    """MCLPruneRecoverySelect (ParFriends.h:185-345) restated on one process; A float32 or float64.
+    With return_cut, also the per-stored-column threshold that the final PruneColumn applied."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    out = cbx_dcsc()
+    cut = np.zeros(A.nzc, np.float32 if A.dtype == "float32" else np.float64)
+    st = L.orc_mcl_prune(C.byref(sa), thr, select, recover, pct, C.byref(out), cut.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"orc_mcl_prune failed with status {st}")
+    X = Dcsc.from_struct(out, L.orc_free)
+    return (X, cut) if return_cut else X
+
+
+def ref_mcl_prune(A: Dcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, pct: float = 0.9) -> Dcsc:
+    """The genuine MCLPruneRecoverySelect on a 1x1 process grid (only where oracle/_ref was built)."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    out = cbx_dcsc()
+    st = L.ref_mcl_prune(C.byref(sa), thr, select, recover, pct, C.byref(out))
+    if st != 0:
+        raise RuntimeError(f"ref_mcl_prune failed with status {st}")
+    return Dcsc.from_struct(out, L.ref_free)
 
 
 # ------------------------------------------------------------------ checks

@@ -21,6 +21,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <functional>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -388,6 +389,91 @@ static void spmxspv_bool(const cbx_dcsc& A, const ColMap& cm, const int32_t* xin
   }
 }
 
+// ------------------------------------------------- MCL prune / recover / select
+// MCLPruneRecoverySelect (ParFriends.h:185-345) on one process, restated per
+// column.  Column statistics follow SpParMat::Reduce(Column, plus, 0)
+// (SpParMat.cpp:961-1010): a sequential sum in the value type, in stored
+// order, `s = v + s`; counts are accumulated in the value type too.
+// Kselect1 (SpParMat.cpp:1413-1690): the k-th largest entry of the
+// unpruned column if it has >= k entries, else its smallest entry, and
+// numeric_limits<T>::min() for an empty column.  PruneColumn with std::less
+// (dcsc.cpp:759-838) keeps v with !(v < threshold) and drops columns left
+// empty, keeping the order of the survivors.
+template <class T>
+static T kth_largest(const T* v, int64_t len, int64_t k, std::vector<T>& buf) {
+  if (len == 0) return std::numeric_limits<T>::min();
+  buf.assign(v, v + len);
+  std::sort(buf.begin(), buf.end(), std::greater<T>());
+  return len >= k ? buf[k - 1] : buf[len - 1];
+}
+
+template <class T>
+static cbx_status mcl_prune_typed(const cbx_dcsc& A, T thr, int64_t select, int64_t recover, T pct,
+                                  cbx_dcsc* out, T* cut_out) {
+  const int64_t nzc = A.nzc;
+  const T* val = (const T*)A.val;
+  std::vector<T> cut(nzc);
+  std::vector<int64_t> keep(nzc + 1, 0);
+#pragma omp parallel
+  {
+    std::vector<T> buf;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t q = 0; q < nzc; ++q) {
+      const int64_t b = A.cp[q], e = A.cp[q + 1], len = e - b;
+      const T* v = val + b;
+      T nP = 0, sP = 0;  // Reduce over PrunedA (values <= thr removed)
+      for (int64_t i = 0; i < len; ++i)
+        if (!(v[i] <= thr)) { nP = T(1) + nP; sP = v[i] + sP; }
+      const T nU = (T)len;
+      T c = thr;
+      const bool rec = nP < (T)recover && nU > nP && sP < pct;
+      if (rec) {
+        c = kth_largest(v, len, recover, buf);
+      } else if (select > 0 && nP > (T)select) {
+        c = kth_largest(v, len, select, buf);
+        if (recover > 0) {
+          T n1 = 0, s1 = 0;
+          for (int64_t i = 0; i < len; ++i)
+            if (!(v[i] < c)) { n1 = T(1) + n1; s1 = v[i] + s1; }
+          if (n1 < (T)recover && s1 < pct) c = kth_largest(v, len, recover, buf);
+        }
+      }
+      cut[q] = c;
+      int64_t k = 0;
+      for (int64_t i = 0; i < len; ++i) k += !(v[i] < c);
+      keep[q + 1] = k;
+    }
+  }
+  if (cut_out) std::copy(cut.begin(), cut.end(), cut_out);
+  int64_t onzc = 0;
+  for (int64_t q = 0; q < nzc; ++q) onzc += keep[q + 1] > 0;
+  for (int64_t q = 0; q < nzc; ++q) keep[q + 1] += keep[q];
+  const int64_t onnz = keep[nzc];
+  std::memset(out, 0, sizeof(*out));
+  out->m = A.m; out->n = A.n; out->nnz = onnz; out->nzc = onzc;
+  out->dtype = A.dtype; out->location = CBX_HOST;
+  out->cp = (int64_t*)std::malloc(sizeof(int64_t) * (onzc + 1));
+  out->jc = (int64_t*)std::malloc(sizeof(int64_t) * (onzc ? onzc : 1));
+  out->ir = (int64_t*)std::malloc(sizeof(int64_t) * (onnz ? onnz : 1));
+  T* ov = (T*)std::malloc(sizeof(T) * (onnz ? onnz : 1));
+  out->val = ov;
+  int64_t c = 0;
+  for (int64_t q = 0; q < nzc; ++q) {
+    if (keep[q + 1] == keep[q]) continue;
+    out->jc[c] = A.jc[q];
+    out->cp[c] = keep[q];
+    ++c;
+  }
+  out->cp[onzc] = onnz;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t q = 0; q < nzc; ++q) {
+    int64_t w = keep[q];
+    for (int64_t p = A.cp[q]; p < A.cp[q + 1]; ++p)
+      if (!(val[p] < cut[q])) { out->ir[w] = A.ir[p]; ov[w] = val[p]; ++w; }
+  }
+  return CBX_OK;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -482,6 +568,17 @@ int32_t orc_spmspv(const cbx_dcsc* A, const int32_t* xind, const void* xval, int
 // fringe = EWiseMult(fringe, parents, true, -1) (ParFriends.h:2700, keep
 // entries whose parent is -1); parents.Set(fringe) (FullyDistVec.cpp:1036).
 // Returns the number of iterations; parents (host int64[n]) are overwritten.
+// cut_out (optional, nzc entries of the value type): the per-column threshold
+// PruneColumn applied (ParFriends.h:339).
+int32_t orc_mcl_prune(const cbx_dcsc* A, double thr, int64_t select, int64_t recover, double pct, cbx_dcsc* out,
+                      void* cut_out) {
+  if (A->dtype == CBX_F32)
+    return orc::mcl_prune_typed<float>(*A, (float)thr, select, recover, (float)pct, out, (float*)cut_out);
+  if (A->dtype == CBX_F64)
+    return orc::mcl_prune_typed<double>(*A, thr, select, recover, pct, out, (double*)cut_out);
+  return CBX_ERR_UNSUPPORTED;
+}
+
 int32_t orc_bfs(const cbx_dcsc* A, int64_t root, int64_t* parents) {
   const int64_t n = A->n;
   std::fill(parents, parents + n, (int64_t)-1);

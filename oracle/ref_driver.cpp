@@ -99,9 +99,52 @@ int run(const cbx_dcsc* A, const cbx_dcsc* B, int mode, cbx_dcsc* C) {
   return 0;
 }
 
+template <class NT>
+void export_dcsc(SpDCCols<int64_t, NT>& M, cbx_dtype dt, cbx_dcsc* C) {
+  std::memset(C, 0, sizeof(*C));
+  C->m = M.getnrow();
+  C->n = M.getncol();
+  C->dtype = dt;
+  C->location = CBX_HOST;
+  const int64_t nnz = M.getnnz();
+  Dcsc<int64_t, NT>* d = nnz > 0 ? M.GetDCSC() : nullptr;
+  const int64_t nzc = d ? d->nzc : 0;
+  C->nnz = nnz;
+  C->nzc = nzc;
+  C->cp = (int64_t*)std::malloc(sizeof(int64_t) * (nzc + 1));
+  C->jc = (int64_t*)std::malloc(sizeof(int64_t) * (nzc ? nzc : 1));
+  C->ir = (int64_t*)std::malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+  NT* v = (NT*)std::malloc(sizeof(NT) * (nnz ? nnz : 1));
+  C->val = v;
+  C->cp[0] = 0;
+  if (!d) return;
+  std::memcpy(C->cp, d->cp, sizeof(int64_t) * (nzc + 1));
+  std::memcpy(C->jc, d->jc, sizeof(int64_t) * nzc);
+  std::memcpy(C->ir, d->ir, sizeof(int64_t) * nnz);
+  std::memcpy(v, d->numx, sizeof(NT) * nnz);
+}
+
+// MCLPruneRecoverySelect (ParFriends.h:185) on a 1x1 process grid.
+template <class NT>
+int mcl_prune(const cbx_dcsc* A, double thr, int64_t select, int64_t recover, double pct, cbx_dcsc* C) {
+  ensure_mpi();
+  typedef SpDCCols<int64_t, NT> DER;
+  std::shared_ptr<CommGrid> grid(new CommGrid(MPI_COMM_WORLD, 1, 1));
+  SpParMat<int64_t, NT, DER> M(make_mat<NT, NT>(*A), grid);
+  MCLPruneRecoverySelect(M, (NT)thr, (int64_t)select, (int64_t)recover, (NT)pct, 1);
+  export_dcsc<NT>(M.seq(), A->dtype, C);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int32_t ref_mcl_prune(const cbx_dcsc* A, double thr, int64_t select, int64_t recover, double pct, cbx_dcsc* C) {
+  if (A->dtype == CBX_F32) return mcl_prune<float>(A, thr, select, recover, pct, C);
+  if (A->dtype == CBX_F64) return mcl_prune<double>(A, thr, select, recover, pct, C);
+  return CBX_ERR_UNSUPPORTED;
+}
 
 // mode 0: LocalHybridSpGEMM; 1: LocalSpGEMMHash(sort=true); 2: sort=false.
 int ref_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semiring, int32_t mode, cbx_dcsc* C) {

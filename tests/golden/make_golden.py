@@ -63,6 +63,17 @@ def main() -> None:
                 a_m=A.m, a_n=A.n, a_cp=A.cp, a_jc=A.jc, a_ir=A.ir, a_val=A.val,
                 b_m=B.m, b_n=B.n, b_cp=B.cp, b_jc=B.jc, b_ir=B.ir, b_val=B.val,
                 c_cp=C.cp, c_jc=C.jc, c_ir=C.ir, c_val=C.val)
+    # MCLPruneRecoverySelect on small MCL-shaped inputs (ParFriends.h:185-345)
+    sys.path.insert(0, str(ROOT / "tests"))
+    from helpers import MCL_PARAMS, mcl_columns
+    for i, params in enumerate(MCL_PARAMS[:3]):
+        dt = "float64" if i == 2 else "float32"
+        A = mcl_columns(100 + i, ncols=40, m=2500, dtype=dt)
+        R = oracle.ref_mcl_prune(A, *params)
+        thr, select, recover, pct = params
+        np.savez_compressed(OUT / f"mcl_prune_{i}_{dt}.npz", dtype=dt, m=A.m, n=A.n, thr=thr, select=select,
+                            recover=recover, pct=pct, a_cp=A.cp, a_jc=A.jc, a_ir=A.ir.astype(np.int32),
+                            a_val=A.val, c_cp=R.cp, c_jc=R.jc, c_ir=R.ir.astype(np.int32), c_val=R.val)
     print("fixtures written to", OUT)
 
 

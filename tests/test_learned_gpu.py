@@ -43,9 +43,21 @@ def test_learned_requires_predictor():
     assert ei.value.status == cb.ERR_INVALID
 
 
-@pytest.mark.parametrize("log2_nnz,expect", [(1.0, "fallback"), (11.0, "overflow"), (20.0, "fits")])
-def test_learned_constant_predictors_exact(log2_nnz, expect):
+def unit_tier_nnz(A, B):
+    """Exact output sizes of the columns that go to the unit tier (flop_j > 4096)."""
+    flop, nnz = oracle.symbolic(A, B)
+    return nnz[flop > 4096]
+
+
+@pytest.mark.parametrize("expect", ["fallback", "overflow", "fits"])
+def test_learned_constant_predictors_exact(expect):
+    """fallback: 1 output per column, far below the total; overflow: every column
+    predicted at 3/4 of the mean size, so the total fits within the capacity
+    margin but the large columns' units exceed their value window; fits: every column predicted at the largest size."""
     A = cb.gen_rmat(14, 8)
+    sizes = unit_tier_nnz(A, A)
+    log2_nnz = {"fallback": 1.0, "overflow": float(np.log2(1 + 0.75 * sizes.mean())),
+                "fits": float(np.log2(1 + sizes.max()))}[expect]
     cb.set_predictor(constant_predictor(log2_nnz))
     got, st = learned(A, A)
     want, _ = oracle.spgemm(A, A)
@@ -56,6 +68,8 @@ def test_learned_constant_predictors_exact(log2_nnz, expect):
         assert st["learned_fallback"] == 0 and st["units_light"] > 0
     if expect == "overflow":
         assert st["overflow_cols"] > 0
+    if expect == "fits":
+        assert st["overflow_cols"] == 0
 
 
 @pytest.mark.parametrize("semiring,dtype", [("plus_times", "int64"), ("plus_times", "bool"), ("min_plus", "int64"),
@@ -65,7 +79,7 @@ def test_learned_semirings(semiring, dtype):
     A = cb.gen_rmat(13, 8).astype(dtype)
     if dtype == "int64":
         A.val = rng.integers(-9, 10, A.nnz).astype(np.int64)
-    cb.set_predictor(constant_predictor(13.0))
+    cb.set_predictor(constant_predictor(float(np.log2(1 + unit_tier_nnz(A, A).max()))))
     got, st = learned(A, A, semiring)
     assert st["learned_fallback"] == 0
     want, _ = oracle.spgemm(A, A, semiring)
