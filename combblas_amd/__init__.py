@@ -107,6 +107,18 @@ def lib() -> C.CDLL:
         L.cbx_gen_powerlaw_stochastic.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_int64, C.c_uint64,
                                                   P(cbx_dcsc)]
         L.cbx_gen_powerlaw_stochastic.restype = C.c_int32
+        L.cbx_gen_stencil27.argtypes = [C.c_int32, P(cbx_dcsc)]
+        L.cbx_gen_stencil27.restype = C.c_int32
+        L.cbx_gen_aggregation.argtypes = [C.c_int32, C.c_int32, P(cbx_dcsc)]
+        L.cbx_gen_aggregation.restype = C.c_int32
+        L.cbx_dcsc_transpose.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p]
+        L.cbx_dcsc_transpose.restype = C.c_int32
+        L.cbx_spgemm_batched.argtypes = [C.c_int64, P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, P(cbx_dcsc),
+                                         P(cbx_spgemm_stats), C.c_void_p]
+        L.cbx_spgemm_batched.restype = C.c_int32
+        L.cbx_galerkin_batched.argtypes = [C.c_int64, P(cbx_dcsc), P(cbx_dcsc), P(cbx_dcsc), P(cbx_spgemm_stats),
+                                           C.c_void_p]
+        L.cbx_galerkin_batched.restype = C.c_int32
         L.cbx_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc),
                                     P(cbx_mcl_stats), C.c_void_p]
         L.cbx_mcl_prune.restype = C.c_int32
@@ -361,6 +373,58 @@ def spgemm(A: DeviceDcsc, B: DeviceDcsc, semiring: str = "plus_times", mode: str
                             C.byref(st), stream))
     stats = {k: getattr(st, k) for k, _ in cbx_spgemm_stats._fields_}
     return DeviceDcsc(out), stats
+
+
+def gen_stencil27(g: int) -> Dcsc:
+    """27-point Poisson stencil on a g^3 grid (FP64), as in GalerkinNew.cpp's grid3d input."""
+    s = cbx_dcsc()
+    _check(lib().cbx_gen_stencil27(g, C.byref(s)))
+    return Dcsc.from_struct(s, lib().cbx_dcsc_free)
+
+
+def gen_aggregation(g: int, transposed: bool = True) -> Dcsc:
+    """2x2x2 aggregation of a g^3 grid: T = R^T (fine x coarse) or R (coarse x fine), FP64."""
+    s = cbx_dcsc()
+    _check(lib().cbx_gen_aggregation(g, 1 if transposed else 0, C.byref(s)))
+    return Dcsc.from_struct(s, lib().cbx_dcsc_free)
+
+
+def transpose(X: DeviceDcsc, stream: int | None = None) -> DeviceDcsc:
+    """SpDCCols::Transpose on the device."""
+    out = cbx_dcsc()
+    _check(lib().cbx_dcsc_transpose(C.byref(X.s), C.byref(out), stream))
+    return DeviceDcsc(out)
+
+
+def _desc_array(mats) -> C.Array:
+    arr = (cbx_dcsc * len(mats))()
+    for i, M in enumerate(mats):
+        arr[i] = M.s
+    return arr
+
+
+def _stats(st: cbx_spgemm_stats) -> dict:
+    return {k: getattr(st, k) for k, _ in cbx_spgemm_stats._fields_}
+
+
+def spgemm_batched(As, Bs, semiring: str = "plus_times", mode: str = "two_phase",
+                   stream: int | None = None) -> tuple[list[DeviceDcsc], dict]:
+    """C[i] = A[i]*B[i] for a batch of device matrices, computed as one packed product."""
+    n = len(As)
+    out = (cbx_dcsc * n)()
+    st = cbx_spgemm_stats()
+    _check(lib().cbx_spgemm_batched(n, _desc_array(As), _desc_array(Bs), SEMIRINGS[semiring], MODES[mode], out,
+                                    C.byref(st), stream))
+    return [DeviceDcsc(out[i]) for i in range(n)], _stats(st)
+
+
+def galerkin_batched(As, Ts, stream: int | None = None) -> tuple[list[DeviceDcsc], dict]:
+    """SAT[i] = T[i]^T * (A[i] * T[i]) for a batch (GalerkinNew.cpp:105-106)."""
+    n = len(As)
+    out = (cbx_dcsc * n)()
+    st = cbx_spgemm_stats()
+    _check(lib().cbx_galerkin_batched(n, _desc_array(As), _desc_array(Ts), out, C.byref(st), stream))
+    return [DeviceDcsc(out[i]) for i in range(n)], _stats(st)
 
 
 def mcl_prune(C_: DeviceDcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, pct: float = 0.9,

@@ -258,3 +258,112 @@ extern "C" cbx_status cbx_gen_powerlaw_stochastic(int64_t n, double mean, double
   }
   return CBX_OK;
 }
+
+// ----------------------------------------------------------------------------
+// Galerkin operands (ReleaseTests/GalerkinNew.cpp reads them from files:
+// grid3d_k5.txt and restrict_T_grid3d_k5.txt).  Vertex (x,y,z) of a g^3 grid
+// is row/column x + g*(y + g*z).
+namespace cbx {
+
+static cbx_status alloc_host_dcsc(cbx_dcsc* out, int64_t m, int64_t n, int64_t nnz, int64_t nzc, cbx_dtype dt,
+                                  size_t vs) {
+  std::memset(out, 0, sizeof(*out));
+  out->m = m; out->n = n; out->nnz = nnz; out->nzc = nzc;
+  out->dtype = dt; out->location = CBX_HOST;
+  out->cp = (int64_t*)std::malloc(sizeof(int64_t) * (nzc + 1));
+  out->jc = (int64_t*)std::malloc(sizeof(int64_t) * (nzc > 0 ? nzc : 1));
+  out->ir = (int64_t*)std::malloc(sizeof(int64_t) * (nnz > 0 ? nnz : 1));
+  out->val = std::malloc(vs * (nnz > 0 ? nnz : 1));
+  if (!out->cp || !out->jc || !out->ir || !out->val) {
+    cbx_dcsc_free(out);
+    return set_error(CBX_ERR_NOMEM, "generator: host allocation failed");
+  }
+  return CBX_OK;
+}
+
+}  // namespace cbx
+
+// 27-point Poisson stencil: 26 on the diagonal, -1 for each of the up to 26
+// neighbours inside the grid (Dirichlet boundary).  Symmetric, FP64.
+extern "C" cbx_status cbx_gen_stencil27(int32_t g, cbx_dcsc* out) {
+  if (!out) return set_error(CBX_ERR_INVALID, "cbx_gen_stencil27: out is null");
+  if (g < 1 || g > 1024) return set_error(CBX_ERR_INVALID, "cbx_gen_stencil27: edge must be in [1, 1024]");
+  const int64_t n = (int64_t)g * g * g;
+  auto deg = [g](int64_t v) {
+    const int x = (int)(v % g), y = (int)((v / g) % g), z = (int)(v / ((int64_t)g * g));
+    const int a = 1 + (x > 0) + (x < g - 1), b = 1 + (y > 0) + (y < g - 1), c = 1 + (z > 0) + (z < g - 1);
+    return (int64_t)(a * b * c);
+  };
+  std::vector<int64_t> cp(n + 1, 0);
+  for (int64_t v = 0; v < n; ++v) cp[v + 1] = cp[v] + deg(v);
+  cbx_status st = alloc_host_dcsc(out, n, n, cp[n], n, CBX_F64, sizeof(double));
+  if (st) return st;
+  std::memcpy(out->cp, cp.data(), sizeof(int64_t) * (n + 1));
+  double* val = (double*)out->val;
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < n; ++v) {
+    out->jc[v] = v;
+    const int x = (int)(v % g), y = (int)((v / g) % g), z = (int)(v / ((int64_t)g * g));
+    int64_t w = cp[v];
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {  // rows ascend: z slowest, x fastest
+          const int X = x + dx, Y = y + dy, Z = z + dz;
+          if (X < 0 || Y < 0 || Z < 0 || X >= g || Y >= g || Z >= g) continue;
+          out->ir[w] = X + (int64_t)g * (Y + (int64_t)g * Z);
+          val[w] = (dx | dy | dz) ? -1.0 : 26.0;
+          ++w;
+        }
+  }
+  return CBX_OK;
+}
+
+// 2x2x2 aggregation: coarse vertex (x/2, y/2, z/2) of a ceil(g/2)^3 grid.
+// transposed = 0: R, coarse x fine, one 1.0 per column.  transposed = 1:
+// T = R^T, fine x coarse, the up to 8 fine vertices of each aggregate in a
+// column (the restrict_T input of GalerkinNew.cpp).
+extern "C" cbx_status cbx_gen_aggregation(int32_t g, int32_t transposed, cbx_dcsc* out) {
+  if (!out) return set_error(CBX_ERR_INVALID, "cbx_gen_aggregation: out is null");
+  if (g < 1 || g > 1024) return set_error(CBX_ERR_INVALID, "cbx_gen_aggregation: edge must be in [1, 1024]");
+  const int gc = (g + 1) / 2;
+  const int64_t n = (int64_t)g * g * g, nc = (int64_t)gc * gc * gc;
+  if (!transposed) {
+    cbx_status st = alloc_host_dcsc(out, nc, n, n, n, CBX_F64, sizeof(double));
+    if (st) return st;
+    double* val = (double*)out->val;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < n; ++v) {
+      const int x = (int)(v % g), y = (int)((v / g) % g), z = (int)(v / ((int64_t)g * g));
+      out->cp[v] = v;
+      out->jc[v] = v;
+      out->ir[v] = x / 2 + (int64_t)gc * (y / 2 + (int64_t)gc * (z / 2));
+      val[v] = 1.0;
+    }
+    out->cp[n] = n;
+    return CBX_OK;
+  }
+  auto span = [g](int c) { return (2 * c + 1 < g) ? 2 : 1; };
+  std::vector<int64_t> cp(nc + 1, 0);
+  for (int64_t J = 0; J < nc; ++J) {
+    const int X = (int)(J % gc), Y = (int)((J / gc) % gc), Z = (int)(J / ((int64_t)gc * gc));
+    cp[J + 1] = cp[J] + span(X) * span(Y) * span(Z);
+  }
+  cbx_status st = alloc_host_dcsc(out, n, nc, n, nc, CBX_F64, sizeof(double));
+  if (st) return st;
+  std::memcpy(out->cp, cp.data(), sizeof(int64_t) * (nc + 1));
+  double* val = (double*)out->val;
+#pragma omp parallel for schedule(static)
+  for (int64_t J = 0; J < nc; ++J) {
+    out->jc[J] = J;
+    const int X = (int)(J % gc), Y = (int)((J / gc) % gc), Z = (int)(J / ((int64_t)gc * gc));
+    int64_t w = cp[J];
+    for (int dz = 0; dz < span(Z); ++dz)
+      for (int dy = 0; dy < span(Y); ++dy)
+        for (int dx = 0; dx < span(X); ++dx) {
+          out->ir[w] = (2 * X + dx) + (int64_t)g * ((2 * Y + dy) + (int64_t)g * (2 * Z + dz));
+          val[w] = 1.0;
+          ++w;
+        }
+  }
+  return CBX_OK;
+}

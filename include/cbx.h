@@ -137,6 +137,18 @@ cbx_status cbx_gen_rmat(int32_t scale, int32_t edgefactor, double a, double b, d
 cbx_status cbx_gen_powerlaw_stochastic(int64_t n, double mean, double alpha, int64_t dmax, uint64_t seed,
                                        cbx_dcsc* out);
 
+/* 27-point Poisson stencil on a g^3 grid (26 on the diagonal, -1 for every
+ * neighbour inside the grid), dtype CBX_F64; vertex (x,y,z) is x+g*(y+g*z).
+ * The operand of ReleaseTests/GalerkinNew.cpp (read there from
+ * grid3d_k5.txt). */
+cbx_status cbx_gen_stencil27(int32_t g, cbx_dcsc* out);
+
+/* 2x2x2 aggregation of a g^3 grid onto a ceil(g/2)^3 grid (odd edges give
+ * smaller aggregates), entries 1.0, dtype CBX_F64: transposed = 0 gives R
+ * (coarse x fine), 1 gives T = R^T (fine x coarse; restrict_T_grid3d_k5.txt
+ * in GalerkinNew.cpp). */
+cbx_status cbx_gen_aggregation(int32_t g, int32_t transposed, cbx_dcsc* out);
+
 /* ----------------------------------------------------------------- SpGEMM */
 
 /* Modes of the local product. */
@@ -320,6 +332,28 @@ cbx_status cbx_mcl_prune(const cbx_dcsc* C, double hard_threshold, int64_t selec
 cbx_status cbx_mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, double hard_threshold, int64_t select_num,
                           int64_t recover_num, double recover_pct, int64_t phases, cbx_spgemm_mode mode,
                           cbx_dcsc* out, cbx_mcl_stats* stats, void* stream);
+
+/* ----------------------------------------------- transpose, batches, Galerkin */
+
+/* SpDCCols::Transpose (SpDCCols.cpp:871-1011): out = X^T as a new device DCSC
+ * (rows sorted within columns). */
+cbx_status cbx_dcsc_transpose(const cbx_dcsc* X, cbx_dcsc* out, void* stream);
+
+/* count independent products C[i] = A[i]*B[i] (arrays of count device
+ * descriptors, one dtype) computed together: the batch is packed into one
+ * block-diagonal product, so every SpGEMM kernel is launched once for the
+ * whole batch, then split into count new device matrices.  Each C[i] equals
+ * what cbx_spgemm(&A[i], &B[i], ...) returns.  1 <= count <= 4096; the packed
+ * batch must hold fewer than 2^31 rows, columns and nonzeros. */
+cbx_status cbx_spgemm_batched(int64_t count, const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring semiring,
+                              cbx_spgemm_mode mode, cbx_dcsc* C, cbx_spgemm_stats* stats, void* stream);
+
+/* Batched Galerkin triple products (ReleaseTests/GalerkinNew.cpp:94-112):
+ * for each i, S = T[i]^T, AT = A[i]*T[i], SAT[i] = S*AT with PlusTimes, where
+ * A[i] is square and T[i] = R^T is the fine x coarse prolongation.  One
+ * transpose and two batched products for the whole batch. */
+cbx_status cbx_galerkin_batched(int64_t count, const cbx_dcsc* A, const cbx_dcsc* T, cbx_dcsc* SAT,
+                                cbx_spgemm_stats* stats, void* stream);
 
 #ifdef __cplusplus
 }

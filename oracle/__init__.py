@@ -64,6 +64,8 @@ def _load_oracle() -> C.CDLL:
         L.orc_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc),
                                     C.c_void_p]
         L.orc_mcl_prune.restype = C.c_int32
+        L.orc_galerkin_direct.argtypes = [C.c_int32, P(cbx_dcsc)]
+        L.orc_galerkin_direct.restype = C.c_int32
         L.orc_max_threads.restype = C.c_int32
         L.orc_set_threads.argtypes = [C.c_int32]
         _orc = L
@@ -206,6 +208,22 @@ def mcl_prune(A: Dcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, 
         raise RuntimeError(f"orc_mcl_prune failed with status {st}")
     X = Dcsc.from_struct(out, L.orc_free)
     return (X, cut) if return_cut else X
+
+
+def galerkin_direct(g: int) -> Dcsc:
+    """R*A*R^T of the 27-point stencil under 2x2x2 aggregation, summed directly (no SpGEMM)."""
+    L = _load_oracle()
+    out = cbx_dcsc()
+    st = L.orc_galerkin_direct(g, C.byref(out))
+    if st != 0:
+        raise RuntimeError(f"orc_galerkin_direct failed with status {st}")
+    return Dcsc.from_struct(out, L.orc_free)
+
+
+def transpose(X: Dcsc) -> Dcsc:
+    """Host transpose (rows sorted within columns)."""
+    cols = X.col_ids()
+    return Dcsc.from_coo(X.n, X.m, cols, X.ir, X.val, X.dtype)
 
 
 def ref_mcl_prune(A: Dcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, pct: float = 0.9) -> Dcsc:

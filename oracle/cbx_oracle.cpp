@@ -579,6 +579,60 @@ int32_t orc_mcl_prune(const cbx_dcsc* A, double thr, int64_t select, int64_t rec
   return CBX_ERR_UNSUPPORTED;
 }
 
+// Galerkin coarse operator of the 27-point stencil under 2x2x2 aggregation,
+// summed directly: (R A R^T)(I,J) = sum over fine i in I, j in J of A(i,j),
+// with A(j,j) = 26 and A(i,j) = -1 for the up to 26 grid neighbours i of j.
+// This is what the product S*(A*T) of GalerkinNew.cpp:105-106 computes,
+// obtained without any SpGEMM; every value is an integer, so FP64 is exact.
+int32_t orc_galerkin_direct(int32_t g, cbx_dcsc* out) {
+  const int gc = (g + 1) / 2;
+  const int64_t nc = (int64_t)gc * gc * gc;
+  std::vector<double> vals(nc * 27, 0.0);
+  std::vector<uint8_t> hit(nc * 27, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t J = 0; J < nc; ++J) {
+    const int X = (int)(J % gc), Y = (int)((J / gc) % gc), Z = (int)(J / ((int64_t)gc * gc));
+    for (int jz = 2 * Z; jz < std::min(2 * Z + 2, g); ++jz)
+      for (int jy = 2 * Y; jy < std::min(2 * Y + 2, g); ++jy)
+        for (int jx = 2 * X; jx < std::min(2 * X + 2, g); ++jx)
+          for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+              for (int dx = -1; dx <= 1; ++dx) {
+                const int ix = jx + dx, iy = jy + dy, iz = jz + dz;
+                if (ix < 0 || iy < 0 || iz < 0 || ix >= g || iy >= g || iz >= g) continue;
+                const int oz = iz / 2 - Z + 1, oy = iy / 2 - Y + 1, ox = ix / 2 - X + 1;
+                const int slot = ox + 3 * (oy + 3 * oz);
+                vals[J * 27 + slot] += (dx | dy | dz) ? -1.0 : 26.0;
+                hit[J * 27 + slot] = 1;
+              }
+  }
+  int64_t nnz = 0;
+  for (size_t i = 0; i < hit.size(); ++i) nnz += hit[i];
+  std::memset(out, 0, sizeof(*out));
+  out->m = nc; out->n = nc; out->nnz = nnz; out->nzc = nc;
+  out->dtype = CBX_F64; out->location = CBX_HOST;
+  out->cp = (int64_t*)std::malloc(sizeof(int64_t) * (nc + 1));
+  out->jc = (int64_t*)std::malloc(sizeof(int64_t) * (nc ? nc : 1));
+  out->ir = (int64_t*)std::malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+  double* v = (double*)std::malloc(sizeof(double) * (nnz ? nnz : 1));
+  out->val = v;
+  int64_t w = 0;
+  for (int64_t J = 0; J < nc; ++J) {
+    const int X = (int)(J % gc), Y = (int)((J / gc) % gc), Z = (int)(J / ((int64_t)gc * gc));
+    out->jc[J] = J;
+    out->cp[J] = w;
+    for (int slot = 0; slot < 27; ++slot) {  // slot order is ascending coarse row order
+      if (!hit[J * 27 + slot]) continue;
+      const int ox = slot % 3 - 1, oy = (slot / 3) % 3 - 1, oz = slot / 9 - 1;
+      out->ir[w] = (X + ox) + (int64_t)gc * ((Y + oy) + (int64_t)gc * (Z + oz));
+      v[w] = vals[J * 27 + slot];
+      ++w;
+    }
+  }
+  out->cp[nc] = nnz;
+  return CBX_OK;
+}
+
 int32_t orc_bfs(const cbx_dcsc* A, int64_t root, int64_t* parents) {
   const int64_t n = A->n;
   std::fill(parents, parents + n, (int64_t)-1);
