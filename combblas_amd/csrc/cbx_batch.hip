@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_unpack_cols(const OutSeg* __restrict__ 
 __global__ void k_entry_keys(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
                              const int64_t* __restrict__ ir, int64_t nzc, uint64_t* __restrict__ key,
                              int32_t* __restrict__ idx) {
-  const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t q = global_wave();
   if (q >= nzc) return;
   const uint64_t c = (uint64_t)jc[q];
   for (int64_t e = cp[q] + lane_id(); e < cp[q + 1]; e += kWave) {
@@ -316,7 +316,7 @@ static cbx_status transpose_typed(const cbx_dcsc* X, cbx_dcsc* out, hipStream_t 
   CBX_TRY(di1.get(&i1, nnz));
   CBX_TRY(dh.get(&head, nnz + 1));
   CBX_TRY(dhp.get(&hpos, nnz + 1));
-  if (X->nzc) k_entry_keys<<<nblocks(X->nzc * kWave, 256), 256, 0, s>>>(X->cp, X->jc, X->ir, X->nzc, k0, i0);
+  if (X->nzc) k_entry_keys<<<wave_grid(X->nzc), 256, 0, s>>>(X->cp, X->jc, X->ir, X->nzc, k0, i0);
   CBX_HIP(hipGetLastError());
   int rbits = 1;
   while (rbits < 31 && ((int64_t)1 << rbits) < X->m) ++rbits;

@@ -34,7 +34,7 @@ __device__ __forceinline__ uint64_t val_bits(T v) {
 template <class T>
 __global__ void k_col_checksum(const int64_t* __restrict__ cp, const int64_t* __restrict__ ir,
                                const T* __restrict__ val, int64_t nzc, uint64_t salt, uint64_t* __restrict__ out) {
-  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t col = global_wave();
   if (col >= nzc) return;
   uint64_t acc = 0;
   for (int64_t e = cp[col] + lane_id(); e < cp[col + 1]; e += kWave)
@@ -48,7 +48,7 @@ __global__ void k_col_checksum(const int64_t* __restrict__ cp, const int64_t* __
 __global__ void k_check_dcsc(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
                              const int64_t* __restrict__ ir, int64_t nzc, int64_t nnz, int64_t nrows, int64_t ncols,
                              int* __restrict__ bad) {
-  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t col = global_wave();
   if (col >= nzc) return;
   const int lane = lane_id();
   const int64_t b = cp[col], e = cp[col + 1];
@@ -78,7 +78,7 @@ extern "C" cbx_status cbx_dcsc_column_checksums(const cbx_dcsc* X, uint64_t salt
   if (X->nzc == 0) return CBX_OK;
   uint64_t* d = nullptr;
   CBX_HIP(hipMalloc(&d, sizeof(uint64_t) * X->nzc));
-  const unsigned g = (unsigned)((X->nzc * kWave + 255) / 256);
+  const dim3 g = wave_grid(X->nzc);
   switch (X->dtype) {
     case CBX_I64: k_col_checksum<int64_t><<<g, 256>>>(X->cp, X->ir, (const int64_t*)X->val, X->nzc, salt, d); break;
     case CBX_F64: k_col_checksum<double><<<g, 256>>>(X->cp, X->ir, (const double*)X->val, X->nzc, salt, d); break;
@@ -99,7 +99,7 @@ extern "C" cbx_status cbx_dcsc_check(const cbx_dcsc* X) {
   int* d = nullptr;
   CBX_HIP(hipMalloc(&d, sizeof(int)));
   CBX_HIP(hipMemset(d, 0, sizeof(int)));
-  k_check_dcsc<<<(unsigned)((X->nzc * kWave + 255) / 256), 256>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->m, X->n, d);
+  k_check_dcsc<<<wave_grid(X->nzc), 256>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->m, X->n, d);
   int bad = 0;
   hipError_t e = hipMemcpy(&bad, d, sizeof(int), hipMemcpyDeviceToHost);
   (void)hipFree(d);

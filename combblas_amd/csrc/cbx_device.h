@@ -19,6 +19,20 @@ namespace cbx {
 
 constexpr int kWave = 64;
 
+// Index of this wave in a 1-D or 2-D grid of 1-D blocks (a 2-D grid keeps
+// the per-dimension work-item count of wave-per-column launches below 2^32).
+__device__ __forceinline__ int64_t global_wave() {
+  return (((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+}
+
+// Grid with one wave per item for blocks of `bs` threads.
+inline dim3 wave_grid(int64_t nwaves, int bs = 256) {
+  int64_t blocks = (nwaves * kWave + bs - 1) / bs;
+  if (blocks < 1) blocks = 1;
+  const int64_t X = (int64_t)1 << 22;
+  return blocks <= X ? dim3((unsigned)blocks) : dim3((unsigned)X, (unsigned)((blocks + X - 1) / X));
+}
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
 

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_mcl_stats(const int64_t* __restrict__ c
                                                    T* __restrict__ cut, int64_t* __restrict__ cnt,
                                                    int32_t* __restrict__ job, int32_t* __restrict__ jobs,
                                                    unsigned long long* __restrict__ counters) {
-  const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t q = global_wave();
   if (q >= nzc) return;  // whole wave
   const int lane = lane_id();
   const int64_t b = cp[q], e = cp[q + 1];
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void k_mcl_write(const int64_t* __restrict__ c
                                                    const int32_t* __restrict__ cpos, int64_t* __restrict__ ocp,
                                                    int64_t* __restrict__ ojc, int64_t* __restrict__ oir,
                                                    T* __restrict__ oval) {
-  const int64_t q = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t q = global_wave();
   if (q >= nzc || cnt[q] == 0) return;  // whole wave
   const int lane = lane_id();
   const T c = cut[q];
@@ -339,7 +339,7 @@ static cbx_status mcl_prune_typed(const cbx_dcsc* C, double thr_d, int64_t selec
   CBX_HIP(hipMemsetAsync(cnt + nzc, 0, sizeof(int64_t), s));
   CBX_HIP(hipMemsetAsync(flag + nzc, 0, sizeof(int32_t), s));
   if (nzc) {
-    k_mcl_stats<T><<<nblocks(nzc * kWave, 256), 256, 0, s>>>(C->cp, (const T*)C->val, nzc, thr, select, recover,
+    k_mcl_stats<T><<<wave_grid(nzc), 256, 0, s>>>(C->cp, (const T*)C->val, nzc, thr, select, recover,
                                                               pct, cut, cnt, job, jobs, counters);
     CBX_HIP(hipGetLastError());
   }
@@ -368,7 +368,7 @@ static cbx_status mcl_prune_typed(const cbx_dcsc* C, double thr_d, int64_t selec
   CBX_HIP(hipStreamSynchronize(s));
   CBX_TRY(alloc_dcsc(out, C->m, C->n, onnz, onzc, C->dtype));
   if (nzc)
-    k_mcl_write<T><<<nblocks(nzc * kWave, 256), 256, 0, s>>>(C->cp, C->jc, C->ir, (const T*)C->val, nzc, cut, cnt,
+    k_mcl_write<T><<<wave_grid(nzc), 256, 0, s>>>(C->cp, C->jc, C->ir, (const T*)C->val, nzc, cut, cnt,
                                                               off, cpos, out->cp, out->jc, out->ir, (T*)out->val);
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipMemcpyAsync(out->cp + onzc, &onnz, sizeof(int64_t), hipMemcpyHostToDevice, s));

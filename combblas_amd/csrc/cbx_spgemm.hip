@@ -170,7 +170,7 @@ __global__ void k_bpos(const int64_t* __restrict__ irB, int64_t nnzB, const int3
 // flop_j (estimateFLOP, mtSpGEMM.h:1112-1130): one wave per column of B.
 __global__ void k_flop(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
                        const int64_t* __restrict__ cpA, int64_t* __restrict__ flop) {
-  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t col = global_wave();
   if (col >= nzcB) return;
   const int lane = lane_id();
   long long f = 0;
@@ -243,7 +243,7 @@ __global__ void k_set_last(int64_t* cp, const int64_t* nsel, const int64_t* tota
 __global__ void k_validate(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
                            const int64_t* __restrict__ ir, int64_t nzc, int64_t nnz, int64_t nrows,
                            int64_t ncols, int flagbit, int* __restrict__ err) {
-  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t col = global_wave();
   if (col >= nzc) return;
   const int lane = lane_id();
   const int64_t b = cp[col], e = cp[col + 1];
@@ -838,7 +838,7 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
 __global__ void k_features(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
                            const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA, int64_t m,
                            float* __restrict__ feats) {
-  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t col = global_wave();
   if (col >= nzcB) return;
   const int lane = lane_id();
   long long f = 0, mx = 0, lo = m, hi = -1;
@@ -906,7 +906,7 @@ __global__ void k_row_hist(const int32_t* __restrict__ rowsA, int64_t nnzA, int 
 __global__ void k_col_blocks(const int64_t* __restrict__ cpB, int64_t nzcB, const int32_t* __restrict__ bpos,
                              const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                              int32_t* __restrict__ blo, int32_t* __restrict__ bhi) {
-  const int64_t col = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t col = global_wave();
   if (col >= nzcB) return;
   int lo = 0x7fffffff, hi = -1;
   for (int64_t e = cpB[col] + lane_id(); e < cpB[col + 1]; e += kWave) {
@@ -1188,7 +1188,7 @@ __global__ __launch_bounds__(NT) void k_num_learned(
 // P[p*(S+1)+t] = #entries of column p with row < t*kBR (binary search).
 __global__ void k_ptable_wave(const int64_t* __restrict__ cpA, int64_t nzcA, const int32_t* __restrict__ rowsA,
                               int S, int32_t* __restrict__ P) {
-  const int64_t p = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t p = global_wave();
   if (p >= nzcA) return;
   const int64_t b = cpA[p];
   const int len = (int)(cpA[p + 1] - b);
@@ -1332,13 +1332,13 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_ovf, 2, s, &ovf));
   CBX_TRY(dalloc(b_ct, nzcB, s, &col_total));
   // prediction
-  k_features<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, m, feats);
+  k_features<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, m, feats);
   mlp_infer_launch(cfg.predictor, feats, nzcB, logpred, s);
   // block prior and per-column block range
   CBX_HIP(hipMemsetAsync(hist, 0, sizeof(unsigned long long) * S, s));
   k_row_hist<<<256, 256, sizeof(unsigned int) * S, s>>>(rowsA, A->nnz, S, hist);
   k_hist_to_w<<<nblk(S, 256), 256, 0, s>>>(hist, S, 1.0 / (double)std::max<int64_t>(A->nnz, 1), w);
-  k_col_blocks<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, blo, bhi);
+  k_col_blocks<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, blo, bhi);
   // units in column order: count, scan, write
   const float target = cfg.learned_target * (float)kLearnK;
   k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
@@ -1439,7 +1439,6 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_TRY(dalloc(b_flop_s, nzcB, s, &flop_s));
   CBX_TRY(dalloc(b_order, nzcB, s, &order));
   CBX_TRY(dalloc(b_order_s, nzcB, s, &order_s));
-  CBX_TRY(dalloc(b_P, (size_t)nzcA * (S + 1), s, &P));
   CBX_TRY(dalloc(b_nnz, nzcB + 1, s, &nnzcol));
   CBX_TRY(dalloc(b_cp, nzcB + 1, s, &cpC));
   CBX_TRY(dalloc(b_thr, 8, s, &thr));
@@ -1455,8 +1454,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const T* valB = (const T*)B->val;
 
   // --- validate both operands before any indexed access
-  k_validate<<<nblk(nzcA * kWave, 256), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
-  k_validate<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
+  k_validate<<<wave_grid(nzcA), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
+  k_validate<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
   {
     int herr = 0;
     CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1472,11 +1471,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   k_colmap<<<nblk(nzcA, 256), 256, 0, s>>>(A->jc, nzcA, colmap);
   k_rows32<<<nblk(nnzA, 256), 256, 0, s>>>(A->ir, nnzA, rowsA);
   k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
-  k_flop<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, flop);
-  if (use_units)
-    k_ptable_wave<<<nblk(nzcA * kWave, 256), 256, 0, s>>>(A->cp, nzcA, rowsA, S, P);
-  else
-    k_ptable<<<nblk(nzcA, 256), 256, 0, s>>>(A->cp, nzcA, rowsA, S, TR, P);
+  k_flop<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, flop);
   k_iota_i32<<<nblk(nzcB, 256), 256, 0, s>>>(order, nzcB);
 
   // sort columns by flop, descending (largest work first)
@@ -1506,6 +1501,16 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const int64_t n_dense = hb[0], n_h8 = hb[1] - hb[0], n_h2 = hb[2] - hb[1], n_h256 = hb[3] - hb[2];
   const int64_t n_heavy_max = hb[4];  // columns whose flops (an upper bound of nnz) exceed heavy_nnz
   const int64_t total_flop = hb[5];
+  // tile pointers only when some column is beyond the hash tier (the table is
+  // nzc(A) x m/4096 entries, large for tall packed batches of small columns)
+  P = nullptr;
+  if (n_dense) {
+    CBX_TRY(dalloc(b_P, (size_t)nzcA * (S + 1), s, &P));
+    if (use_units)
+      k_ptable_wave<<<wave_grid(nzcA), 256, 0, s>>>(A->cp, nzcA, rowsA, S, P);
+    else
+      k_ptable<<<nblk(nzcA, 256), 256, 0, s>>>(A->cp, nzcA, rowsA, S, TR, P);
+  }
   tm.mark(1, s);
 
   // --- learned single pass (unit-tier columns), with the exact hash-tier
@@ -1657,8 +1662,8 @@ cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, h
   CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
   CBX_TRY(dalloc(b_err, 1, s, &err));
   CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
-  k_validate<<<nblk(nzcA * kWave, 256), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
-  k_validate<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
+  k_validate<<<wave_grid(nzcA), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
+  k_validate<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
   int herr = 0;
   CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipStreamSynchronize(s));
@@ -1667,7 +1672,7 @@ cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, h
   if (nzcA) k_colmap<<<nblk(nzcA, 256), 256, 0, s>>>(A->jc, nzcA, colmap);
   if (nnzA) k_rows32<<<nblk(nnzA, 256), 256, 0, s>>>(A->ir, nnzA, rowsA);
   if (nnzB) k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
-  k_features<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, A->m, feats);
+  k_features<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, A->m, feats);
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipStreamSynchronize(s));
   return CBX_OK;
@@ -1685,7 +1690,7 @@ cbx_status spgemm_flops(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop_host
   k_fill_i32<<<nblk(A->n, 256), 256, 0, s>>>(colmap, A->n, -1);
   if (nzcA) k_colmap<<<nblk(nzcA, 256), 256, 0, s>>>(A->jc, nzcA, colmap);
   if (nnzB) k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
-  k_flop<<<nblk(nzcB * kWave, 256), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, flop);
+  k_flop<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, flop);
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipMemcpyAsync(flop_host, flop, sizeof(int64_t) * nzcB, hipMemcpyDeviceToHost, s));
   CBX_HIP(hipStreamSynchronize(s));
