@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark of the node-local SpGEMM hot path on one MI355X.
+"""Benchmark of the node-local sparse kernels on one MI355X.
 
-Workload: C = A*A on PlusTimes<int64>, A = R-MAT scale 20, edge factor 16
-(a=.57, b=c=.19, d=.05, duplicates summed, values uniform in [1,9]),
-int64 DCSC in and out; A is resident in HBM before the timed region.  A
-"step" is one complete local product (symbolic + numeric, output arrays
-allocated from the library's cache and returned after the step).
+Headline workload: C = A*A on PlusTimes<int64>, A = R-MAT scale 20, edge
+factor 16 (a=.57, b=c=.19, d=.05, duplicates summed, values uniform in
+[1,9]), int64 DCSC in and out; A is resident in HBM before the timed region.
+A "step" is one complete local product, output arrays allocated from the
+library's cache and returned after the step.  It is timed for the plain
+two-phase path (symbolic + numeric) and for the learned single pass (the
+trained nnz predictor replaces the symbolic pass of the large columns);
+`value` is the learned single pass's GFLOPS (2 x multiply-adds / s) when the
+trained model is present, and both outputs are checked bit-exact at full
+size (device structure check + per-column linear checksums against their
+closed form).
 
-Prints one JSON line: GFLOPS (2 x multiply-adds / s) as `value`, the
-algorithmic HBM GB/s of the whole step, the roofline of the dominant
-kernel (the row-range-unit numeric kernels, timed with HIP events on the
-library's stream), a full-size bit-exact check of the output (device
-structure check + per-column linear checksums against their closed form),
-and the CPU baseline (the reference's LocalHybridSpGEMM on this host's
-cores over a bounded column sample).
+Beside it, in the same line: the roofline of the dominant kernel (HIP
+events on the library's stream), the CPU baseline (the reference's
+LocalHybridSpGEMM on this host's cores over a bounded column sample), and
+the other workloads of the same kernels: top-down BFS through SpMSpV on the
+symmetrised s20 graph (64 roots), one MCL expansion step in FP32 (n = 10^6,
+power-law degrees, prune 1e-4, top-1000), and 256 batched Galerkin products
+R*A*R^T in FP64.
 """
 from __future__ import annotations
 
@@ -33,9 +39,13 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "SpGEMM GFLOPS + HBM GB/s (R-MAT s20 ef16 A·A); learned-vs-2-phase speedup"
 
 
-def dcsc_bytes(nnz: int, nzc: int) -> int:
-    """Algorithmic bytes of an int64 DCSC: cp+jc (16 per column) + 8 + ir+val (16 per entry)."""
-    return 16 * nzc + 8 + 16 * nnz
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def dcsc_bytes(nnz: int, nzc: int, vbytes: int = 8) -> int:
+    """Algorithmic bytes of a DCSC with int64 indices: cp+jc (16 per column) + 8 + ir+val per entry."""
+    return 16 * nzc + 8 + (8 + vbytes) * nnz
 
 
 def cpu_baseline(scale: int, ef: int, budget_s: float) -> dict | None:
@@ -79,7 +89,7 @@ print(json.dumps(dict(kind=kind, threads=threads, secs=secs, flops=flops, slices
         line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
         d = json.loads(line)
     except Exception as e:  # noqa: BLE001 - a failed baseline must not hide the GPU number
-        print(f"cpu baseline failed: {e}", file=sys.stderr)
+        log(f"cpu baseline failed: {e}")
         return None
     gflops = 2.0 * d["flops"] / d["secs"] / 1e9 if d["secs"] > 0 else None
     frac = d["flops"] / d["total"]
@@ -91,6 +101,132 @@ print(json.dumps(dict(kind=kind, threads=threads, secs=secs, flops=flops, slices
     }
 
 
+def run_spgemm(cb, A, dA, mode: str, steps: int, warmup: int) -> dict:
+    from combblas_amd.checks import product_matches
+    for _ in range(warmup):
+        C, st = cb.spgemm(dA, dA, "plus_times", mode)
+        C.release()
+    cb.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        C, st = cb.spgemm(dA, dA, "plus_times", mode)
+        stats.append(st)
+        if k + 1 < steps:
+            C.release()
+    cb.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    exact = product_matches(C, A, A)  # full size, outside the timed region
+    C.release()
+    return {"ms": ms, "stats": stats, "exact": exact}
+
+
+def bench_bfs(cb, scale: int, ef: int, nroots: int = 64) -> dict:
+    G = cb.gen_rmat(scale, ef, symmetrize=True, no_self_loops=True, pattern=True).astype("bool")
+    deg = np.zeros(G.n, np.int64)
+    deg[G.jc] = np.diff(G.cp)
+    plan = cb.BfsPlan(G.to_device())
+    roots = cb.bfs_roots(G, nroots)
+    plan.run(int(roots[0]), copy=False)
+    ms, teps, levels = [], [], []
+    for r in roots:
+        par, st = plan.run(int(r), copy=True)
+        edges = int(deg[par >= 0].sum()) // 2  # undirected edges in the reached component
+        ms.append(st["ms"])
+        levels.append(st["levels"])
+        teps.append(edges / (st["ms"] * 1e-3))
+    plan.release()
+    hm = len(teps) / sum(1.0 / t for t in teps)
+    return {"workload": f"top-down BFS, symmetrised R-MAT s{scale} ef{ef} without self-loops, SelectMax SpMSpV, "
+                        f"{len(roots)} roots (seed 4242), one captured graph per level",
+            "gteps_harmonic_mean": round(hm / 1e9, 3), "ms_per_bfs_mean": round(float(np.mean(ms)), 3),
+            "levels_mean": round(float(np.mean(levels)), 2)}
+
+
+def bench_mcl(cb, n: int = 1_000_000, reps: int = 2) -> dict:
+    A = cb.gen_powerlaw_stochastic(n)
+    dA = A.to_device()
+    params = (1e-4, 1000, 0, 0.9)
+    P, st = cb.mcl_expand(dA, dA, *params)
+    P.release()
+    runs = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        P, st = cb.mcl_expand(dA, dA, *params)
+        cb.synchronize()
+        runs.append(((time.perf_counter() - t) * 1e3, st))
+        P.release()
+    ms = float(np.mean([r[0] for r in runs]))
+    st = runs[-1][1]
+    # FP32 rounding on a sample of 64 columns against the same product in FP64
+    rng = np.random.default_rng(7)
+    qs = np.sort(rng.choice(A.nzc, 64, replace=False))
+    idx = np.concatenate([np.arange(A.cp[q], A.cp[q + 1]) for q in qs])
+    Bs = cb.Dcsc.from_coo(A.m, A.n, A.ir[idx], np.repeat(A.jc[qs], np.diff(A.cp)[qs]), A.val[idx], "float32")
+    C32, s32 = cb.spgemm(dA, Bs.to_device(), "plus_times")
+    A64 = A.astype("float64")
+    C64, _ = cb.spgemm(A64.to_device(), Bs.astype("float64").to_device(), "plus_times")
+    h32, h64 = C32.to_host(), C64.to_host()
+    same = np.array_equal(h32.cp, h64.cp) and np.array_equal(h32.ir, h64.ir)
+    rel = np.abs(h32.val.astype(np.float64) - h64.val) / np.maximum(np.abs(h64.val), 1e-300)
+    for X in (C32, C64, dA):
+        X.release()
+    return {"workload": f"MCL expansion A*A + MCLPruneRecoverySelect(prune 1e-4, select 1000), FP32, "
+                        f"column-stochastic power-law n={n} (alpha 2.1, mean degree 50, configuration model, "
+                        f"log-normal weights), column phases sized to free memory",
+            "ms_per_expansion": round(ms, 2), "gflops": round(2.0 * st["flops"] / (ms * 1e-3) / 1e9, 2),
+            "multiply_adds": st["flops"], "nnz_a": A.nnz, "nnz_product": st["nnz_product"], "nnz_kept": st["nnz"],
+            "phases": st["phases"], "selected_cols": st["selected_cols"],
+            "ms_product": round(st["ms_product"], 2), "ms_prune": round(st["ms_prune"], 2),
+            "fp32_vs_fp64_sample": {"columns": 64, "same_pattern": bool(same),
+                                    "max_rel_err": float(rel.max()) if rel.size else 0.0,
+                                    "mean_rel_err": float(rel.mean()) if rel.size else 0.0}}
+
+
+def bench_galerkin(cb, count: int = 256, lo: int = 16, hi: int = 96, reps: int = 2) -> dict:
+    from combblas_amd.checks import stencil27_coarse_rowsums
+    rng = np.random.default_rng(0xC5)
+    grids = [int(g) for g in rng.integers(lo, hi + 1, count)]
+    As, Ts = [], []
+    nnz_a = 0
+    for g in grids:
+        A = cb.gen_stencil27(g)
+        nnz_a += A.nnz
+        As.append(A.to_device())
+        Ts.append(cb.gen_aggregation(g, True).to_device())
+        del A
+    S, st = cb.galerkin_batched(As, Ts)
+    for X in S:
+        X.release()
+    runs = []
+    for i in range(reps):
+        t = time.perf_counter()
+        S, st = cb.galerkin_batched(As, Ts)
+        cb.synchronize()
+        runs.append(((time.perf_counter() - t) * 1e3, st))
+        if i + 1 < reps:
+            for X in S:
+                X.release()
+    ms = float(np.mean([r[0] for r in runs]))
+    st = runs[-1][1]
+    rowsums_ok = True
+    for g, X in zip(grids, S):  # closed-form row sums and symmetry of every coarse operator
+        h = X.to_host()
+        rows = np.bincount(h.ir, weights=h.val, minlength=h.m)
+        cols = h.col_ids()
+        key = np.sort(h.ir * h.n + cols)
+        keyt = np.sort(cols * h.m + h.ir)
+        rowsums_ok &= bool(np.array_equal(rows, stencil27_coarse_rowsums(g)) and np.array_equal(key, keyt))
+        X.release()
+    for X in As + Ts:
+        X.release()
+    return {"workload": f"{count} batched Galerkin products S*(A*T), T = R^T 2x2x2 aggregation, A 27-point "
+                        f"Poisson stencil, grid edge uniform in [{lo},{hi}] (seed 0xC5), FP64, one packed batch",
+            "ms_per_batch": round(ms, 2), "gflops": round(2.0 * st["flops"] / (ms * 1e-3) / 1e9, 2),
+            "multiply_adds": st["flops"], "nnz_a_total": nnz_a, "nnz_out_total": st["nnz"],
+            "rowsums_and_symmetric_pattern_all": rowsums_ok}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,8 +234,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edgefactor", type=int, default=16)
-    ap.add_argument("--mode", default="two_phase", choices=["two_phase", "learned"])
+    ap.add_argument("--mode", default="both", choices=["both", "two_phase", "learned"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the BFS, MCL and Galerkin workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's block of the local product; --gpus {args.gpus} is not supported "
@@ -107,73 +244,94 @@ def main() -> int:
         return 2
 
     import combblas_amd as cb
-    import oracle
+    from combblas_amd import predictor
 
-    A = cb.gen_rmat(args.scale, args.edgefactor)
-    dA = cb.Dcsc.to_device(A)
     if cb.device_count() < 1:
         raise SystemExit("no GPU visible")
+    A = cb.gen_rmat(args.scale, args.edgefactor)
+    dA = cb.Dcsc.to_device(A)
 
-    for _ in range(args.warmup):
-        C, st = cb.spgemm(dA, dA, "plus_times", args.mode)
-        C.release()
-    cb.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        C, st = cb.spgemm(dA, dA, "plus_times", args.mode)
-        stats.append(st)
-        if k + 1 < args.steps:
-            C.release()
-    cb.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-
-    # full-size bit-exact check of the last output: structure + per-column checksums
-    C.check()
-    got = C.column_checksums(salt=0x5EED)
-    _, jcC = C.structure()
-    jcB, want = oracle.product_checksums(A, A, salt=0x5EED)
-    pos = np.searchsorted(jcB, jcC)
-    dropped = np.ones(jcB.size, bool)
-    dropped[pos] = False
-    exact = bool(np.array_equal(jcB[pos], jcC) and np.array_equal(got, want[pos]) and not np.any(want[dropped]))
-    nnz, flops, nzc = stats[-1]["nnz"], stats[-1]["flops"], stats[-1]["nzc"]
-    C.release()
-
+    modes = ["two_phase", "learned"] if args.mode == "both" else [args.mode]
+    if "learned" in modes:
+        if predictor.MODEL_PATH.exists():
+            cb.set_predictor(predictor.load())
+        else:
+            log("no trained predictor: learned mode skipped")
+            modes.remove("learned")
+    res = {}
+    for mode in modes:
+        log(f"spgemm {mode}: {args.warmup} warmup + {args.steps} steps")
+        res[mode] = run_spgemm(cb, A, dA, mode, args.steps, args.warmup)
+        log(f"spgemm {mode}: {res[mode]['ms']:.2f} ms/step, exact={res[mode]['exact']}")
+    primary = "learned" if "learned" in res else modes[0]
+    r = res[primary]
+    st_last = r["stats"][-1]
+    nnz, flops, nzc = st_last["nnz"], st_last["flops"], st_last["nzc"]
+    ms = r["ms"]
     gflops = 2.0 * flops / (ms * 1e-3) / 1e9
     bytes_step = dcsc_bytes(A.nnz, A.nzc) * 2 + dcsc_bytes(nnz, nzc)
-    if args.mode == "two_phase":  # symbolic pass re-reads the structures and writes counts
+    if primary == "two_phase":  # the symbolic pass re-reads the structures and writes counts
         bytes_step += 16 * A.nzc + 8 * A.nnz + 16 * A.nzc + 8 * A.nnz + 8 * A.nzc
     gbps = bytes_step / (ms * 1e-3) / 1e9
-
-    main_ms = float(np.mean([s["ms_main_kernel"] for s in stats]))
-    main_nnz = stats[-1]["nnz_main"]
+    main_ms = float(np.mean([s["ms_main_kernel"] for s in r["stats"]]))
+    main_nnz = st_last["nnz_main"]
     main_bytes = 16 * main_nnz + dcsc_bytes(A.nnz, A.nzc) * 2
     achieved = main_bytes / (main_ms * 1e-3) / 1e9 if main_ms > 0 else None
+    kernel = "k_num_learned" if primary == "learned" else "k_num_units (light+heavy launches)"
+
+    per_mode = {}
+    for mode, rr in res.items():
+        s = rr["stats"][-1]
+        per_mode[mode] = {
+            "ms_per_step": round(rr["ms"], 3), "gflops": round(2.0 * s["flops"] / (rr["ms"] * 1e-3) / 1e9, 3),
+            "output_bit_exact": rr["exact"],
+            "phase_ms": {k: round(float(np.mean([x[k] for x in rr["stats"]])), 3)
+                         for k in ("ms_setup", "ms_symbolic", "ms_alloc", "ms_numeric", "ms_main_kernel",
+                                   "ms_total")}}
+    speedup = None
+    mispredict = None
+    if "learned" in res and "two_phase" in res:
+        speedup = round(res["two_phase"]["ms"] / res["learned"]["ms"], 4)
+        ls = res["learned"]["stats"]
+        units = ls[-1]["units_light"]
+        mispredict = {
+            "units": units, "overflow_units": ls[-1]["overflow_cols"],
+            "overflow_pct": round(100.0 * ls[-1]["overflow_cols"] / max(units, 1), 3),
+            "fallback_steps": int(sum(x["learned_fallback"] for x in ls)),
+            "predicted_vs_actual_unit_tier_nnz": round(ls[-1]["predicted_nnz"] / max(ls[-1]["nnz_main"], 1), 4)}
 
     out = {
         "metric": METRIC, "value": round(gflops, 3), "unit": "GFLOPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int64", "data": "synthetic",
         "config": {"workload": f"R-MAT s{args.scale} ef{args.edgefactor} A*A, PlusTimes<int64>, DCSC int64 in/out",
-                   "mode": args.mode, "nnz_a": A.nnz, "nnz_c": nnz, "multiply_adds": flops},
+                   "mode": primary, "nnz_a": A.nnz, "nnz_c": nnz, "multiply_adds": flops},
         "hbm_gbps_alg": round(gbps, 1), "hbm_roofline_frac_step": round(gbps / HBM_PEAK_GBS, 4),
         "bytes_alg_step": bytes_step,
-        "learned_vs_two_phase_speedup": None,
-        "phase_ms": {k: round(float(np.mean([s[k] for s in stats])), 3)
-                     for k in ("ms_setup", "ms_symbolic", "ms_alloc", "ms_numeric", "ms_main_kernel", "ms_total")},
-        "output_bit_exact": exact,
+        "learned_vs_two_phase_speedup": speedup, "learned_mispredict": mispredict, "modes": per_mode,
+        "output_bit_exact": all(rr["exact"] for rr in res.values()),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "k_num_units (light+heavy launches)",
-                     "alg_bytes_per_step": main_bytes, "kernel_ms": round(main_ms, 3)},
+                     "traffic": None, "kernel": kernel, "alg_bytes_per_launch": main_bytes,
+                     "kernel_ms": round(main_ms, 3)},
         "cpu_baseline": None,
     }
+    dA.release()
+    cb.release_cached_memory()
+    if not args.no_extra:
+        for name, fn in (("bfs", lambda: bench_bfs(cb, args.scale, args.edgefactor)),
+                         ("mcl", lambda: bench_mcl(cb)), ("galerkin", lambda: bench_galerkin(cb))):
+            log(f"{name} ...")
+            try:
+                out[name] = fn()
+            except Exception as e:  # noqa: BLE001 - one workload failing must not hide the others
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name}: {json.dumps(out[name])}")
+            cb.release_cached_memory()
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(args.scale, args.edgefactor, args.cpu_seconds)
     print(json.dumps(out))
-    return 0 if exact else 1
+    return 0 if out["output_bit_exact"] else 1
 
 
 if __name__ == "__main__":

@@ -476,6 +476,15 @@ static cbx_status mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, double thr, i
   if (phases <= 0) {  // size phases so a phase's product (bounded by its flops) fits in 40% of free memory
     size_t freeb = 0, totb = 0;
     CBX_HIP(hipMemGetInfo(&freeb, &totb));
+    freeb += cache_idle_bytes();  // cached output blocks and the stream pool's idle reserve are reusable
+    int dev = 0;
+    hipMemPool_t pool;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t reserved = 0, used = 0;
+      if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) == hipSuccess &&
+          hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess && reserved > used)
+        freeb += (size_t)(reserved - used);
+    }
     const double per_flop = (double)(sizeof(int64_t) + dtype_size(A->dtype));
     phases = (int64_t)std::ceil((double)total * per_flop / (0.4 * (double)freeb));
   }

@@ -122,6 +122,12 @@ void dev_free(void* p) {
   c.live.erase(it);
 }
 
+size_t cache_idle_bytes() {
+  Cache& c = cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  return c.idle_bytes;
+}
+
 void release_cache() {
   Cache& c = cache();
   std::lock_guard<std::mutex> g(c.mu);

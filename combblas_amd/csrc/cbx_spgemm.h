@@ -31,6 +31,7 @@ const float* predictor_params();
 cbx_status dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
 void release_cache();
+size_t cache_idle_bytes();  // bytes held by the cache and free for reuse
 
 // Shared front-door helpers (cbx_runtime.hip).
 size_t dtype_size(cbx_dtype t);

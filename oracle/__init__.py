@@ -238,12 +238,7 @@ def ref_mcl_prune(A: Dcsc, thr: float = 1e-4, select: int = 1000, recover: int =
 
 
 # ------------------------------------------------------------------ checks
-def _mix(x: np.ndarray) -> np.ndarray:
-    """splitmix64 finaliser on uint64 arrays (wrapping arithmetic)."""
-    x = x + np.uint64(0x9E3779B97F4A7C15)
-    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-    return x ^ (x >> np.uint64(31))
+from combblas_amd.checks import mix64 as _mix, product_checksums  # noqa: E402,F401  (closed forms)
 
 
 def column_digests(X: Dcsc) -> np.ndarray:
@@ -264,23 +259,6 @@ def column_digests(X: Dcsc) -> np.ndarray:
         pos = np.arange(X.nnz, dtype=np.int64) - np.repeat(X.cp[:-1], np.diff(X.cp))
         h = h * (pos.astype(np.uint64) + np.uint64(1))
         return np.add.reduceat(h, X.cp[:-1]) if X.nzc else np.zeros(0, np.uint64)
-
-
-def product_checksums(A: Dcsc, B: Dcsc, salt: int = 0) -> tuple[np.ndarray, np.ndarray]:
-    """Closed form of the per-column checksums of C = A*B (plus-times,
-    integer values, wrapping uint64): for B's stored column j,
-    sum_i u(i) C(i,j) = sum_k B(k,j) * w_k with w_k = sum_i u(i) A(i,k),
-    u(i) = mix64(i ^ salt).  Returns (B.jc, checksum per B stored column);
-    columns whose product is empty get 0."""
-    with np.errstate(over="ignore"):
-        u = _mix(A.ir.astype(np.uint64) ^ np.uint64(salt))
-        contrib = u * A.val.astype(np.int64).view(np.uint64)
-        w_col = np.add.reduceat(contrib, A.cp[:-1]) if A.nzc else np.zeros(0, np.uint64)
-        w = np.zeros(A.n, np.uint64)
-        w[A.jc] = w_col
-        t = w[B.ir] * B.val.astype(np.int64).view(np.uint64)
-        cs = np.add.reduceat(t, B.cp[:-1]) if B.nzc else np.zeros(0, np.uint64)
-    return B.jc.copy(), cs
 
 
 _XDT = {np.dtype(np.int64): 0, np.dtype(np.float64): 1, np.dtype(np.float32): 2}
