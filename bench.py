@@ -237,6 +237,9 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["both", "two_phase", "learned"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
     ap.add_argument("--no-extra", action="store_true", help="skip the BFS, MCL and Galerkin workloads")
+    ap.add_argument("--copy-gib", type=float, default=4.0,
+                    help="size of a device-to-device copy timed before the workload: the achievable HBM rate, and "
+                         "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters (0 disables)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's block of the local product; --gpus {args.gpus} is not supported "
@@ -248,6 +251,19 @@ def main() -> int:
 
     if cb.device_count() < 1:
         raise SystemExit("no GPU visible")
+    copy = None
+    if args.copy_gib > 0:
+        nb = int(args.copy_gib * 2 ** 30)
+        src, dst = cb.DeviceArray(nb, np.uint8), cb.DeviceArray(nb, np.uint8)
+        cb.lib().cbx_memcpy(dst.ptr, src.ptr, nb, 2)
+        cb.synchronize()
+        t = time.perf_counter()
+        cb.lib().cbx_memcpy(dst.ptr, src.ptr, nb, 2)
+        cb.synchronize()
+        dt = time.perf_counter() - t
+        copy = {"bytes": nb, "gbps_read_plus_write": round(2 * nb / dt / 1e9, 1)}
+        src.release()
+        dst.release()
     A = cb.gen_rmat(args.scale, args.edgefactor)
     dA = cb.Dcsc.to_device(A)
 
@@ -315,6 +331,7 @@ def main() -> int:
                      "traffic": None, "kernel": kernel, "alg_bytes_per_launch": main_bytes,
                      "kernel_ms": round(main_ms, 3)},
         "cpu_baseline": None,
+        "device_copy": copy,
     }
     dA.release()
     cb.release_cached_memory()
