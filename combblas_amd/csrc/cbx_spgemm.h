@@ -19,7 +19,7 @@ struct SpgemmConfig {
   // unit size target as a fraction of the value window, and the headroom of
   // the output capacity over the predicted total.
   const float* predictor = nullptr;
-  float learned_target = 0.5f;
+  float learned_target = 0.75f;
   float capacity_margin = 0.15f;
 };
 

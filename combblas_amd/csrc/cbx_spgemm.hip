@@ -1110,30 +1110,42 @@ __global__ __launch_bounds__(NT) void k_num_learned(
     }
     cnt = (int)total;
   }
-  // publish the size, then look back for the offset (bounded wait)
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&status[t], kAggFlag | (unsigned long long)cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  // publish the size, then look back for the offset with the whole first
+  // wave: 64 predecessors per step, summing back to the nearest one that
+  // holds an inclusive prefix (bounded wait on predecessors still counting)
+  if (threadIdx.x < kWave) {
+    const int lane = (int)threadIdx.x;
+    if (lane == 0)
+      __hip_atomic_store(&status[t], kAggFlag | (unsigned long long)cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     long long excl = 0;
-    long long spins = 0;
+    int spins = 0;
     for (int i = t - 1; i >= 0;) {
-      const unsigned long long v = __hip_atomic_load(&status[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      if ((v >> 62) == 0) {
-        if (++spins > (1ll << 22)) {  // a predecessor never published: give up, flag it
-          atomicOr(err, 256);
+      const int j = i - lane;
+      const unsigned long long v =
+          j >= 0 ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kInclFlag;
+      const unsigned long long incl = __ballot((v >> 62) == 2);
+      const unsigned long long none = __ballot((v >> 62) == 0);
+      const int first = incl ? __ffsll((unsigned long long)incl) - 1 : kWave;  // nearest inclusive prefix
+      const unsigned long long need = first >= kWave - 1 ? ~0ull : ((2ull << first) - 1ull);
+      if (none & need) {  // a predecessor in range has not published yet
+        if (++spins > (1 << 22)) {
+          if (lane == 0) atomicOr(err, 256);
           break;
         }
         __builtin_amdgcn_s_sleep(2);
         continue;
       }
-      excl += (long long)(v & kValMask);
-      if ((v >> 62) == 2) break;
-      --i;
+      excl += wave_sum_ll(lane <= first ? (long long)(v & kValMask) : 0ll);
+      if (first < kWave) break;
+      i -= kWave;
     }
-    __hip_atomic_store(&status[t], kInclFlag | (unsigned long long)(excl + cnt), __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    s_excl = excl;
-    if (cnt) atomicAdd(&col_total[col], (unsigned long long)cnt);
-    if (cnt > KMAX) atomicAdd(ovf_units, 1);
+    if (lane == 0) {
+      __hip_atomic_store(&status[t], kInclFlag | (unsigned long long)(excl + cnt), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      s_excl = excl;
+      if (cnt) atomicAdd(&col_total[col], (unsigned long long)cnt);
+      if (cnt > KMAX) atomicAdd(ovf_units, 1);
+    }
   }
   __syncthreads();
   const int64_t out = s_excl + hcs[col];
