@@ -83,6 +83,11 @@ class cbx_mcl_stats(C.Structure):
                [(k, C.c_double) for k in ("ms_product", "ms_prune", "ms_total")]
 
 
+class cbx_spgemm_config(C.Structure):
+    _fields_ = [("hash_max_flop", C.c_int64), ("hash_mid_flop", C.c_int64), ("hash_small_flop", C.c_int64),
+                ("heavy_nnz", C.c_int64), ("learned_target", C.c_float), ("capacity_margin", C.c_float)]
+
+
 _lib_handle = None
 
 
@@ -107,6 +112,10 @@ def lib() -> C.CDLL:
         L.cbx_gen_powerlaw_stochastic.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_int64, C.c_uint64,
                                                   P(cbx_dcsc)]
         L.cbx_gen_powerlaw_stochastic.restype = C.c_int32
+        L.cbx_spgemm_get_config.argtypes = [P(cbx_spgemm_config)]
+        L.cbx_spgemm_get_config.restype = C.c_int32
+        L.cbx_spgemm_set_config.argtypes = [P(cbx_spgemm_config)]
+        L.cbx_spgemm_set_config.restype = C.c_int32
         L.cbx_gen_stencil27.argtypes = [C.c_int32, P(cbx_dcsc)]
         L.cbx_gen_stencil27.restype = C.c_int32
         L.cbx_gen_aggregation.argtypes = [C.c_int32, C.c_int32, P(cbx_dcsc)]
@@ -373,6 +382,24 @@ def spgemm(A: DeviceDcsc, B: DeviceDcsc, semiring: str = "plus_times", mode: str
                             C.byref(st), stream))
     stats = {k: getattr(st, k) for k, _ in cbx_spgemm_stats._fields_}
     return DeviceDcsc(out), stats
+
+
+def get_config() -> dict:
+    """The process-wide SpGEMM tunables (bin thresholds, hash-table classes, unit sizes)."""
+    c = cbx_spgemm_config()
+    _check(lib().cbx_spgemm_get_config(C.byref(c)))
+    return {k: getattr(c, k) for k, _ in cbx_spgemm_config._fields_}
+
+
+def set_config(**kw) -> dict:
+    """Updates some tunables (validated by the library); returns the full new configuration."""
+    cur = get_config()
+    unknown = set(kw) - set(cur)
+    if unknown:
+        raise ValueError(f"unknown tunables: {sorted(unknown)}")
+    cur.update(kw)
+    _check(lib().cbx_spgemm_set_config(C.byref(cbx_spgemm_config(**cur))))
+    return cur
 
 
 def gen_stencil27(g: int) -> Dcsc:

@@ -473,8 +473,7 @@ extern "C" cbx_status cbx_spgemm_batched(int64_t count, const cbx_dcsc* A, const
   CBX_RETURN_IF(mode != CBX_SPGEMM_TWO_PHASE && mode != CBX_SPGEMM_UNSORTED && mode != CBX_SPGEMM_LEARNED,
                 CBX_ERR_INVALID, "cbx_spgemm_batched: unknown mode %d", (int)mode);
   if ((st = init_device_once())) return st;
-  SpgemmConfig cfg;
-  cfg.predictor = predictor_params();
+  SpgemmConfig cfg = current_config();
   CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor, CBX_ERR_INVALID,
                 "cbx_spgemm_batched: learned mode needs a predictor (cbx_spgemm_set_predictor)");
   cbx_spgemm_stats local;
@@ -499,7 +498,7 @@ extern "C" cbx_status cbx_galerkin_batched(int64_t count, const cbx_dcsc* A, con
     CBX_RETURN_IF(A[i].m != A[i].n || A[i].n != T[i].m, CBX_ERR_INVALID,
                   "cbx_galerkin_batched: product %lld: A must be square with as many rows as T", (long long)i);
   if ((st = init_device_once())) return st;
-  SpgemmConfig cfg;
+  SpgemmConfig cfg = current_config();
   cbx_spgemm_stats local;
   cbx_spgemm_stats* sp = stats ? stats : &local;
   std::memset(sp, 0, sizeof(*sp));

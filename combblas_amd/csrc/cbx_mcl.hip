@@ -576,8 +576,7 @@ extern "C" cbx_status cbx_mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, doubl
   CBX_RETURN_IF(mode != CBX_SPGEMM_TWO_PHASE && mode != CBX_SPGEMM_LEARNED, CBX_ERR_INVALID,
                 "cbx_mcl_expand: mode must be two-phase or learned");
   if ((st = init_device_once())) return st;
-  SpgemmConfig cfg;
-  cfg.predictor = predictor_params();
+  SpgemmConfig cfg = current_config();
   CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor, CBX_ERR_INVALID,
                 "cbx_mcl_expand: learned mode needs a predictor (cbx_spgemm_set_predictor)");
   cbx_mcl_stats local;

@@ -255,8 +255,7 @@ extern "C" cbx_status cbx_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semir
   CBX_RETURN_IF(mode != CBX_SPGEMM_TWO_PHASE && mode != CBX_SPGEMM_UNSORTED && mode != CBX_SPGEMM_LEARNED,
                 CBX_ERR_INVALID, "cbx_spgemm: unknown mode %d", (int)mode);
   if ((st = init_device_once())) return st;
-  SpgemmConfig cfg;
-  cfg.predictor = predictor_params();
+  SpgemmConfig cfg = current_config();
   CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor, CBX_ERR_INVALID,
                 "cbx_spgemm: learned mode needs a predictor (cbx_spgemm_set_predictor)");
   return spgemm_dispatch(A, B, semiring, mode, cfg, C, stats, (hipStream_t)stream);
@@ -305,5 +304,49 @@ extern "C" cbx_status cbx_spgemm_set_predictor(const float* params) {
   if (st) return st;
   if (!g_predictor) CBX_HIP(hipMalloc(&g_predictor, sizeof(float) * CBX_MLP_PARAMS));
   CBX_HIP(hipMemcpy(g_predictor, params, sizeof(float) * CBX_MLP_PARAMS, hipMemcpyHostToDevice));
+  return CBX_OK;
+}
+
+// ------------------------------------------------------------- tunables
+static std::mutex g_config_mu;
+static cbx_spgemm_config g_config = {4096, 1024, 128, 65536, 0.75f, 0.15f};
+
+SpgemmConfig cbx::current_config() {
+  SpgemmConfig c;
+  {
+    std::lock_guard<std::mutex> g(g_config_mu);
+    c.small_flop_max = g_config.hash_max_flop;
+    c.hash_mid_flop = g_config.hash_mid_flop;
+    c.hash_small_flop = g_config.hash_small_flop;
+    c.heavy_nnz = g_config.heavy_nnz;
+    c.learned_target = g_config.learned_target;
+    c.capacity_margin = g_config.capacity_margin;
+  }
+  c.predictor = predictor_params();
+  return c;
+}
+
+extern "C" cbx_status cbx_spgemm_get_config(cbx_spgemm_config* cfg) {
+  CBX_RETURN_IF(!cfg, CBX_ERR_INVALID, "cbx_spgemm_get_config: null argument");
+  std::lock_guard<std::mutex> g(g_config_mu);
+  *cfg = g_config;
+  return CBX_OK;
+}
+
+extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
+  CBX_RETURN_IF(!cfg, CBX_ERR_INVALID, "cbx_spgemm_set_config: null argument");
+  CBX_RETURN_IF(cfg->hash_small_flop < 0 || cfg->hash_small_flop > 128, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: hash_small_flop must be in [0, 128]");
+  CBX_RETURN_IF(cfg->hash_mid_flop < cfg->hash_small_flop || cfg->hash_mid_flop > 1024, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: hash_mid_flop must be in [hash_small_flop, 1024]");
+  CBX_RETURN_IF(cfg->hash_max_flop < cfg->hash_mid_flop || cfg->hash_max_flop > 4096, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: hash_max_flop must be in [hash_mid_flop, 4096]");
+  CBX_RETURN_IF(cfg->heavy_nnz < 4096, CBX_ERR_INVALID, "cbx_spgemm_set_config: heavy_nnz must be >= 4096");
+  CBX_RETURN_IF(!(cfg->learned_target > 0.05f && cfg->learned_target <= 1.0f), CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: learned_target must be in (0.05, 1]");
+  CBX_RETURN_IF(!(cfg->capacity_margin >= 0.0f && cfg->capacity_margin <= 4.0f), CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: capacity_margin must be in [0, 4]");
+  std::lock_guard<std::mutex> g(g_config_mu);
+  g_config = *cfg;
   return CBX_OK;
 }

@@ -13,6 +13,11 @@ struct SpgemmConfig {
   // row-tiled dense tier.  At most 4096 (the largest hash table holds 8192
   // slots and is kept at most half full).
   int64_t small_flop_max = 4096;
+  // Hash tier table sizes: flop_j in (hash_mid_flop, small_flop_max] use the
+  // 8192-slot table, (hash_small_flop, hash_mid_flop] the 2048-slot table,
+  // the rest the 256-slot table (each table kept at most half full).
+  int64_t hash_mid_flop = 1024;
+  int64_t hash_small_flop = 128;
   // Unit tier: columns whose output exceeds heavy_nnz use the large units.
   int64_t heavy_nnz = 65536;
   // Learned mode: device parameters of the nnz predictor (null = not loaded),
@@ -22,6 +27,10 @@ struct SpgemmConfig {
   float learned_target = 0.75f;
   float capacity_margin = 0.15f;
 };
+
+// The process-wide tunables (cbx_spgemm_set_config) with the installed
+// predictor: the starting configuration of every product.
+SpgemmConfig current_config();
 
 // Launches the predictor's inference kernel (cbx_mlp.hip).
 void mlp_infer_launch(const float* params, const float* X, int64_t n, float* out, hipStream_t s);

@@ -1512,7 +1512,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_HIP(hipcub::DeviceReduce::Sum(tmp, tmp_bytes, flop, bounds + 5, (int)nzcB, s));
 
   // class thresholds: dense > small_max ; hash8k > 1024 ; hash2k > 128 ; hash256 > 0 ; heavy > heavy_nnz
-  int64_t hthr[5] = {cfg.small_flop_max, 1024, 128, 0, cfg.heavy_nnz};
+  int64_t hthr[5] = {cfg.small_flop_max, cfg.hash_mid_flop, cfg.hash_small_flop, 0, cfg.heavy_nnz};
   CBX_HIP(hipMemcpyAsync(thr, hthr, sizeof(hthr), hipMemcpyHostToDevice, s));
   k_class_bounds<<<1, 64, 0, s>>>(flop_s, nzcB, thr, 5, bounds);
   int64_t hb[6];
@@ -1720,7 +1720,7 @@ cbx_status spgemm_flops(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop_host
 
 cbx_status spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, hipStream_t s) {
   cbx_dcsc C;
-  SpgemmConfig cfg;
+  SpgemmConfig cfg = current_config();
   return spgemm_typed<kPlusTimes, int64_t>(A, B, CBX_SPGEMM_TWO_PHASE, cfg, &C, nullptr, s, nnz);
 }
 

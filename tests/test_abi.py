@@ -56,3 +56,19 @@ def test_generator_rmat_shape():
     key = cols * A.m + A.ir
     assert np.all(np.diff(key) > 0), "rows strictly increasing within columns"
     assert A.val.min() >= 1
+
+
+def test_tunables_validated_without_gpu():
+    base = cb.get_config()
+    try:
+        got = cb.set_config(hash_max_flop=2048, heavy_nnz=131072)
+        assert got["hash_max_flop"] == 2048 and cb.get_config()["heavy_nnz"] == 131072
+        for bad in ({"hash_small_flop": 500}, {"hash_mid_flop": 2000}, {"hash_max_flop": 8192},
+                    {"heavy_nnz": 10}, {"learned_target": 0.0}, {"capacity_margin": -1.0}):
+            with pytest.raises(cb.CbxError):
+                cb.set_config(**bad)
+        with pytest.raises(ValueError):
+            cb.set_config(no_such_knob=1)
+    finally:
+        cb.set_config(**base)
+    assert cb.get_config() == base
