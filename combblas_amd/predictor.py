@@ -104,6 +104,43 @@ def train(steps: int = 4000, batch: int = 4096, lr: float = 2e-3, seed: int = 7,
     return meta
 
 
+def ratio_params(ratio: float) -> np.ndarray:
+    """A non-learned predictor in the network's parameter format:
+    log2(1 + nnz_j) = log2(1 + flop_j) + log2(ratio), i.e. every column's size
+    is its flop count times one global compression ratio."""
+    F, H = cb.MLP_FEATURES, cb.MLP_HIDDEN
+    P = np.zeros(cb.MLP_PARAMS, np.float32)
+    oW1, oB1 = 0, H * F
+    oW2 = oB1 + H
+    oB2 = oW2 + H * H
+    oW3 = oB2 + H
+    oB3 = oW3 + H
+    oMU = oB3 + 1
+    oSD = oMU + F
+    P[oW1 + 0 * F + 1] = 1.0  # hidden unit 0 = log2(1 + flop) (>= 0, passes the ReLU)
+    P[oW2 + 0 * H + 0] = 1.0
+    P[oW3 + 0] = 1.0
+    P[oB3] = np.log2(ratio)
+    P[oSD:oSD + F] = 1.0
+    return P
+
+
+def sampling_ratio(A: cb.Dcsc, dA, frac: float = 0.01, seed: int = 3) -> float:
+    """Global compression nnz(C)/flop(C) of C = A*A estimated from the exact
+    sizes of a random `frac` of the columns (the idea of the reference's
+    estimateNNZ_sampling, mtSpGEMM.h:936-1054: a global estimate from a sample)."""
+    rng = np.random.default_rng(seed)
+    k = max(1, int(frac * A.nzc))
+    qs = np.sort(rng.choice(A.nzc, k, replace=False))
+    idx = np.concatenate([np.arange(A.cp[q], A.cp[q + 1]) for q in qs])
+    Bs = cb.Dcsc.from_coo(A.m, A.n, A.ir[idx], np.repeat(A.jc[qs], np.diff(A.cp)[qs]), A.val[idx], A.dtype)
+    lens = np.zeros(A.n, np.int64)
+    lens[A.jc] = np.diff(A.cp)
+    flop = int(lens[Bs.ir].sum())
+    nnz = int(cb.spgemm_symbolic(dA, Bs.to_device()).sum())
+    return nnz / max(flop, 1)
+
+
 def load() -> np.ndarray:
     if not MODEL_PATH.exists():
         raise FileNotFoundError(f"{MODEL_PATH} missing: train it with `python -m combblas_amd.predictor train`")

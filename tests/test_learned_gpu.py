@@ -113,3 +113,22 @@ def test_learned_trained_model_s18_checksums():
     jcB, want = oracle.product_checksums(A, A, salt=99)
     pos = np.searchsorted(jcB, jcC)
     assert np.array_equal(jcB[pos], jcC) and np.array_equal(got, want[pos])
+
+
+@pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
+def test_trained_network_beats_sampling_baseline_s18():
+    """What the trained network gains over the method without it: the same
+    single pass driven by a non-learned estimate (flop_j times a global
+    compression ratio sampled from 1% of the columns) sends far more units
+    through the exact overflow re-run."""
+    A = cb.gen_rmat(18, 16)
+    dA = A.to_device()
+    cb.set_predictor(predictor.ratio_params(predictor.sampling_ratio(A, dA)))
+    C0, base = cb.spgemm(dA, dA, "plus_times", "learned")
+    C0.release()
+    cb.set_predictor(predictor.load())
+    C1, mlp = cb.spgemm(dA, dA, "plus_times", "learned")
+    C1.release()
+    assert mlp["learned_fallback"] == 0
+    ovf = lambda st: st["overflow_cols"] / max(st["units_light"], 1)
+    assert ovf(mlp) < 0.5 * ovf(base) or base["learned_fallback"] == 1, (mlp, base)
