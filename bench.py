@@ -279,6 +279,18 @@ def main() -> int:
         log(f"spgemm {mode}: {args.warmup} warmup + {args.steps} steps")
         res[mode] = run_spgemm(cb, A, dA, mode, args.steps, args.warmup)
         log(f"spgemm {mode}: {res[mode]['ms']:.2f} ms/step, exact={res[mode]['exact']}")
+    baseline = None
+    if "learned" in res:  # the same single pass driven by a non-learned estimate, for comparison
+        ratio = predictor.sampling_ratio(A, dA)
+        cb.set_predictor(predictor.ratio_params(ratio))
+        rb = run_spgemm(cb, A, dA, "learned", 1, 1)
+        sb = rb["stats"][-1]
+        baseline = {"predictor": f"flop_j x sampled global ratio {ratio:.4f} (1% of columns)",
+                    "ms_per_step": round(rb["ms"], 3), "output_bit_exact": rb["exact"],
+                    "overflow_units": sb["overflow_cols"], "units": sb["units_light"],
+                    "overflow_pct": round(100.0 * sb["overflow_cols"] / max(sb["units_light"], 1), 3),
+                    "fallback": int(sb["learned_fallback"])}
+        cb.set_predictor(predictor.load())
     primary = "learned" if "learned" in res else modes[0]
     r = res[primary]
     st_last = r["stats"][-1]
@@ -314,7 +326,8 @@ def main() -> int:
             "units": units, "overflow_units": ls[-1]["overflow_cols"],
             "overflow_pct": round(100.0 * ls[-1]["overflow_cols"] / max(units, 1), 3),
             "fallback_steps": int(sum(x["learned_fallback"] for x in ls)),
-            "predicted_vs_actual_unit_tier_nnz": round(ls[-1]["predicted_nnz"] / max(ls[-1]["nnz_main"], 1), 4)}
+            "predicted_vs_actual_unit_tier_nnz": round(ls[-1]["predicted_nnz"] / max(ls[-1]["nnz_main"], 1), 4),
+            "non_learned_baseline": baseline}
 
     out = {
         "metric": METRIC, "value": round(gflops, 3), "unit": "GFLOPS", "n_gpus": 1, "steps": args.steps,
