@@ -819,18 +819,24 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
     }
   }
   __syncthreads();
-  // emit: values in rank order (coalesced), rows from the bitmap
+  // emit: values in rank order (coalesced); then the rows, staged in rank
+  // order in the value array's LDS (every Acc is at least 4 bytes) so that
+  // they too leave in full, coalesced lines
   for (int i = threadIdx.x; i < cnt; i += NT) valC[out + i] = SR::out(vals[i]);
+  __syncthreads();
+  uint32_t* rstage = reinterpret_cast<uint32_t*>(vals);
   for (int w = threadIdx.x; w < nw64; w += NT) {
     uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
     int q = pre[w];
     while (word) {
       const int b = __ffsll((unsigned long long)word) - 1;
       word &= word - 1;
-      if (q < cnt) irC[out + q] = (int64_t)row0 + w * 64 + b;
+      if (q < cnt) rstage[q] = (uint32_t)(w * 64 + b);
       ++q;
     }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < cnt; i += NT) irC[out + i] = (int64_t)row0 + rstage[i];
 }
 
 // Predictor features per stored column of B (one wave per column); see
@@ -1183,17 +1189,27 @@ __global__ __launch_bounds__(NT) void k_num_learned(
       for (int i = threadIdx.x; i < wn; i += NT) valC[out + q0 + i] = SR::out(vals[i]);
     __syncthreads();
   }
-  if (can_write)
+  if (can_write) {
+    // rows in rank order; staged in the value array's LDS when they fit in
+    // one window (the common case), so that they leave in coalesced lines
+    const bool staged = cnt <= KMAX;
+    uint32_t* rstage = reinterpret_cast<uint32_t*>(vals);
     for (int w = threadIdx.x; w < nw64; w += NT) {
       uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
       int q = pre[w];
       while (word) {
         const int b = __ffsll((unsigned long long)word) - 1;
         word &= word - 1;
-        irC[out + q] = (int64_t)row0 + w * 64 + b;
+        if (staged) rstage[q] = (uint32_t)(w * 64 + b);
+        else irC[out + q] = (int64_t)row0 + w * 64 + b;
         ++q;
       }
     }
+    if (staged) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < cnt; i += NT) irC[out + i] = (int64_t)row0 + rstage[i];
+    }
+  }
 }
 
 // Tile pointers at kBR granularity, one wave per stored A column:
