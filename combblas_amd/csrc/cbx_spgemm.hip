@@ -1638,18 +1638,19 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   tm.mark(4, s);
   }  // !learned_done
   const bool sort = mode != CBX_SPGEMM_UNSORTED;
-#define CBX_LAUNCH_HASH(HM, NCOL, FIRST)                                                                          \
+#define CBX_LAUNCH_HASH(HM, HNT, NCOL, FIRST)                                                                     \
   if (NCOL) {                                                                                                     \
     if (sort)                                                                                                     \
-      k_num_hash<K, T, HM, kHashNT, true><<<(unsigned)(NCOL), kHashNT, 0, s>>>(                                   \
+      k_num_hash<K, T, HM, HNT, true><<<(unsigned)(NCOL), HNT, 0, s>>>(                                           \
           order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);           \
     else                                                                                                          \
-      k_num_hash<K, T, HM, kHashNT, false><<<(unsigned)(NCOL), kHashNT, 0, s>>>(                                  \
+      k_num_hash<K, T, HM, HNT, false><<<(unsigned)(NCOL), HNT, 0, s>>>(                                          \
           order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);           \
   }
-  CBX_LAUNCH_HASH(8192, n_h8, hb[0]);
-  CBX_LAUNCH_HASH(2048, n_h2, hb[1]);
-  CBX_LAUNCH_HASH(256, n_h256, hb[2]);
+  // the 8192-slot table takes ~100 KB of LDS, so one workgroup per CU: give it 16 waves
+  CBX_LAUNCH_HASH(8192, 1024, n_h8, hb[0]);
+  CBX_LAUNCH_HASH(2048, kHashNT, n_h2, hb[1]);
+  CBX_LAUNCH_HASH(256, kHashNT, n_h256, hb[2]);
 #undef CBX_LAUNCH_HASH
   CBX_HIP(hipGetLastError());
 
