@@ -934,23 +934,29 @@ __global__ void k_col_blocks(const int64_t* __restrict__ cpB, int64_t nzcB, cons
   }
 }
 
-// Greedy cut of a column into units from expected block counts
-// pred * hist[b] / nnzA: a unit closes when its expected size would pass
-// `target` or its span would pass rblk blocks.  Pass 1 counts (out == null),
-// pass 2 writes at `base`.
-__device__ __forceinline__ int cut_units(int col, float pred, int lo, int hi, const float* __restrict__ w,
-                                         float target, int rblk, UnitL* out) {
+// Greedy cut of a column into units from expected block counts: block b of
+// column j is expected to hold pred * w[b] outputs and flop_j * w[b]
+// products (w = A's row histogram).  A unit closes when its expected outputs
+// would pass `target`, its expected products would pass `fcap`, or its span
+// would pass rblk blocks.  Capping the products keeps every unit's counting
+// sweep short, so no unit holds its successors' look-back for long.  Pass 1
+// counts (out == null), pass 2 writes at `base`.
+__device__ __forceinline__ int cut_units(int col, float pred, float flop, int lo, int hi,
+                                         const float* __restrict__ w, float target, float fcap, int rblk,
+                                         UnitL* out) {
   int n = 0, start = lo;
-  float acc = 0.f;
+  float acc = 0.f, accf = 0.f;
   for (int b = lo; b <= hi; ++b) {
-    const float e = pred * w[b];
-    if (b > start && (acc + e > target || b - start >= rblk)) {
+    const float e = pred * w[b], ef = flop * w[b];
+    if (b > start && (acc + e > target || accf + ef > fcap || b - start >= rblk)) {
       if (out) out[n] = UnitL{col, start, b};
       ++n;
       start = b;
       acc = 0.f;
+      accf = 0.f;
     }
     acc += e;
+    accf += ef;
   }
   if (out) out[n] = UnitL{col, start, hi + 1};
   return n + 1;
@@ -958,8 +964,8 @@ __device__ __forceinline__ int cut_units(int col, float pred, int lo, int hi, co
 
 __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, int64_t small_max,
                                 const float* __restrict__ logpred, const int32_t* __restrict__ blo,
-                                const int32_t* __restrict__ bhi, const float* __restrict__ w, float target, int rblk,
-                                int64_t m, const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
+                                const int32_t* __restrict__ bhi, const float* __restrict__ w, float target, float fcap,
+                                int rblk, int64_t m, const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
                                 UnitL* __restrict__ units, float* __restrict__ pred_nnz) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= nzcB) return;
@@ -974,10 +980,10 @@ __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, 
   const float cap = (float)min<int64_t>(flop[j], m);
   pred = fminf(fmaxf(pred, 1.f), cap);
   if (!units) {
-    ucount[j] = cut_units((int)j, pred, blo[j], bhi[j], w, target, rblk, nullptr);
+    ucount[j] = cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, nullptr);
     pred_nnz[j] = pred;
   } else {
-    cut_units((int)j, pred, blo[j], bhi[j], w, target, rblk, units + ubase[j]);
+    cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, units + ubase[j]);
   }
 }
 
@@ -1318,6 +1324,7 @@ constexpr int kSymUnitNT = 1024;
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 32;    // <= 4096 outputs, <= 128K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
 constexpr int kLearnNT = 512, kLearnK = 4096, kLearnRB = 32;    // learned units: value window, span
+constexpr int kLearnFlopCap = 32768;                            // learned units: expected products
 
 }  // namespace
 
@@ -1370,8 +1377,9 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   k_col_blocks<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, blo, bhi);
   // units in column order: count, scan, write
   const float target = cfg.learned_target * (float)kLearnK;
+  const float fcap = (float)kLearnFlopCap;
   k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                  kLearnRB, m, nullptr, ucount, nullptr, pred_nnz);
+                                                  fcap, kLearnRB, m, nullptr, ucount, nullptr, pred_nnz);
   CBX_HIP(hipMemsetAsync(ucount + nzcB, 0, sizeof(int64_t), s));
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ucount, ubase, (int)(nzcB + 1), s));
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
@@ -1390,7 +1398,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_status, std::max<int64_t>(nunits, 1), s, &status));
   if (nunits)
     k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                    kLearnRB, m, ubase, nullptr, units, nullptr);
+                                                    fcap, kLearnRB, m, ubase, nullptr, units, nullptr);
   // output capacity from the predictions (plus the exact hash-tier sizes)
   const int64_t cap = hash_total + (int64_t)(pred_sum * (1.0 + cfg.capacity_margin)) + (1 << 20);
   CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * cap));
