@@ -934,20 +934,31 @@ __global__ void k_col_blocks(const int64_t* __restrict__ cpB, int64_t nzcB, cons
   }
 }
 
-// Greedy cut of a column into units from expected block counts: block b of
-// column j is expected to hold pred * w[b] outputs and flop_j * w[b]
-// products (w = A's row histogram).  A unit closes when its expected outputs
-// would pass `target`, its expected products would pass `fcap`, or its span
-// would pass rblk blocks.  Capping the products keeps every unit's counting
-// sweep short, so no unit holds its successors' look-back for long.  Pass 1
-// counts (out == null), pass 2 writes at `base`.
+// Greedy cut of a column into units from expected block counts.  Block b of
+// column j is expected to hold flop_j * w[b] products (w = A's row
+// histogram) and, its 4096 rows each being hit at rate
+// lambda = flop_j * w[b] / 4096, g_b = 4096 (1 - e^-lambda) distinct rows
+// (saturating for dense columns, proportional to w for sparse ones); the
+// predicted column size is spread over the blocks in proportion to g_b.  A
+// unit closes when its expected outputs would pass `target`, its expected
+// products would pass `fcap`, or its span would pass rblk blocks.  Capping
+// the products keeps every unit's counting sweep short, so no unit holds its
+// successors' look-back for long.  Pass 1 counts (out == null), pass 2
+// writes at `base`.
+__device__ __forceinline__ float distinct_rows(float flop, float wb) {
+  return (float)kBR * (1.f - __expf(-flop * wb * (1.f / (float)kBR)));
+}
+
 __device__ __forceinline__ int cut_units(int col, float pred, float flop, int lo, int hi,
                                          const float* __restrict__ w, float target, float fcap, int rblk,
                                          UnitL* out) {
+  float gsum = 0.f;
+  for (int b = lo; b <= hi; ++b) gsum += distinct_rows(flop, w[b]);
+  const float scale = gsum > 0.f ? pred / gsum : 0.f;
   int n = 0, start = lo;
   float acc = 0.f, accf = 0.f;
   for (int b = lo; b <= hi; ++b) {
-    const float e = pred * w[b], ef = flop * w[b];
+    const float e = scale * distinct_rows(flop, w[b]), ef = flop * w[b];
     if (b > start && (acc + e > target || accf + ef > fcap || b - start >= rblk)) {
       if (out) out[n] = UnitL{col, start, b};
       ++n;
