@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_mcl_stats(const int64_t* __restrict__ c
                                                    int64_t nzc, T thr, int64_t select, int64_t recover, double pct,
                                                    T* __restrict__ cut, int64_t* __restrict__ cnt,
                                                    int32_t* __restrict__ job, int32_t* __restrict__ jobs,
-                                                   unsigned long long* __restrict__ counters) {
+                                                   int32_t* __restrict__ npos, unsigned long long* __restrict__ counters) {
   const int64_t q = global_wave();
   if (q >= nzc) return;  // whole wave
   const int lane = lane_id();
@@ -109,6 +109,7 @@ __global__ __launch_bounds__(256) void k_mcl_stats(const int64_t* __restrict__ c
     const bool sel = !rec && select > 0 && nP > select;
     const int j = rec ? kJobRecover : (sel ? kJobSelect : kJobNone);
     job[q] = j;
+    npos[q] = (int32_t)nP;
     cut[q] = thr;
     cnt[q] = nK;
     if (j != kJobNone) {
@@ -173,29 +174,75 @@ __device__ double block_sum_f64(double v, double* wsum) {
   return t;
 }
 
+// A selected column's cut is its k-th largest entry with k <= nP, so it lies
+// among the nP entries above the hard threshold; when those fit in LDS they
+// are gathered once and every later step (selection, the recovery test,
+// the kept count) runs on them instead of re-reading the column.
+constexpr int kCandCap = 8192;
+
 template <class T, int NT>
 __global__ __launch_bounds__(NT) void k_mcl_select(const int64_t* __restrict__ cp, const T* __restrict__ val,
                                                    const int32_t* __restrict__ jobs, const int32_t* __restrict__ job,
-                                                   int64_t select, int64_t recover, double pct,
-                                                   T* __restrict__ cut, int64_t* __restrict__ cnt,
+                                                   const int32_t* __restrict__ npos, T thr, int64_t select,
+                                                   int64_t recover, double pct, T* __restrict__ cut,
+                                                   int64_t* __restrict__ cnt,
                                                    unsigned long long* __restrict__ counters) {
   __shared__ unsigned hist[256];
   __shared__ typename OrderKey<T>::U sh_prefix;
   __shared__ unsigned long long sh_rank;
   __shared__ long long wsum[NT / kWave];
   __shared__ double wsumd[NT / kWave];
+  __shared__ T cand[kCandCap];
+  __shared__ int ncand;
   const int32_t q = jobs[blockIdx.x];
   const int64_t b = cp[q], N = cp[q + 1] - b;
   const T* v = val + b;
   const int j = job[q];
-  const int64_t k0 = j == kJobRecover ? recover : select;
-  T c = select_kth<T, NT>(v, N, k0 < N ? k0 : N, hist, &sh_prefix,
-                          &sh_rank);
+  const int64_t nP = npos[q];
+  int64_t k0 = j == kJobRecover ? recover : select;
+  k0 = k0 < N ? k0 : N;
+  const bool use_cand = nP <= kCandCap && k0 <= nP;  // block-uniform
+  if (use_cand) {
+    if (threadIdx.x == 0) ncand = 0;
+    __syncthreads();
+    const int lane = lane_id();
+    for (int64_t p0 = 0; p0 < N; p0 += NT) {
+      const int64_t p = p0 + threadIdx.x;
+      T x = T(0);
+      bool take = false;
+      if (p < N) {
+        x = v[p];
+        take = !(x <= thr);
+      }
+      const unsigned long long m = __ballot(take);
+      int base = 0;
+      if (lane == 0 && m) base = atomicAdd(&ncand, __popcll(m));
+      base = __shfl(base, 0, kWave);
+      if (take) cand[base + __popcll(m & ((1ull << lane) - 1ull))] = x;
+    }
+    __syncthreads();
+  }
+  const int M = use_cand ? ncand : 0;
+  T c = use_cand ? select_kth<T, NT>(cand, M, k0, hist, &sh_prefix, &sh_rank)
+                 : select_kth<T, NT>(v, N, k0, hist, &sh_prefix, &sh_rank);
+  // entries >= c all lie among the candidates when c is above the threshold
+  auto src_for = [&](T cc, const T*& src, int64_t& len) {
+    if (use_cand && cc > thr) {
+      src = cand;
+      len = M;
+    } else {
+      src = v;
+      len = N;
+    }
+  };
   if (j == kJobSelect && recover > 0) {
+    const T* src;
+    int64_t len;
+    src_for(c, src, len);
     long long n1 = 0;
     double s1 = 0;
-    for (int64_t p = threadIdx.x; p < N; p += NT) {
-      const T x = v[p];
+    for (int64_t p = threadIdx.x; p < len; p += NT) {
+      const T x = src[p];
       if (!(x < c)) {
         ++n1;
         s1 += (double)x;
@@ -204,12 +251,17 @@ __global__ __launch_bounds__(NT) void k_mcl_select(const int64_t* __restrict__ c
     n1 = block_sum_ll<NT>(n1, wsum);
     s1 = block_sum_f64<NT>(s1, wsumd);
     if (n1 < recover && s1 < pct) {
-      c = select_kth<T, NT>(v, N, recover < N ? recover : N, hist, &sh_prefix, &sh_rank);
+      const int64_t kr = recover < N ? recover : N;
+      c = (use_cand && kr <= nP) ? select_kth<T, NT>(cand, M, kr, hist, &sh_prefix, &sh_rank)
+                                 : select_kth<T, NT>(v, N, kr, hist, &sh_prefix, &sh_rank);
       if (threadIdx.x == 0) atomicAdd(&counters[3], 1ull);
     }
   }
+  const T* src;
+  int64_t len;
+  src_for(c, src, len);
   long long kept = 0;
-  for (int64_t p = threadIdx.x; p < N; p += NT) kept += !(v[p] < c);
+  for (int64_t p = threadIdx.x; p < len; p += NT) kept += !(src[p] < c);
   kept = block_sum_ll<NT>(kept, wsum);
   if (threadIdx.x == 0) {
     cut[q] = c;
@@ -325,13 +377,14 @@ static cbx_status mcl_prune_typed(const cbx_dcsc* C, double thr_d, int64_t selec
   Scratch sc;
   T* cut;
   int64_t *cnt, *off;
-  int32_t *job, *jobs, *flag, *cpos;
+  int32_t *job, *jobs, *flag, *cpos, *npos;
   unsigned long long* counters;
   CBX_TRY(sc.get(&cut, nzc));
   CBX_TRY(sc.get(&cnt, nzc + 1));
   CBX_TRY(sc.get(&off, nzc + 1));
   CBX_TRY(sc.get(&job, nzc));
   CBX_TRY(sc.get(&jobs, nzc));
+  CBX_TRY(sc.get(&npos, nzc));
   CBX_TRY(sc.get(&flag, nzc + 1));
   CBX_TRY(sc.get(&cpos, nzc + 1));
   CBX_TRY(sc.get(&counters, 4));
@@ -340,15 +393,15 @@ static cbx_status mcl_prune_typed(const cbx_dcsc* C, double thr_d, int64_t selec
   CBX_HIP(hipMemsetAsync(flag + nzc, 0, sizeof(int32_t), s));
   if (nzc) {
     k_mcl_stats<T><<<wave_grid(nzc), 256, 0, s>>>(C->cp, (const T*)C->val, nzc, thr, select, recover,
-                                                              pct, cut, cnt, job, jobs, counters);
+                                                              pct, cut, cnt, job, jobs, npos, counters);
     CBX_HIP(hipGetLastError());
   }
   unsigned long long hc[4] = {0, 0, 0, 0};
   CBX_HIP(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipStreamSynchronize(s));
   if (hc[0]) {
-    k_mcl_select<T, kSelNT><<<(unsigned)hc[0], kSelNT, 0, s>>>(C->cp, (const T*)C->val, jobs, job, select, recover,
-                                                               pct, cut, cnt, counters);
+    k_mcl_select<T, kSelNT><<<(unsigned)hc[0], kSelNT, 0, s>>>(C->cp, (const T*)C->val, jobs, job, npos, thr,
+                                                               select, recover, pct, cut, cnt, counters);
     CBX_HIP(hipGetLastError());
   }
   size_t tmp_bytes = 0, tb2 = 0;
