@@ -237,6 +237,8 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["both", "two_phase", "learned"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
     ap.add_argument("--no-extra", action="store_true", help="skip the BFS, MCL and Galerkin workloads")
+    ap.add_argument("--only", choices=["bfs", "mcl", "galerkin"], default=None,
+                    help="run only this workload and print its JSON (for profiling)")
     ap.add_argument("--copy-gib", type=float, default=4.0,
                     help="size of a device-to-device copy timed before the workload: the achievable HBM rate, and "
                          "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters (0 disables)")
@@ -251,6 +253,11 @@ def main() -> int:
 
     if cb.device_count() < 1:
         raise SystemExit("no GPU visible")
+    if args.only:
+        fn = {"bfs": lambda: bench_bfs(cb, args.scale, args.edgefactor), "mcl": lambda: bench_mcl(cb),
+              "galerkin": lambda: bench_galerkin(cb)}[args.only]
+        print(json.dumps({args.only: fn()}))
+        return 0
     copy = None
     if args.copy_gib > 0:
         nb = int(args.copy_gib * 2 ** 30)
