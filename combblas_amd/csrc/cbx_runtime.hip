@@ -309,7 +309,7 @@ extern "C" cbx_status cbx_spgemm_set_predictor(const float* params) {
 
 // ------------------------------------------------------------- tunables
 static std::mutex g_config_mu;
-static cbx_spgemm_config g_config = {4096, 1024, 128, 65536, 0.75f, 0.15f};
+static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 0.75f, 0.15f};
 
 SpgemmConfig cbx::current_config() {
   SpgemmConfig c;
@@ -317,7 +317,7 @@ SpgemmConfig cbx::current_config() {
     std::lock_guard<std::mutex> g(g_config_mu);
     c.small_flop_max = g_config.hash_max_flop;
     c.hash_mid_flop = g_config.hash_mid_flop;
-    c.hash_small_flop = g_config.hash_small_flop;
+    c.tiny_flop = g_config.tiny_flop;
     c.heavy_nnz = g_config.heavy_nnz;
     c.learned_target = g_config.learned_target;
     c.capacity_margin = g_config.capacity_margin;
@@ -335,10 +335,10 @@ extern "C" cbx_status cbx_spgemm_get_config(cbx_spgemm_config* cfg) {
 
 extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
   CBX_RETURN_IF(!cfg, CBX_ERR_INVALID, "cbx_spgemm_set_config: null argument");
-  CBX_RETURN_IF(cfg->hash_small_flop < 0 || cfg->hash_small_flop > 128, CBX_ERR_INVALID,
-                "cbx_spgemm_set_config: hash_small_flop must be in [0, 128]");
-  CBX_RETURN_IF(cfg->hash_mid_flop < cfg->hash_small_flop || cfg->hash_mid_flop > 1024, CBX_ERR_INVALID,
-                "cbx_spgemm_set_config: hash_mid_flop must be in [hash_small_flop, 1024]");
+  CBX_RETURN_IF(cfg->tiny_flop < 0 || cfg->tiny_flop > 256, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: tiny_flop must be in [0, 256]");
+  CBX_RETURN_IF(cfg->hash_mid_flop < cfg->tiny_flop || cfg->hash_mid_flop > 1024, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: hash_mid_flop must be in [tiny_flop, 1024]");
   CBX_RETURN_IF(cfg->hash_max_flop < cfg->hash_mid_flop || cfg->hash_max_flop > 4096, CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: hash_max_flop must be in [hash_mid_flop, 4096]");
   CBX_RETURN_IF(cfg->heavy_nnz < 4096, CBX_ERR_INVALID, "cbx_spgemm_set_config: heavy_nnz must be >= 4096");

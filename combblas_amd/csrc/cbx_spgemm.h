@@ -14,10 +14,11 @@ struct SpgemmConfig {
   // slots and is kept at most half full).
   int64_t small_flop_max = 4096;
   // Hash tier table sizes: flop_j in (hash_mid_flop, small_flop_max] use the
-  // 8192-slot table, (hash_small_flop, hash_mid_flop] the 2048-slot table,
-  // the rest the 256-slot table (each table kept at most half full).
+  // 8192-slot table, (tiny_flop, hash_mid_flop] the 2048-slot table (each
+  // table kept at most half full); columns with flop_j <= tiny_flop (<= 256)
+  // take one wave each (the tiny tier).
   int64_t hash_mid_flop = 1024;
-  int64_t hash_small_flop = 128;
+  int64_t tiny_flop = 256;
   // Unit tier: columns whose output exceeds heavy_nnz use the large units.
   int64_t heavy_nnz = 65536;
   // Learned mode: device parameters of the nnz predictor (null = not loaded),

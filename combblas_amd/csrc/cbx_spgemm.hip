@@ -314,6 +314,179 @@ __global__ __launch_bounds__(NT) void k_sym_hash(const int32_t* __restrict__ ord
   if (threadIdx.x == 0) nnzcol[col] = tot;
 }
 
+// ------------------------------------------------------------ tiny tier
+// Columns with at most 256 products (most columns of multigrid triple
+// products, and the sparsest R-MAT columns): one wave per column, no LDS.
+// Lane l holds elements e = u*64 + l (u < 4).  The products are gathered with
+// a wave scan of the B entries' A-segment lengths and a 6-step binary search
+// over it by shuffles; a bitonic network sorts the 256 (row, value) pairs in
+// registers (partners in another lane through shuffles, in another register
+// of the same lane by swaps); equal rows are then adjacent, a segmented scan
+// in element order reduces each run, and the run's last element writes the
+// output at its rank.  SYM only counts the distinct rows.
+constexpr int kTinyMax = 256;
+
+template <class X>
+__device__ __forceinline__ X shfl_x(X v, int src) {
+  if constexpr (sizeof(X) < 4) return (X)__shfl((int)v, src, kWave);
+  else return __shfl(v, src, kWave);
+}
+
+template <int K, class T, bool SYM>
+__global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order, int64_t first, int64_t ncols,
+                                              const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+                                              const T* __restrict__ valB, const int64_t* __restrict__ cpA,
+                                              const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
+                                              int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpC,
+                                              int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int kE = kTinyMax / kWave;  // 4 elements per lane
+  constexpr int32_t kNone = 0x7fffffff;
+  const int lane = lane_id();
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave_id(); i < ncols; i += waves) {
+    const int col = order[first + i];
+    const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+    int32_t key[kE];
+    Acc val[kE];
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      key[u] = kNone;
+      val[u] = SR::ident();
+    }
+    int filled = 0;  // wave-uniform
+    for (int64_t eb = b0; eb < b1; eb += kWave) {
+      int64_t base;
+      int len;
+      T bv;
+      load_seg<T, !SYM>(eb + lane, b1, bpos, cpA, nullptr, 0, 0, 0, true, valB, base, len, bv);
+      const int inc = wave_incl_scan(len);
+      const int off = filled + inc - len;  // first element of this lane's segment
+      const int total = __shfl(inc, kWave - 1, kWave);
+#pragma unroll
+      for (int u = 0; u < kE; ++u) {
+        const int e = u * kWave + lane;
+        // last lane s with off_s <= e (segments of length 0 share offsets; the
+        // search lands on the one that holds e when e is in this chunk)
+        int lo = 0;
+#pragma unroll
+        for (int step = kWave / 2; step > 0; step >>= 1) {
+          const int o = __shfl(off, lo + step, kWave);
+          if (o <= e) lo += step;
+        }
+        const int o = __shfl(off, lo, kWave);
+        const int l = __shfl(len, lo, kWave);
+        const int64_t bs = shfl_x(base, lo);
+        const T bval = shfl_x(bv, lo);
+        if (e >= filled && e < filled + total && e - o < l) {
+          const int64_t idx = bs + (e - o);
+          key[u] = rowsA[idx];
+          if constexpr (!SYM) val[u] = SR::mul(valA[idx], bval);
+        }
+      }
+      filled += total;
+    }
+    if (filled > kTinyMax && lane == 0) atomicOr(err, 4);
+    // bitonic sort of 256 elements, ascending by row
+#pragma unroll
+    for (int k = 2; k <= kTinyMax; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        if (j >= kWave) {
+          const int du = j / kWave;
+#pragma unroll
+          for (int u = 0; u < kE; ++u) {
+            const int u2 = u ^ du;
+            if (u2 > u) {
+              const bool asc = (((u * kWave + lane) & k) == 0);
+              if ((key[u] > key[u2]) == asc && key[u] != key[u2]) {
+                const int32_t tk = key[u];
+                key[u] = key[u2];
+                key[u2] = tk;
+                if constexpr (!SYM) {
+                  const Acc tv = val[u];
+                  val[u] = val[u2];
+                  val[u2] = tv;
+                }
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < kE; ++u) {
+            const int e = u * kWave + lane;
+            const bool asc = ((e & k) == 0);
+            const bool lower = (lane & j) == 0;
+            const int32_t pk = __shfl_xor(key[u], j, kWave);
+            Acc pv{};
+            if constexpr (!SYM) pv = __shfl_xor(val[u], j, kWave);
+            const bool take = lower == asc ? pk < key[u] : pk > key[u];
+            if (take) {
+              key[u] = pk;
+              if constexpr (!SYM) val[u] = pv;
+            }
+          }
+        }
+      }
+    }
+    // run heads (first element of each distinct row)
+    bool head[kE];
+    unsigned long long hm[kE];
+    int count = 0;
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      int32_t prev = __shfl_up(key[u], 1, kWave);
+      const int32_t carry = u > 0 ? __shfl(key[u > 0 ? u - 1 : 0], kWave - 1, kWave) : kNone;
+      if (lane == 0) prev = u > 0 ? carry : -1;
+      head[u] = key[u] != kNone && key[u] != prev;
+      hm[u] = __ballot(head[u]);
+      count += __popcll(hm[u]);
+    }
+    if constexpr (SYM) {
+      if (lane == 0) nnzcol[col] = count;
+      continue;
+    } else {
+      const int64_t have = nnzcol[col];  // the column's allocation from the symbolic pass
+      if (lane == 0 && count != have) atomicOr(err, 8);
+      const int64_t out = cpC[col];
+      Acc carry = SR::ident();
+      bool open = false;  // a run continues from the previous register
+      int before = 0;     // heads in earlier registers
+#pragma unroll
+      for (int u = 0; u < kE; ++u) {
+        // segmented inclusive scan in element order
+        Acc x = val[u];
+        bool f = head[u];
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+          const Acc y = __shfl_up(x, d, kWave);
+          const bool g = __shfl_up((int)f, d, kWave) != 0;
+          if (lane >= d) {
+            if (!f) x = SR::add(y, x);
+            f = f || g;
+          }
+        }
+        if (!f && open) x = SR::add(carry, x);
+        // the last element of a run writes it at its rank
+        int32_t next = __shfl_down(key[u], 1, kWave);
+        const int32_t nfirst = u + 1 < kE ? __shfl(key[u + 1 < kE ? u + 1 : u], 0, kWave) : kNone;
+        if (lane == kWave - 1) next = nfirst;
+        if (key[u] != kNone && next != key[u]) {
+          const int rank = before + __popcll(hm[u] & ((2ull << lane) - 1ull)) - 1;
+          if (rank < have) {
+            irC[out + rank] = key[u];
+            valC[out + rank] = SR::out(x);
+          }
+        }
+        carry = __shfl(x, kWave - 1, kWave);
+        open = __shfl((int)(key[u] != kNone), kWave - 1, kWave) != 0;
+        before += __popcll(hm[u]);
+      }
+    }
+  }
+}
+
 // -------------------------------------------------------- symbolic: dense
 // Distinct-row count with an LDS bitmap of RBW*32 rows per pass; a pass
 // covers G row tiles of the tile-pointer table.
@@ -1335,7 +1508,10 @@ constexpr int kSymUnitNT = 1024;
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 32;    // <= 4096 outputs, <= 128K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
 constexpr int kLearnNT = 512, kLearnK = 4096, kLearnRB = 32;    // learned units: value window, span
-constexpr int kLearnFlopCap = 32768;                            // learned units: expected products
+constexpr int kLearnFlopCap = 32768;
+inline unsigned tiny_grid(int64_t ncols) {  // 4 columns per workgroup, grid-stride beyond 64K workgroups
+  return (unsigned)std::min<int64_t>((ncols + 3) / 4, 65536);
+}                            // learned units: expected products
 
 }  // namespace
 
@@ -1546,8 +1722,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
                                                        s));
   CBX_HIP(hipcub::DeviceReduce::Sum(tmp, tmp_bytes, flop, bounds + 5, (int)nzcB, s));
 
-  // class thresholds: dense > small_max ; hash8k > 1024 ; hash2k > 128 ; hash256 > 0 ; heavy > heavy_nnz
-  int64_t hthr[5] = {cfg.small_flop_max, cfg.hash_mid_flop, cfg.hash_small_flop, 0, cfg.heavy_nnz};
+  // class thresholds: units > small_max ; hash8k > mid ; hash2k > tiny ; tiny (one wave) > 0 ; heavy > heavy_nnz
+  int64_t hthr[5] = {cfg.small_flop_max, cfg.hash_mid_flop, cfg.tiny_flop, 0, cfg.heavy_nnz};
   CBX_HIP(hipMemcpyAsync(thr, hthr, sizeof(hthr), hipMemcpyHostToDevice, s));
   k_class_bounds<<<1, 64, 0, s>>>(flop_s, nzcB, thr, 5, bounds);
   int64_t hb[6];
@@ -1579,7 +1755,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (learned) {
     if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
     if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
-    if (n_h256) k_sym_hash<256, kHashNT><<<(unsigned)n_h256, kHashNT, 0, s>>>(order_s, hb[2], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+    if (n_h256) k_tiny<K, T, true><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr, nullptr, nullptr, err);
     tm.mark(2, s);
     tm.mark(3, s);
     CBX_TRY((learned_numeric<K, T>(A, B, cfg, s, rowsA, bpos, flop, P, S, nnzcol, cpC, tmp, tmp_bytes, C, err, linfo)));
@@ -1617,7 +1793,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   }
   if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
   if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
-  if (n_h256) k_sym_hash<256, kHashNT><<<(unsigned)n_h256, kHashNT, 0, s>>>(order_s, hb[2], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
+  if (n_h256) k_tiny<K, T, true><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr, nullptr, nullptr, err);
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
   if (n_dense) {
@@ -1669,7 +1845,9 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   // the 8192-slot table takes ~100 KB of LDS, so one workgroup per CU: give it 16 waves
   CBX_LAUNCH_HASH(8192, 1024, n_h8, hb[0]);
   CBX_LAUNCH_HASH(2048, kHashNT, n_h2, hb[1]);
-  CBX_LAUNCH_HASH(256, kHashNT, n_h256, hb[2]);
+  if (n_h256)
+    k_tiny<K, T, false><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
+                                                         valA, nnzcol, cpC, C->ir, (T*)C->val, err);
 #undef CBX_LAUNCH_HASH
   CBX_HIP(hipGetLastError());
 

@@ -172,12 +172,12 @@ enum {
 /* Process-wide tunables of the local product (the bin thresholds and table
  * sizes the reference hard-codes: the heap/hash switch cr < 2 at
  * mtSpGEMM.h:311 and minHashTableSize / hashScale at :366-367).  Picked per
- * workload by combblas_amd/autotune.py; defaults {4096, 1024, 128, 65536,
+ * workload by combblas_amd/autotune.py; defaults {4096, 1024, 256, 65536,
  * 0.75, 0.15}. */
 typedef struct cbx_spgemm_config {
   int64_t hash_max_flop;   /* columns with flop_j <= this use the LDS hash tier (<= 4096) */
   int64_t hash_mid_flop;   /* ... with flop_j > this use the 8192-slot table, else the 2048-slot one (<= 1024) */
-  int64_t hash_small_flop; /* ... with flop_j <= this use the 256-slot table (<= 128) */
+  int64_t tiny_flop;       /* columns with flop_j <= this take one wave each, sorted in registers (<= 256) */
   int64_t heavy_nnz;       /* unit tier: columns with more outputs use the 16384-output units (>= 4096) */
   float learned_target;    /* learned mode: predicted outputs per unit, as a fraction of its 4096-entry window */
   float capacity_margin;   /* learned mode: output capacity headroom over the predicted total */
