@@ -1308,17 +1308,20 @@ __global__ __launch_bounds__(NT) void k_num_learned(
   }
   // publish the size, then look back for the offset with the whole first
   // wave: 64 predecessors per step, summing back to the nearest one that
-  // holds an inclusive prefix (bounded wait on predecessors still counting)
+  // holds an inclusive prefix (bounded wait on predecessors still counting).
+  // A status word carries its flag and value together and no other data is
+  // passed between units, so the atomics are relaxed: at agent scope an
+  // acquire or release would invalidate or write back this XCD's whole L2.
   if (threadIdx.x < kWave) {
     const int lane = (int)threadIdx.x;
     if (lane == 0)
-      __hip_atomic_store(&status[t], kAggFlag | (unsigned long long)cnt, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&status[t], kAggFlag | (unsigned long long)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     long long excl = 0;
     int spins = 0;
     for (int i = t - 1; i >= 0;) {
       const int j = i - lane;
       const unsigned long long v =
-          j >= 0 ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kInclFlag;
+          j >= 0 ? __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kInclFlag;
       const unsigned long long incl = __ballot((v >> 62) == 2);
       const unsigned long long none = __ballot((v >> 62) == 0);
       const int first = incl ? __ffsll((unsigned long long)incl) - 1 : kWave;  // nearest inclusive prefix
@@ -1336,7 +1339,7 @@ __global__ __launch_bounds__(NT) void k_num_learned(
       i -= kWave;
     }
     if (lane == 0) {
-      __hip_atomic_store(&status[t], kInclFlag | (unsigned long long)(excl + cnt), __ATOMIC_RELEASE,
+      __hip_atomic_store(&status[t], kInclFlag | (unsigned long long)(excl + cnt), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       s_excl = excl;
       if (cnt) atomicAdd(&col_total[col], (unsigned long long)cnt);
