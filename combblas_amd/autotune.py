@@ -34,7 +34,8 @@ WORKLOADS = {
 GRID = {
     "hash_max_flop": [1024, 2048, 4096],
     "hash_mid_flop": [512, 1024],
-    "heavy_nnz": [32768, 65536, 131072],
+    "tiny_flop": [128, 256],
+    "heavy_nnz": [65536, 131072],
 }
 LEARNED_GRID = {"learned_target": [0.6, 0.75, 0.9]}
 
@@ -74,7 +75,7 @@ def tune(name: str, reps: int = 2, log=print) -> dict:
     ref_nnz = None
     for combo in itertools.product(*(grid[k] for k in keys)):
         cfg = dict(zip(keys, combo))
-        if cfg["hash_mid_flop"] > cfg["hash_max_flop"]:
+        if cfg["hash_mid_flop"] > cfg["hash_max_flop"] or cfg["tiny_flop"] > cfg["hash_mid_flop"]:
             continue
         cb.set_config(**dict(base, **cfg))
         ms, nnz = _time_config(dA, semiring, mode, reps)
@@ -100,9 +101,10 @@ def load_table(path: Path = TABLE_PATH) -> dict:
 
 
 def apply(name: str, path: Path = TABLE_PATH) -> dict | None:
-    """Installs the stored configuration for `name` (None if the table has none)."""
+    """Installs the stored configuration for `name` (None if the table has none
+    or it was written for other tunables)."""
     e = load_table(path)["entries"].get(name)
-    if e is None:
+    if e is None or set(e["best"]) != set(cb.get_config()):
         return None
     return cb.set_config(**e["best"])
 
