@@ -1148,8 +1148,9 @@ __device__ __forceinline__ int cut_units(int col, float pred, float flop, int lo
 
 __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, int64_t small_max,
                                 const float* __restrict__ logpred, const int32_t* __restrict__ blo,
-                                const int32_t* __restrict__ bhi, const float* __restrict__ w, float target, float fcap,
-                                int rblk, int64_t m, const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
+                                const int32_t* __restrict__ bhi, const float* __restrict__ w, float target_heavy,
+                                float target_light, float heavy_pred, float fcap, int rblk, int64_t m,
+                                const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
                                 UnitL* __restrict__ units, float* __restrict__ pred_nnz) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= nzcB) return;
@@ -1163,6 +1164,7 @@ __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, 
   float pred = exp2f(logpred[j]) - 1.f;
   const float cap = (float)min<int64_t>(flop[j], m);
   pred = fminf(fmaxf(pred, 1.f), cap);
+  const float target = pred > heavy_pred ? target_heavy : target_light;
   if (!units) {
     ucount[j] = cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, nullptr);
     pred_nnz[j] = pred;
@@ -1190,7 +1192,9 @@ __global__ __launch_bounds__(NT) void k_num_learned(
   constexpr int WPT = (NW64 + NT - 1) / NT;
   __shared__ Acc vals[KMAX];
   __shared__ uint32_t bits[NW32];
-  __shared__ int32_t pre[NW64];
+  // exclusive prefix counts stay below 64 * NW64 <= 65472 for RBLK <= 16
+  static_assert(64 * NW64 <= 65536, "prefix counts must fit 16 bits");
+  __shared__ uint16_t pre[NW64];
   __shared__ SegStage<T, NT> st;
   __shared__ long long wsum[NT / kWave];
   __shared__ int s_t;
@@ -1302,7 +1306,7 @@ __global__ __launch_bounds__(NT) void k_num_learned(
 #pragma unroll
     for (int q = 0; q < WPT; ++q) {
       const int w = threadIdx.x * WPT + q;
-      if (w < nw64) pre[w] = ex + local[q];
+      if (w < nw64) pre[w] = (uint16_t)(ex + local[q]);
     }
     cnt = (int)total;
   }
@@ -1510,8 +1514,11 @@ constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass 
 constexpr int kSymUnitNT = 1024;
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 32;    // <= 4096 outputs, <= 128K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
-constexpr int kLearnNT = 512, kLearnK = 4096, kLearnRB = 32;    // learned units: value window, span
-constexpr int kLearnFlopCap = 32768;
+// learned units: one workgroup shape for all (1024 threads, 16384-entry value
+// window, 16 row blocks); columns predicted larger than heavy_nnz are cut to
+// target * 16384 outputs, the others to target * 4096
+constexpr int kLearnNT = 1024, kLearnK = 16384, kLearnRB = 16, kLearnCR = 2;
+constexpr int kLearnFlopCap = 131072;  // expected products per learned unit
 inline unsigned tiny_grid(int64_t ncols) {  // 4 columns per workgroup, grid-stride beyond 64K workgroups
   return (unsigned)std::min<int64_t>((ncols + 3) / 4, 65536);
 }                            // learned units: expected products
@@ -1567,9 +1574,12 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   k_col_blocks<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, blo, bhi);
   // units in column order: count, scan, write
   const float target = cfg.learned_target * (float)kLearnK;
+  const float target_light = cfg.learned_target * 4096.f;
+  const float heavy_pred = (float)cfg.heavy_nnz;
   const float fcap = (float)kLearnFlopCap;
   k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                  fcap, kLearnRB, m, nullptr, ucount, nullptr, pred_nnz);
+                                                  target_light, heavy_pred, fcap, kLearnRB, m, nullptr, ucount,
+                                                  nullptr, pred_nnz);
   CBX_HIP(hipMemsetAsync(ucount + nzcB, 0, sizeof(int64_t), s));
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ucount, ubase, (int)(nzcB + 1), s));
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
@@ -1588,7 +1598,8 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_status, std::max<int64_t>(nunits, 1), s, &status));
   if (nunits)
     k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                    fcap, kLearnRB, m, ubase, nullptr, units, nullptr);
+                                                    target_light, heavy_pred, fcap, kLearnRB, m, ubase, nullptr,
+                                                    units, nullptr);
   // output capacity from the predictions (plus the exact hash-tier sizes)
   const int64_t cap = hash_total + (int64_t)(pred_sum * (1.0 + cfg.capacity_margin)) + (1 << 20);
   CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * cap));
@@ -1602,7 +1613,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   (void)hipEventCreate(&kev[1]);
   (void)hipEventRecord(kev[0], s);
   if (nunits)
-    k_num_learned<K, T, kLearnNT, 4, kLearnK, kLearnRB, 2><<<(unsigned)nunits, kLearnNT, 0, s>>>(
+    k_num_learned<K, T, kLearnNT, 4, kLearnK, kLearnRB, kLearnCR><<<(unsigned)nunits, kLearnNT, 0, s>>>(
         units, ticket, status, hcs, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cap, C->ir,
         (T*)C->val, col_total, ovf, err);
   (void)hipEventRecord(kev[1], s);
