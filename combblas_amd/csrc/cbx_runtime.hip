@@ -342,8 +342,8 @@ extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
   CBX_RETURN_IF(cfg->hash_max_flop < cfg->hash_mid_flop || cfg->hash_max_flop > 4096, CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: hash_max_flop must be in [hash_mid_flop, 4096]");
   CBX_RETURN_IF(cfg->heavy_nnz < 4096, CBX_ERR_INVALID, "cbx_spgemm_set_config: heavy_nnz must be >= 4096");
-  CBX_RETURN_IF(!(cfg->learned_target > 0.05f && cfg->learned_target <= 1.0f), CBX_ERR_INVALID,
-                "cbx_spgemm_set_config: learned_target must be in (0.05, 1]");
+  CBX_RETURN_IF(!(cfg->learned_target > 0.05f && cfg->learned_target <= 4.0f), CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: learned_target must be in (0.05, 4]");
   CBX_RETURN_IF(!(cfg->capacity_margin >= 0.0f && cfg->capacity_margin <= 4.0f), CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: capacity_margin must be in [0, 4]");
   std::lock_guard<std::mutex> g(g_config_mu);

