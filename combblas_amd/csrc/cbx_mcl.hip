@@ -399,6 +399,8 @@ static cbx_status mcl_prune_typed(const cbx_dcsc* C, double thr_d, int64_t selec
   unsigned long long hc[4] = {0, 0, 0, 0};
   CBX_HIP(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipStreamSynchronize(s));
+  CBX_RETURN_IF(hc[0] * (unsigned long long)kSelNT >= (1ull << 32), CBX_ERR_LIMIT,
+                "cbx_mcl_prune: too many selected columns for a single launch");
   if (hc[0]) {
     k_mcl_select<T, kSelNT><<<(unsigned)hc[0], kSelNT, 0, s>>>(C->cp, (const T*)C->val, jobs, job, npos, thr,
                                                                select, recover, pct, cut, cnt, counters);

@@ -1519,6 +1519,7 @@ constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outp
 // target * 16384 outputs, the others to target * 4096
 constexpr int kLearnNT = 1024, kLearnK = 16384, kLearnRB = 16, kLearnCR = 2;
 constexpr int kLearnFlopCap = 131072;  // expected products per learned unit
+constexpr int64_t kLearnChunk = 1 << 21;  // workgroups per learned launch
 inline unsigned tiny_grid(int64_t ncols) {  // 4 columns per workgroup, grid-stride beyond 64K workgroups
   return (unsigned)std::min<int64_t>((ncols + 3) / 4, 65536);
 }                            // learned units: expected products
@@ -1612,10 +1613,15 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   (void)hipEventCreate(&kev[0]);
   (void)hipEventCreate(&kev[1]);
   (void)hipEventRecord(kev[0], s);
-  if (nunits)
-    k_num_learned<K, T, kLearnNT, 4, kLearnK, kLearnRB, kLearnCR><<<(unsigned)nunits, kLearnNT, 0, s>>>(
+  // launches of at most 2^21 workgroups (a dispatch holds fewer than 2^32
+  // work-items); the ticket counter runs on across them, and a later launch's
+  // units look back into statuses the earlier launches completed
+  for (int64_t done = 0; done < nunits; done += kLearnChunk) {
+    const int64_t nb = std::min<int64_t>(kLearnChunk, nunits - done);
+    k_num_learned<K, T, kLearnNT, 4, kLearnK, kLearnRB, kLearnCR><<<(unsigned)nb, kLearnNT, 0, s>>>(
         units, ticket, status, hcs, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cap, C->ir,
         (T*)C->val, col_total, ovf, err);
+  }
   (void)hipEventRecord(kev[1], s);
   CBX_HIP(hipGetLastError());
   k_add_col_totals<<<nblk(nzcB, 256), 256, 0, s>>>(nnzcol, col_total, nzcB);
@@ -1747,6 +1753,10 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const int64_t n_dense = hb[0], n_h8 = hb[1] - hb[0], n_h2 = hb[2] - hb[1], n_h256 = hb[3] - hb[2];
   const int64_t n_heavy_max = hb[4];  // columns whose flops (an upper bound of nnz) exceed heavy_nnz
   const int64_t total_flop = hb[5];
+  // one workgroup per column (hash tiers, symbolic units): a dispatch holds
+  // fewer than 2^32 work-items
+  CBX_RETURN_IF(n_dense * 1024 >= (1ll << 32) || n_h8 * 1024 >= (1ll << 32) || n_h2 * kHashNT >= (1ll << 32),
+                CBX_ERR_LIMIT, "cbx_spgemm: too many columns in one tier for a single launch");
   // tile pointers only when some column is beyond the hash tier (the table is
   // nzc(A) x m/4096 entries, large for tall packed batches of small columns)
   P = nullptr;
@@ -1833,6 +1843,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   tm.mark(3, s);
 
   // --- numeric
+  CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32), CBX_ERR_LIMIT,
+                "cbx_spgemm: too many units for a single launch");
   if (use_units) {
     if (hu[1])
       k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
