@@ -317,6 +317,13 @@ def main() -> int:
     main_bytes = 16 * main_nnz + dcsc_bytes(A.nnz, A.nzc) * 2
     achieved = main_bytes / (main_ms * 1e-3) / 1e9 if main_ms > 0 else None
     kernel = "k_num_learned" if primary == "learned" else "k_num_units (light+heavy launches)"
+    traffic = None  # HBM bytes per launch from the committed rocprofv3 --pmc summary of this kernel
+    tpath = os.path.join(ROOT, "profiles", f"traffic_rmat_s{args.scale}_ef{args.edgefactor}.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        k = t["kernels"].get(primary)
+        if k:
+            traffic = int((k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"]) * 1024)
 
     per_mode = {}
     for mode, rr in res.items():
@@ -352,7 +359,7 @@ def main() -> int:
         "output_bit_exact": all(rr["exact"] for rr in res.values()),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": kernel, "alg_bytes_per_launch": main_bytes,
+                     "traffic": traffic, "kernel": kernel, "alg_bytes_per_launch": main_bytes,
                      "kernel_ms": round(main_ms, 3)},
         "cpu_baseline": None,
         "device_copy": copy, "tunables": cb.get_config(), "tuned_from_table": tuned is not None,

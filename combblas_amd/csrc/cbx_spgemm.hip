@@ -26,6 +26,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -1192,9 +1193,9 @@ __global__ __launch_bounds__(NT) void k_num_learned(
   constexpr int WPT = (NW64 + NT - 1) / NT;
   __shared__ Acc vals[KMAX];
   __shared__ uint32_t bits[NW32];
-  // exclusive prefix counts stay below 64 * NW64 <= 65472 for RBLK <= 16
-  static_assert(64 * NW64 <= 65536, "prefix counts must fit 16 bits");
-  __shared__ uint16_t pre[NW64];
+  // exclusive prefix counts: 16 bits while 64 * NW64 <= 65536
+  using PreT = typename std::conditional<(64 * NW64 <= 65536), uint16_t, int32_t>::type;
+  __shared__ PreT pre[NW64];
   __shared__ SegStage<T, NT> st;
   __shared__ long long wsum[NT / kWave];
   __shared__ int s_t;
@@ -1306,7 +1307,7 @@ __global__ __launch_bounds__(NT) void k_num_learned(
 #pragma unroll
     for (int q = 0; q < WPT; ++q) {
       const int w = threadIdx.x * WPT + q;
-      if (w < nw64) pre[w] = (uint16_t)(ex + local[q]);
+      if (w < nw64) pre[w] = (PreT)(ex + local[q]);
     }
     cnt = (int)total;
   }
@@ -1514,10 +1515,9 @@ constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass 
 constexpr int kSymUnitNT = 1024;
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 32;    // <= 4096 outputs, <= 128K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
-// learned units: one workgroup shape for all (1024 threads, 16384-entry value
-// window, 16 row blocks); columns predicted larger than heavy_nnz are cut to
-// target * 16384 outputs, the others to target * 4096
-constexpr int kLearnNT = 1024, kLearnK = 16384, kLearnRB = 16, kLearnCR = 2;
+// learned units: one workgroup shape for all (512 threads, 4096-entry value
+// window, 32 row blocks), cut to target * 4096 predicted outputs
+constexpr int kLearnNT = 512, kLearnK = 4096, kLearnRB = 32, kLearnCR = 2;
 constexpr int kLearnFlopCap = 131072;  // expected products per learned unit
 constexpr int64_t kLearnChunk = 1 << 21;  // workgroups per learned launch
 inline unsigned tiny_grid(int64_t ncols) {  // 4 columns per workgroup, grid-stride beyond 64K workgroups
@@ -1575,7 +1575,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   k_col_blocks<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, blo, bhi);
   // units in column order: count, scan, write
   const float target = cfg.learned_target * (float)kLearnK;
-  const float target_light = cfg.learned_target * 4096.f;
+  const float target_light = target;
   const float heavy_pred = (float)cfg.heavy_nnz;
   const float fcap = (float)kLearnFlopCap;
   k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
