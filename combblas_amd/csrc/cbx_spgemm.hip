@@ -1152,9 +1152,11 @@ __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, 
                                 const int32_t* __restrict__ bhi, const float* __restrict__ w, float target_heavy,
                                 float target_light, float heavy_pred, float fcap, int rblk, int64_t m,
                                 const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
-                                UnitL* __restrict__ units, float* __restrict__ pred_nnz) {
+                                UnitL* __restrict__ units, float* __restrict__ pred_nnz,
+                                uint8_t* __restrict__ heavy_flag) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= nzcB) return;
+  if (!units) heavy_flag[j] = 0;
   if (flop[j] <= small_max || bhi[j] < 0) {
     if (!units) {
       ucount[j] = 0;
@@ -1165,7 +1167,15 @@ __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, 
   float pred = exp2f(logpred[j]) - 1.f;
   const float cap = (float)min<int64_t>(flop[j], m);
   pred = fminf(fmaxf(pred, 1.f), cap);
-  const float target = pred > heavy_pred ? target_heavy : target_light;
+  if (pred > heavy_pred) {  // predicted heavy: exact symbolic and the large two-phase units
+    if (!units) {
+      heavy_flag[j] = 1;
+      ucount[j] = 0;
+      pred_nnz[j] = 0.f;
+    }
+    return;
+  }
+  const float target = target_light;
   if (!units) {
     ucount[j] = cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, nullptr);
     pred_nnz[j] = pred;
@@ -1527,7 +1537,7 @@ inline unsigned tiny_grid(int64_t ncols) {  // 4 columns per workgroup, grid-str
 }  // namespace
 
 struct LearnedInfo {
-  int64_t nnzC = 0, units = 0, overflow_units = 0, hash_total = 0;
+  int64_t nnzC = 0, units = 0, overflow_units = 0, hash_total = 0, heavy_units = 0;
   double pred_sum = 0;
   float kernel_ms = 0;  // the single-pass numeric kernel alone
   bool over_capacity = false;
@@ -1543,7 +1553,13 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
                                   int S, int64_t* nnzcol, int64_t* cpC, void* tmp, size_t tmp_bytes, cbx_dcsc* C,
                                   int* err, LearnedInfo& info) {
   const int64_t nzcB = B->nzc, m = A->m;
-  DevBuf b_feat, b_lp, b_hist, b_w, b_blo, b_bhi, b_uc, b_ub, b_pn, b_units, b_status, b_tick, b_ct, b_ovf, b_hcs, b_sum;
+  DevBuf b_feat, b_lp, b_hist, b_w, b_blo, b_bhi, b_uc, b_ub, b_pn, b_units, b_status, b_tick, b_ct, b_ovf, b_hcs, b_sum,
+      b_hflag, b_hcols, b_nh, b_ul, b_uh, b_hu;
+  uint8_t* heavy_flag;
+  int32_t* hcols;
+  int64_t* nh_dev;
+  int* hucnt;
+  Unit *units_l = nullptr, *units_h = nullptr;
   float *feats, *logpred, *w, *pred_nnz;
   unsigned long long *hist, *status, *col_total;
   int32_t *blo, *bhi;
@@ -1565,6 +1581,10 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_tick, 2, s, &ticket));
   CBX_TRY(dalloc(b_ovf, 2, s, &ovf));
   CBX_TRY(dalloc(b_ct, nzcB, s, &col_total));
+  CBX_TRY(dalloc(b_hflag, nzcB, s, &heavy_flag));
+  CBX_TRY(dalloc(b_hcols, nzcB, s, &hcols));
+  CBX_TRY(dalloc(b_nh, 1, s, &nh_dev));
+  CBX_TRY(dalloc(b_hu, 2, s, &hucnt));
   // prediction
   k_features<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, m, feats);
   mlp_infer_launch(cfg.predictor, feats, nzcB, logpred, s);
@@ -1580,9 +1600,46 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   const float fcap = (float)kLearnFlopCap;
   k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
                                                   target_light, heavy_pred, fcap, kLearnRB, m, nullptr, ucount,
-                                                  nullptr, pred_nnz);
+                                                  nullptr, pred_nnz, heavy_flag);
   CBX_HIP(hipMemsetAsync(ucount + nzcB, 0, sizeof(int64_t), s));
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ucount, ubase, (int)(nzcB + 1), s));
+  // columns predicted heavy: exact sizes from the symbolic unit pass (their
+  // units then go through the two-phase unit kernels), so the single pass
+  // covers the many light columns and its look-back sees the heavy columns'
+  // sizes up front, like the hash tier's
+  int64_t hash_only = 0;
+  {
+    size_t need = 0, need2 = 0;
+    CBX_HIP(hipcub::DeviceReduce::Sum(nullptr, need, nnzcol, nh_dev, (int)nzcB, s));
+    CBX_HIP(hipcub::DeviceSelect::Flagged(nullptr, need2, hipcub::CountingInputIterator<int32_t>(0), heavy_flag,
+                                          hcols, nh_dev, (int)nzcB, s));
+    need = std::max(need, need2);
+    DevBuf b_tmp2;
+    void* t2 = tmp;
+    if (need > tmp_bytes) CBX_TRY(dalloc(b_tmp2, need, s, (uint8_t**)&t2));
+    const size_t tb = std::max(need, tmp_bytes);
+    size_t tb1 = tb, tb2 = tb;
+    CBX_HIP(hipcub::DeviceReduce::Sum(t2, tb1, nnzcol, nh_dev, (int)nzcB, s));
+    CBX_HIP(hipMemcpyAsync(&hash_only, nh_dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CBX_HIP(hipcub::DeviceSelect::Flagged(t2, tb2, hipcub::CountingInputIterator<int32_t>(0), heavy_flag, hcols,
+                                          nh_dev, (int)nzcB, s));
+    CBX_HIP(hipStreamSynchronize(s));  // b_tmp2 is released at scope exit
+  }
+  int64_t nheavy = 0;
+  CBX_HIP(hipMemcpyAsync(&nheavy, nh_dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  int64_t cap_l = 0, cap_h = 0;
+  if (nheavy) {
+    cap_l = std::max<int64_t>(1, std::min<int64_t>(nheavy * (int64_t)S, 1ll << 30));
+    cap_h = cap_l;
+    CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
+    CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
+    CBX_HIP(hipMemsetAsync(hucnt, 0, 2 * sizeof(int), s));
+    k_sym_units<kSymUnitNT, 4><<<(unsigned)nheavy, kSymUnitNT, 0, s>>>(
+        hcols, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.heavy_nnz, nnzcol,
+        units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, err);
+    CBX_HIP(hipGetLastError());
+  }
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
   CBX_HIP(hipMemsetAsync(dsum, 0, sizeof(double), s));
   k_sum_f32<<<256, 256, 0, s>>>(pred_nnz, nzcB, dsum);
@@ -1594,13 +1651,13 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_HIP(hipStreamSynchronize(s));
   info.units = nunits;
   info.pred_sum = pred_sum;
-  info.hash_total = hash_total;
+  info.hash_total = hash_only;
   CBX_TRY(dalloc(b_units, std::max<int64_t>(nunits, 1), s, &units));
   CBX_TRY(dalloc(b_status, std::max<int64_t>(nunits, 1), s, &status));
   if (nunits)
     k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
                                                     target_light, heavy_pred, fcap, kLearnRB, m, ubase, nullptr,
-                                                    units, nullptr);
+                                                    units, nullptr, nullptr);
   // output capacity from the predictions (plus the exact hash-tier sizes)
   const int64_t cap = hash_total + (int64_t)(pred_sum * (1.0 + cfg.capacity_margin)) + (1 << 20);
   CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * cap));
@@ -1609,9 +1666,8 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_HIP(hipMemsetAsync(ticket, 0, sizeof(int) * 2, s));
   CBX_HIP(hipMemsetAsync(ovf, 0, sizeof(int) * 2, s));
   CBX_HIP(hipMemsetAsync(col_total, 0, sizeof(unsigned long long) * nzcB, s));
-  hipEvent_t kev[2] = {nullptr, nullptr};
-  (void)hipEventCreate(&kev[0]);
-  (void)hipEventCreate(&kev[1]);
+  hipEvent_t kev[3] = {nullptr, nullptr, nullptr};
+  for (auto& e : kev) (void)hipEventCreate(&e);
   (void)hipEventRecord(kev[0], s);
   // launches of at most 2^21 workgroups (a dispatch holds fewer than 2^32
   // work-items); the ticket counter runs on across them, and a later launch's
@@ -1633,10 +1689,35 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_HIP(hipStreamSynchronize(s));
   info.overflow_units = hovf;
   (void)hipEventElapsedTime(&info.kernel_ms, kev[0], kev[1]);
-  (void)hipEventDestroy(kev[0]);
-  (void)hipEventDestroy(kev[1]);
   info.over_capacity = (herr & 512) != 0 || info.nnzC > cap;
-  if (herr & ~512) return set_error(CBX_ERR_HIP, "cbx_spgemm: learned pass inconsistency (flag %d)", herr);
+  if (herr & ~512 || info.over_capacity || !nheavy) {
+    for (auto& e : kev) (void)hipEventDestroy(e);
+    if (herr & ~512) return set_error(CBX_ERR_HIP, "cbx_spgemm: learned pass inconsistency (flag %d)", herr);
+    return CBX_OK;
+  }
+  // the heavy columns' values, at the offsets the scan above fixed
+  int hu[2] = {0, 0};
+  CBX_HIP(hipMemcpy(hu, hucnt, sizeof(hu), hipMemcpyDeviceToHost));
+  (void)hipEventRecord(kev[2], s);  // numeric of the heavy columns, after the offsets
+  CBX_RETURN_IF(hu[0] > cap_l || hu[1] > cap_h, CBX_ERR_HIP, "cbx_spgemm: unit list overflow");
+  CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32), CBX_ERR_LIMIT,
+                "cbx_spgemm: too many units for a single launch");
+  info.heavy_units = hu[0] + hu[1];
+  if (hu[1])
+    k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+        units_h, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err);
+  if (hu[0])
+    k_num_units<K, T, kLightNT, 4, kLightK, kLightRB, 2><<<(unsigned)hu[0], kLightNT, 0, s>>>(
+        units_l, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err);
+  CBX_HIP(hipGetLastError());
+  (void)hipEventRecord(kev[1], s);
+  CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  float heavy_ms = 0;
+  (void)hipEventElapsedTime(&heavy_ms, kev[2], kev[1]);
+  info.kernel_ms += heavy_ms;
+  for (auto& e : kev) (void)hipEventDestroy(e);
+  if (herr) return set_error(CBX_ERR_HIP, "cbx_spgemm: learned pass, large columns inconsistent (flag %d)", herr);
   return CBX_OK;
 }
 
@@ -1903,7 +1984,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     stats->ms_main_kernel = learned_done ? linfo.kernel_ms : tm.ms(3, 4);
     stats->ms_total = tm.ms(0, 5);
     stats->units_light = learned_done ? linfo.units : hu[0];
-    stats->units_heavy = learned_done ? 0 : hu[1];
+    stats->units_heavy = learned_done ? linfo.heavy_units : hu[1];
     stats->nnz_main = nnz_main;
     stats->overflow_cols = linfo.overflow_units;
     stats->predicted_nnz = (int64_t)linfo.pred_sum;

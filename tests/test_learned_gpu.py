@@ -75,7 +75,7 @@ def test_learned_overflow_windows_exact():
     base = cb.get_config()
     cb.set_predictor(predictor.load())
     try:
-        cb.set_config(learned_target=3.0, heavy_nnz=4096)  # large units for every column above 4096
+        cb.set_config(learned_target=3.0)  # units three times their 4096-entry window
         got, st = learned(A, A)
     finally:
         cb.set_config(**base)
