@@ -274,7 +274,8 @@ def main() -> int:
     A = cb.gen_rmat(args.scale, args.edgefactor)
     dA = cb.Dcsc.to_device(A)
     from combblas_amd import autotune
-    tuned = autotune.apply(f"rmat_s{args.scale}_ef{args.edgefactor}")  # measured tunables, if stored
+    default_cfg = cb.get_config()
+    tuned = {}
 
     modes = ["two_phase", "learned"] if args.mode == "both" else [args.mode]
     if "learned" in modes:
@@ -285,7 +286,11 @@ def main() -> int:
             modes.remove("learned")
     res = {}
     for mode in modes:
-        log(f"spgemm {mode}: {args.warmup} warmup + {args.steps} steps")
+        # the autotuner's stored choice for this workload and mode, if any
+        name = f"rmat_s{args.scale}_ef{args.edgefactor}" + ("_learned" if mode == "learned" else "")
+        cb.set_config(**default_cfg)
+        tuned[mode] = autotune.apply(name)
+        log(f"spgemm {mode}: {args.warmup} warmup + {args.steps} steps, tunables {cb.get_config()}")
         res[mode] = run_spgemm(cb, A, dA, mode, args.steps, args.warmup)
         log(f"spgemm {mode}: {res[mode]['ms']:.2f} ms/step, exact={res[mode]['exact']}")
     baseline = None
@@ -362,7 +367,7 @@ def main() -> int:
                      "traffic": traffic, "kernel": kernel, "alg_bytes_per_launch": main_bytes,
                      "kernel_ms": round(main_ms, 3)},
         "cpu_baseline": None,
-        "device_copy": copy, "tunables": cb.get_config(), "tuned_from_table": tuned is not None,
+        "device_copy": copy, "tunables": {m: (t or default_cfg) for m, t in tuned.items()},
     }
     dA.release()
     cb.release_cached_memory()
