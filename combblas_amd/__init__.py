@@ -112,6 +112,8 @@ def lib() -> C.CDLL:
         L.cbx_gen_powerlaw_stochastic.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_int64, C.c_uint64,
                                                   P(cbx_dcsc)]
         L.cbx_gen_powerlaw_stochastic.restype = C.c_int32
+        L.cbx_spmv_dense.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        L.cbx_spmv_dense.restype = C.c_int32
         L.cbx_spgemm_get_config.argtypes = [P(cbx_spgemm_config)]
         L.cbx_spgemm_get_config.restype = C.c_int32
         L.cbx_spgemm_set_config.argtypes = [P(cbx_spgemm_config)]
@@ -382,6 +384,20 @@ def spgemm(A: DeviceDcsc, B: DeviceDcsc, semiring: str = "plus_times", mode: str
                             C.byref(st), stream))
     stats = {k: getattr(st, k) for k, _ in cbx_spgemm_stats._fields_}
     return DeviceDcsc(out), stats
+
+
+def spmv_dense(A: DeviceDcsc, x: np.ndarray, y: np.ndarray, semiring: str = "plus_times") -> np.ndarray:
+    """dcsc_gespmv on the device: returns y accumulated with A*x (host arrays in and out)."""
+    x = np.ascontiguousarray(x)
+    dt = {np.dtype(np.int64): DTYPES["int64"], np.dtype(np.float64): DTYPES["float64"],
+          np.dtype(np.float32): DTYPES["float32"]}[x.dtype]
+    dx = DeviceArray.from_host(x)
+    dy = DeviceArray.from_host(np.ascontiguousarray(y, dtype=x.dtype))
+    _check(lib().cbx_spmv_dense(C.byref(A.s), dx.ptr, dy.ptr, SEMIRINGS[semiring], dt, None))
+    out = dy.to_host()
+    dx.release()
+    dy.release()
+    return out
 
 
 def get_config() -> dict:

@@ -472,6 +472,40 @@ extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* pare
 
 extern "C" void cbx_bfs_plan_destroy(cbx_bfs_plan_s* p) { bfs_plan_free(p); }
 
+
+// ------------------------------------------------------ SpMV, dense vector
+// dcsc_gespmv (Friends.h:63-77): y[i] = add(y[i], multiply(A(i,j), x[j])) for
+// every stored entry, y accumulated in place.  One wave per stored column
+// (x[j] read once), the semiring's add applied with a global atomic; a
+// boolean A contributes x[j] itself.
+template <int K, class T, bool BOOLA>
+__global__ void k_spmv_dense(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
+                             const int64_t* __restrict__ ir, const void* __restrict__ valA_, int64_t nzc,
+                             const T* __restrict__ x, T* __restrict__ y) {
+  using SR = Sem<K, T>;
+  const int64_t q = global_wave();
+  if (q >= nzc) return;
+  const T xj = x[jc[q]];
+  for (int64_t e = cp[q] + lane_id(); e < cp[q + 1]; e += kWave) {
+    T prod;
+    if constexpr (BOOLA) prod = xj;
+    else prod = (T)SR::out(SR::mul(((const T*)valA_)[e], xj));
+    GAcc<K, T>::acc(&y[ir[e]], prod);
+  }
+}
+
+template <int K, class T>
+static cbx_status spmv_dense_typed(const cbx_dcsc* A, bool boolA, const T* x, T* y, hipStream_t s) {
+  if (A->nzc == 0) return CBX_OK;
+  if (boolA)
+    k_spmv_dense<K, T, true><<<wave_grid(A->nzc), 256, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, x, y);
+  else
+    k_spmv_dense<K, T, false><<<wave_grid(A->nzc), 256, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, x, y);
+  CBX_HIP(hipGetLastError());
+  CBX_HIP(hipStreamSynchronize(s));
+  return CBX_OK;
+}
+
 extern "C" cbx_status cbx_spmspv(const cbx_dcsc* A, const int32_t* xind, const void* xval, int32_t nx,
                                  cbx_semiring semiring, cbx_dtype xdtype, int32_t* yind, void* yval, int32_t* ny,
                                  void* stream) {
@@ -503,4 +537,35 @@ extern "C" cbx_status cbx_spmspv(const cbx_dcsc* A, const int32_t* xind, const v
   }
   return set_error(CBX_ERR_UNSUPPORTED, "cbx_spmspv: semiring %d with vector dtype %d is not provided", (int)semiring,
                    (int)xdtype);
+}
+
+extern "C" cbx_status cbx_spmv_dense(const cbx_dcsc* A, const void* x, void* y, cbx_semiring semiring,
+                                     cbx_dtype xdtype, void* stream) {
+  CBX_RETURN_IF(!A || (A->n > 0 && !x) || (A->m > 0 && !y), CBX_ERR_INVALID, "cbx_spmv_dense: null argument");
+  CBX_RETURN_IF(A->location != CBX_DEVICE, CBX_ERR_INVALID, "cbx_spmv_dense: expected a device matrix");
+  CBX_RETURN_IF(A->m >= (1ll << 31) || A->n >= (1ll << 31), CBX_ERR_LIMIT,
+                "cbx_spmv_dense: dimensions must be below 2^31");
+  const bool boolA = A->dtype == CBX_BOOL;
+  CBX_RETURN_IF(!boolA && A->dtype != xdtype, CBX_ERR_UNSUPPORTED,
+                "cbx_spmv_dense: A must be boolean or share the vectors' dtype");
+  cbx_status st = cbx_dcsc_check(A);  // the kernel indexes y by A's rows and x by its columns
+  if (st != CBX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  switch (semiring * 8 + xdtype) {
+#define CBX_SV(SRK, SRE, T, DT) \
+  case SRE * 8 + DT:            \
+    return spmv_dense_typed<SRK, T>(A, boolA, (const T*)x, (T*)y, s);
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, int64_t, CBX_I64)
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, double, CBX_F64)
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, float, CBX_F32)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, int64_t, CBX_I64)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, double, CBX_F64)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, float, CBX_F32)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, int64_t, CBX_I64)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, double, CBX_F64)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, float, CBX_F32)
+#undef CBX_SV
+  }
+  return set_error(CBX_ERR_UNSUPPORTED, "cbx_spmv_dense: semiring %d with vector dtype %d is not provided",
+                   (int)semiring, (int)xdtype);
 }

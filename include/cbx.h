@@ -286,6 +286,16 @@ cbx_status cbx_spmspv(const cbx_dcsc* A, const int32_t* xind, const void* xval, 
 
 /* -------------------------------------------------------------------- BFS */
 
+/* y = A*x with a dense x: dcsc_gespmv<SR>(A, x, y) (Friends.h:63-77), y[i] =
+ * add(y[i], multiply(A(i,j), x[j])) for every stored entry, accumulated into
+ * y in place (initialise it, e.g. to the semiring's identity).  x: device
+ * array of A->n values, y: device array of A->m values, both of xdtype; A
+ * shares that dtype or is boolean (then x[j] itself is added).  Integer
+ * semirings are exact; floating-point sums are formed in an unspecified
+ * order. */
+cbx_status cbx_spmv_dense(const cbx_dcsc* A, const void* x, void* y, cbx_semiring semiring, cbx_dtype xdtype,
+                          void* stream);
+
 /* Top-down BFS of TopDownBFS.cpp:442-448 on one GPU:
  *   fringe.setNumToInd(); fringe = SpMV<SelectMaxSRing<bool,int64>>(A, fringe);
  *   fringe = EWiseMult(fringe, parents, true, -1); parents.Set(fringe);

@@ -64,6 +64,8 @@ def _load_oracle() -> C.CDLL:
         L.orc_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc),
                                     C.c_void_p]
         L.orc_mcl_prune.restype = C.c_int32
+        L.orc_spmv_dense.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
+        L.orc_spmv_dense.restype = C.c_int32
         L.orc_galerkin_direct.argtypes = [C.c_int32, P(cbx_dcsc)]
         L.orc_galerkin_direct.restype = C.c_int32
         L.orc_max_threads.restype = C.c_int32
@@ -94,6 +96,8 @@ def _load_ref() -> C.CDLL:
             getattr(L, nm).restype = C.c_int32
         L.ref_bfs.argtypes = [P(cbx_dcsc), C.c_int64, C.c_void_p]
         L.ref_bfs.restype = C.c_int32
+        L.ref_spmv_dense.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32]
+        L.ref_spmv_dense.restype = C.c_int32
         L.ref_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc)]
         L.ref_mcl_prune.restype = C.c_int32
         L.ref_free.argtypes = [P(cbx_dcsc)]
@@ -208,6 +212,33 @@ def mcl_prune(A: Dcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, 
         raise RuntimeError(f"orc_mcl_prune failed with status {st}")
     X = Dcsc.from_struct(out, L.orc_free)
     return (X, cut) if return_cut else X
+
+
+_VDT = {np.dtype(np.int64): 0, np.dtype(np.float64): 1, np.dtype(np.float32): 2}
+
+
+def spmv_dense(A: Dcsc, x: np.ndarray, y: np.ndarray, semiring: str = "plus_times") -> np.ndarray:
+    """dcsc_gespmv restated: returns y accumulated with A*x (y is copied, not modified)."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    x = np.ascontiguousarray(x)
+    out = np.array(y, dtype=x.dtype, copy=True)
+    st = L.orc_spmv_dense(C.byref(sa), x.ctypes.data, out.ctypes.data, SEMIRINGS[semiring], _VDT[x.dtype])
+    if st != 0:
+        raise RuntimeError(f"orc_spmv_dense failed with status {st}")
+    return out
+
+
+def ref_spmv_dense(A: Dcsc, x: np.ndarray, y: np.ndarray, semiring: str = "plus_times") -> np.ndarray:
+    """The genuine dcsc_gespmv (only where oracle/_ref was built)."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    x = np.ascontiguousarray(x)
+    out = np.array(y, dtype=x.dtype, copy=True)
+    st = L.ref_spmv_dense(C.byref(sa), x.ctypes.data, out.ctypes.data, SEMIRINGS[semiring])
+    if st != 0:
+        raise RuntimeError(f"ref_spmv_dense failed with status {st}")
+    return out
 
 
 def galerkin_direct(g: int) -> Dcsc:

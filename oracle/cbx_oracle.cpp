@@ -633,6 +633,37 @@ int32_t orc_galerkin_direct(int32_t g, cbx_dcsc* out) {
   return CBX_OK;
 }
 
+// dcsc_gespmv (Friends.h:63-77): for every stored column in order, every entry
+// in order, y[row] = add(y[row], multiply(value, x[col])); a boolean A uses
+// x[col] itself (SelectMaxSRing<bool,T>::multiply of true and x).
+int32_t orc_spmv_dense(const cbx_dcsc* A, const void* x, void* y, int32_t semiring, int32_t xdtype) {
+  auto run = [&](auto* tag, auto sr) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    using SR = decltype(sr);
+    const T* xv = (const T*)x;
+    T* yv = (T*)y;
+    for (int64_t q = 0; q < A->nzc; ++q) {
+      const T xj = xv[A->jc[q]];
+      for (int64_t e = A->cp[q]; e < A->cp[q + 1]; ++e) {
+        const T prod = A->dtype == CBX_BOOL ? xj : SR::multiply(((const T*)A->val)[e], xj);
+        yv[A->ir[e]] = SR::add(yv[A->ir[e]], prod);
+      }
+    }
+  };
+  switch (semiring * 8 + xdtype) {
+    case CBX_SR_PLUS_TIMES * 8 + CBX_I64: run((int64_t*)nullptr, PlusTimes<int64_t>()); return 0;
+    case CBX_SR_PLUS_TIMES * 8 + CBX_F64: run((double*)nullptr, PlusTimes<double>()); return 0;
+    case CBX_SR_PLUS_TIMES * 8 + CBX_F32: run((float*)nullptr, PlusTimes<float>()); return 0;
+    case CBX_SR_MIN_PLUS * 8 + CBX_I64: run((int64_t*)nullptr, MinPlus<int64_t>()); return 0;
+    case CBX_SR_MIN_PLUS * 8 + CBX_F64: run((double*)nullptr, MinPlus<double>()); return 0;
+    case CBX_SR_MIN_PLUS * 8 + CBX_F32: run((float*)nullptr, MinPlus<float>()); return 0;
+    case CBX_SR_SELECT_MAX * 8 + CBX_I64: run((int64_t*)nullptr, SelectMax<int64_t>()); return 0;
+    case CBX_SR_SELECT_MAX * 8 + CBX_F64: run((double*)nullptr, SelectMax<double>()); return 0;
+    case CBX_SR_SELECT_MAX * 8 + CBX_F32: run((float*)nullptr, SelectMax<float>()); return 0;
+  }
+  return CBX_ERR_UNSUPPORTED;
+}
+
 int32_t orc_bfs(const cbx_dcsc* A, int64_t root, int64_t* parents) {
   const int64_t n = A->n;
   std::fill(parents, parents + n, (int64_t)-1);

@@ -272,6 +272,35 @@ int32_t ref_bfs(const cbx_dcsc* A, int64_t root, int64_t* parents) {
   return it;
 }
 
+// dcsc_gespmv<SR>(A, x, y) (Friends.h:63-77), the genuine template.
+int32_t ref_spmv_dense(const cbx_dcsc* A, const void* x, void* y, int32_t semiring) {
+  ensure_mpi();
+  if (A->dtype == CBX_BOOL) {  // SelectMax over a boolean matrix, int64 vectors (the BFS form)
+    auto* M = make_mat<bool, uint8_t>(*A);
+    dcsc_gespmv<SelectMaxSRing<bool, int64_t>>(*M, (const int64_t*)x, (int64_t*)y);
+    delete M;
+    return 0;
+  }
+  if (A->dtype == CBX_I64) {
+    auto* M = make_mat<int64_t, int64_t>(*A);
+    if (semiring == CBX_SR_PLUS_TIMES)
+      dcsc_gespmv<PlusTimesSRing<int64_t, int64_t>>(*M, (const int64_t*)x, (int64_t*)y);
+    else if (semiring == CBX_SR_MIN_PLUS)
+      dcsc_gespmv<MinPlusSRing<int64_t, int64_t>>(*M, (const int64_t*)x, (int64_t*)y);
+    else
+      dcsc_gespmv<SelectMaxSRing<int64_t, int64_t>>(*M, (const int64_t*)x, (int64_t*)y);
+    delete M;
+    return 0;
+  }
+  if (A->dtype == CBX_F64 && semiring == CBX_SR_PLUS_TIMES) {
+    auto* M = make_mat<double, double>(*A);
+    dcsc_gespmv<PlusTimesSRing<double, double>>(*M, (const double*)x, (double*)y);
+    delete M;
+    return 0;
+  }
+  return CBX_ERR_UNSUPPORTED;
+}
+
 void ref_free(cbx_dcsc* C) {
   std::free(C->cp);
   std::free(C->jc);

@@ -79,3 +79,29 @@ def test_bfs_rmat_s20_ef16_config():
         got, st = plan.run(int(root))
         want, it = oracle.bfs(A, int(root))
         assert st["levels"] == it and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("semiring", ["plus_times", "min_plus", "select_max"])
+@pytest.mark.parametrize("dtype", ["int64", "float64", "float32", "bool"])
+def test_spmv_dense_matches_oracle(semiring, dtype):
+    """dcsc_gespmv on the device: integer semirings bit-exact, FP within 1e-12
+    (FP64) / 1e-5 (FP32) relative (atomic accumulation order)."""
+    rng = np.random.default_rng(8)
+    A = cb.gen_rmat(14, 8, seed=21)
+    xd = {"int64": np.int64, "float64": np.float64, "float32": np.float32, "bool": np.int64}[dtype]
+    if dtype == "bool":
+        A = A.astype("bool")
+    elif dtype == "int64":
+        A.val = rng.integers(-9, 10, A.nnz).astype(np.int64)
+    else:
+        A = A.astype(dtype)
+        A.val = rng.uniform(0.5, 2.0, A.nnz).astype(xd)
+    x = (rng.uniform(0.5, 2.0, A.n) if xd != np.int64 else rng.integers(-20, 20, A.n)).astype(xd)
+    y = (rng.uniform(0.5, 2.0, A.m) if xd != np.int64 else rng.integers(-5, 5, A.m)).astype(xd)
+    got = cb.spmv_dense(A.to_device(), x, y, semiring)
+    want = oracle.spmv_dense(A, x, y, semiring)
+    if xd == np.int64 or semiring != "plus_times":
+        assert np.array_equal(got, want)
+    else:
+        tol = 1e-12 if xd == np.float64 else 1e-5
+        assert np.allclose(got, want, rtol=tol, atol=0)
