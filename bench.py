@@ -371,6 +371,7 @@ def main() -> int:
     }
     dA.release()
     cb.release_cached_memory()
+    cb.set_config(**default_cfg)  # the other workloads run with the library defaults
     if not args.no_extra:
         for name, fn in (("bfs", lambda: bench_bfs(cb, args.scale, args.edgefactor)),
                          ("mcl", lambda: bench_mcl(cb)), ("galerkin", lambda: bench_galerkin(cb))):

@@ -92,13 +92,25 @@ __global__ __launch_bounds__(256) void k_mcl_stats(const int64_t* __restrict__ c
   const int64_t b = cp[q], e = cp[q + 1];
   long long nP = 0, nK = 0;
   double sP = 0;
-  for (int64_t p = b + lane; p < e; p += kWave) {
-    const T v = val[p];
-    if (!(v <= thr)) {
-      ++nP;
-      sP += (double)v;
+  // kUnroll independent loads in flight per lane (columns are long and the
+  // wave's loads would otherwise wait on each other)
+  constexpr int kUnroll = 8;
+  for (int64_t p0 = b; p0 < e; p0 += kWave * kUnroll) {
+    T v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t p = p0 + u * kWave + lane;
+      v[u] = p < e ? val[p] : thr;  // thr: counted as kept, not as above the threshold
     }
-    nK += !(v < thr);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (p0 + u * kWave + lane >= e) continue;
+      if (!(v[u] <= thr)) {
+        ++nP;
+        sP += (double)v[u];
+      }
+      nK += !(v[u] < thr);
+    }
   }
   nP = wave_sum_ll(nP);
   nK = wave_sum_ll(nK);
@@ -288,21 +300,27 @@ __global__ __launch_bounds__(256) void k_mcl_write(const int64_t* __restrict__ c
   const T c = cut[q];
   int64_t w = off[q];
   const int64_t b = cp[q], e = cp[q + 1];
-  for (int64_t base = b; base < e; base += kWave) {
-    const int64_t p = base + lane;
-    T v = T(0);
-    bool keep = false;
-    if (p < e) {
-      v = val[p];
-      keep = !(v < c);
+  constexpr int kUnroll = 8;  // loads in flight per lane
+  const int64_t end_w = w + cnt[q];
+  for (int64_t base0 = b; base0 < e && w < end_w; base0 += kWave * kUnroll) {
+    T v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t p = base0 + u * kWave + lane;
+      v[u] = p < e ? val[p] : T(0);
     }
-    const unsigned long long m = __ballot(keep);
-    if (keep) {
-      const int64_t at = w + __popcll(m & ((1ull << lane) - 1ull));
-      oir[at] = ir[p];
-      oval[at] = v;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int64_t p = base0 + u * kWave + lane;
+      const bool keep = p < e && !(v[u] < c);
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const int64_t at = w + __popcll(m & ((1ull << lane) - 1ull));
+        oir[at] = ir[p];
+        oval[at] = v[u];
+      }
+      w += __popcll(m);
     }
-    w += __popcll(m);
   }
   if (lane == 0) {
     ojc[cpos[q]] = jc[q];
