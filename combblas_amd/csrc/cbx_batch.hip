@@ -413,7 +413,9 @@ static cbx_status galerkin_batched(int64_t count, const cbx_dcsc* A, const cbx_d
   cbx_spgemm_stats one;
   CBX_TRY(spgemm_dispatch(&PA.x, &PT.x, CBX_SR_PLUS_TIMES, CBX_SPGEMM_TWO_PHASE, cfg, &PAT.x, &one, s));
   add_stats(st, one);
-  CBX_TRY(spgemm_dispatch(&PS.x, &PAT.x, CBX_SR_PLUS_TIMES, CBX_SPGEMM_TWO_PHASE, cfg, &PSAT.x, &one, s));
+  SpgemmConfig cfg2 = cfg;  // S (a transpose) and A*T (a product) come from this library
+  cfg2.trusted_a = cfg2.trusted_b = true;
+  CBX_TRY(spgemm_dispatch(&PS.x, &PAT.x, CBX_SR_PLUS_TIMES, CBX_SPGEMM_TWO_PHASE, cfg2, &PSAT.x, &one, s));
   add_stats(st, one);
   std::vector<int64_t> bound(count + 1, 0), roff(count, 0), ms(count), ns(count);
   for (int64_t i = 0; i < count; ++i) {

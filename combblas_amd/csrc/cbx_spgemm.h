@@ -26,6 +26,9 @@ struct SpgemmConfig {
   // the output capacity over the predicted total.
   const float* predictor = nullptr;
   float learned_target = 0.75f;
+  // Operands the library produced itself (a transpose, an earlier product)
+  // are valid by construction and skip the device validation pass.
+  bool trusted_a = false, trusted_b = false;
   float capacity_margin = 0.15f;
 };
 

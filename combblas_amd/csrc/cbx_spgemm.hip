@@ -333,6 +333,113 @@ __device__ __forceinline__ X shfl_x(X v, int src) {
   else return __shfl(v, src, kWave);
 }
 
+// Sort, reduce and emit the first NR registers (NR * 64 elements) of a tiny
+// column; elements beyond them are empty (row kNone).
+template <int K, class T, bool SYM, int NR>
+__device__ __forceinline__ void tiny_finish(int32_t (&key)[kTinyMax / kWave],
+                                            typename Sem<K, T>::Acc (&val)[kTinyMax / kWave], int lane, int col,
+                                            int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpC,
+                                            int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int32_t kNone = 0x7fffffff;
+  // bitonic sort of NR*64 elements, ascending by row
+#pragma unroll
+  for (int k = 2; k <= NR * kWave; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= kWave) {
+        const int du = j / kWave;
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+          const int u2 = u ^ du;
+          if (u2 > u) {
+            const bool asc = (((u * kWave + lane) & k) == 0);
+            if ((key[u] > key[u2]) == asc && key[u] != key[u2]) {
+              const int32_t tk = key[u];
+              key[u] = key[u2];
+              key[u2] = tk;
+              if constexpr (!SYM) {
+                const Acc tv = val[u];
+                val[u] = val[u2];
+                val[u2] = tv;
+              }
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+          const int e = u * kWave + lane;
+          const bool asc = ((e & k) == 0);
+          const bool lower = (lane & j) == 0;
+          const int32_t pk = __shfl_xor(key[u], j, kWave);
+          Acc pv{};
+          if constexpr (!SYM) pv = __shfl_xor(val[u], j, kWave);
+          const bool take = lower == asc ? pk < key[u] : pk > key[u];
+          if (take) {
+            key[u] = pk;
+            if constexpr (!SYM) val[u] = pv;
+          }
+        }
+      }
+    }
+  }
+  // run heads (first element of each distinct row)
+  bool head[NR];
+  unsigned long long hm[NR];
+  int count = 0;
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    int32_t prev = __shfl_up(key[u], 1, kWave);
+    const int32_t carry = u > 0 ? __shfl(key[u > 0 ? u - 1 : 0], kWave - 1, kWave) : kNone;
+    if (lane == 0) prev = u > 0 ? carry : -1;
+    head[u] = key[u] != kNone && key[u] != prev;
+    hm[u] = __ballot(head[u]);
+    count += __popcll(hm[u]);
+  }
+  if constexpr (SYM) {
+    if (lane == 0) nnzcol[col] = count;
+  } else {
+    const int64_t have = nnzcol[col];  // the column's allocation from the symbolic pass
+    if (lane == 0 && count != have) atomicOr(err, 8);
+    const int64_t out = cpC[col];
+    Acc carry = SR::ident();
+    bool open = false;  // a run continues from the previous register
+    int before = 0;     // heads in earlier registers
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      // segmented inclusive scan in element order
+      Acc x = val[u];
+      bool f = head[u];
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const Acc y = __shfl_up(x, d, kWave);
+        const bool g = __shfl_up((int)f, d, kWave) != 0;
+        if (lane >= d) {
+          if (!f) x = SR::add(y, x);
+          f = f || g;
+        }
+      }
+      if (!f && open) x = SR::add(carry, x);
+      // the last element of a run writes it at its rank
+      int32_t next = __shfl_down(key[u], 1, kWave);
+      const int32_t nfirst = u + 1 < NR ? __shfl(key[u + 1 < NR ? u + 1 : u], 0, kWave) : kNone;
+      if (lane == kWave - 1) next = nfirst;
+      if (key[u] != kNone && next != key[u]) {
+        const int rank = before + __popcll(hm[u] & ((2ull << lane) - 1ull)) - 1;
+        if (rank < have) {
+          irC[out + rank] = key[u];
+          valC[out + rank] = SR::out(x);
+        }
+      }
+      carry = __shfl(x, kWave - 1, kWave);
+      open = __shfl((int)(key[u] != kNone), kWave - 1, kWave) != 0;
+      before += __popcll(hm[u]);
+    }
+  }
+}
+
 template <int K, class T, bool SYM>
 __global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order, int64_t first, int64_t ncols,
                                               const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
@@ -389,102 +496,10 @@ __global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order,
       filled += total;
     }
     if (filled > kTinyMax && lane == 0) atomicOr(err, 4);
-    // bitonic sort of 256 elements, ascending by row
-#pragma unroll
-    for (int k = 2; k <= kTinyMax; k <<= 1) {
-#pragma unroll
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        if (j >= kWave) {
-          const int du = j / kWave;
-#pragma unroll
-          for (int u = 0; u < kE; ++u) {
-            const int u2 = u ^ du;
-            if (u2 > u) {
-              const bool asc = (((u * kWave + lane) & k) == 0);
-              if ((key[u] > key[u2]) == asc && key[u] != key[u2]) {
-                const int32_t tk = key[u];
-                key[u] = key[u2];
-                key[u2] = tk;
-                if constexpr (!SYM) {
-                  const Acc tv = val[u];
-                  val[u] = val[u2];
-                  val[u2] = tv;
-                }
-              }
-            }
-          }
-        } else {
-#pragma unroll
-          for (int u = 0; u < kE; ++u) {
-            const int e = u * kWave + lane;
-            const bool asc = ((e & k) == 0);
-            const bool lower = (lane & j) == 0;
-            const int32_t pk = __shfl_xor(key[u], j, kWave);
-            Acc pv{};
-            if constexpr (!SYM) pv = __shfl_xor(val[u], j, kWave);
-            const bool take = lower == asc ? pk < key[u] : pk > key[u];
-            if (take) {
-              key[u] = pk;
-              if constexpr (!SYM) val[u] = pv;
-            }
-          }
-        }
-      }
-    }
-    // run heads (first element of each distinct row)
-    bool head[kE];
-    unsigned long long hm[kE];
-    int count = 0;
-#pragma unroll
-    for (int u = 0; u < kE; ++u) {
-      int32_t prev = __shfl_up(key[u], 1, kWave);
-      const int32_t carry = u > 0 ? __shfl(key[u > 0 ? u - 1 : 0], kWave - 1, kWave) : kNone;
-      if (lane == 0) prev = u > 0 ? carry : -1;
-      head[u] = key[u] != kNone && key[u] != prev;
-      hm[u] = __ballot(head[u]);
-      count += __popcll(hm[u]);
-    }
-    if constexpr (SYM) {
-      if (lane == 0) nnzcol[col] = count;
-      continue;
-    } else {
-      const int64_t have = nnzcol[col];  // the column's allocation from the symbolic pass
-      if (lane == 0 && count != have) atomicOr(err, 8);
-      const int64_t out = cpC[col];
-      Acc carry = SR::ident();
-      bool open = false;  // a run continues from the previous register
-      int before = 0;     // heads in earlier registers
-#pragma unroll
-      for (int u = 0; u < kE; ++u) {
-        // segmented inclusive scan in element order
-        Acc x = val[u];
-        bool f = head[u];
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-          const Acc y = __shfl_up(x, d, kWave);
-          const bool g = __shfl_up((int)f, d, kWave) != 0;
-          if (lane >= d) {
-            if (!f) x = SR::add(y, x);
-            f = f || g;
-          }
-        }
-        if (!f && open) x = SR::add(carry, x);
-        // the last element of a run writes it at its rank
-        int32_t next = __shfl_down(key[u], 1, kWave);
-        const int32_t nfirst = u + 1 < kE ? __shfl(key[u + 1 < kE ? u + 1 : u], 0, kWave) : kNone;
-        if (lane == kWave - 1) next = nfirst;
-        if (key[u] != kNone && next != key[u]) {
-          const int rank = before + __popcll(hm[u] & ((2ull << lane) - 1ull)) - 1;
-          if (rank < have) {
-            irC[out + rank] = key[u];
-            valC[out + rank] = SR::out(x);
-          }
-        }
-        carry = __shfl(x, kWave - 1, kWave);
-        open = __shfl((int)(key[u] != kNone), kWave - 1, kWave) != 0;
-        before += __popcll(hm[u]);
-      }
-    }
+    // sort only as many registers as the column fills (wave-uniform)
+    if (filled <= kWave) tiny_finish<K, T, SYM, 1>(key, val, lane, col, nnzcol, cpC, irC, valC, err);
+    else if (filled <= 2 * kWave) tiny_finish<K, T, SYM, 2>(key, val, lane, col, nnzcol, cpC, irC, valC, err);
+    else tiny_finish<K, T, SYM, 4>(key, val, lane, col, nnzcol, cpC, irC, valC, err);
   }
 }
 
@@ -1787,8 +1802,10 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const T* valB = (const T*)B->val;
 
   // --- validate both operands before any indexed access
-  k_validate<<<wave_grid(nzcA), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
-  k_validate<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
+  if (!cfg.trusted_a)
+    k_validate<<<wave_grid(nzcA), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
+  if (!cfg.trusted_b)
+    k_validate<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
   {
     int herr = 0;
     CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
