@@ -818,6 +818,10 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
           if (bpre[mid] <= lim) lo = mid;
           else hi = mid - 1;
         }
+        // a unit whose first blocks alone fill half the value window stops
+        // at the window's rows, so the numeric pass handles it in one sweep
+        const int dl = kmax / kBR;
+        if (lo - start + 1 > dl && bpre[start + dl - 1] - before >= kmax / 2) lo = start + dl - 1;
         const int c = bpre[lo] - before;
         if (c > 0) {
           su[nu].col = col;
@@ -879,12 +883,17 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
   const int cnt = un.cnt;
   const int64_t out = cpC[col] + un.off;
   const int64_t b0 = cpB[col], b1 = cpB[col + 1];
-  const bool single_chunk = (b1 - b0) <= NT;
+  // A unit whose rows fit the value window (dense output) accumulates at
+  // vals[row - row0] in a single sweep; the others count first and
+  // accumulate at the rank in a second sweep.
+  const bool direct = (t_hi - t_lo) * kBR <= KMAX;
+  const bool single_chunk = !direct && (b1 - b0) <= NT;
   if (cnt > KMAX || t_hi - t_lo > RBLK) {  // units are cut to fit; guard anyway
     if (threadIdx.x == 0) atomicOr(err, 64);
     return;
   }
-  for (int i = threadIdx.x; i < cnt; i += NT) vals[i] = SR::ident();
+  const int ninit = direct ? (t_hi - t_lo) * kBR : cnt;
+  for (int i = threadIdx.x; i < ninit; i += NT) vals[i] = SR::ident();
   const int nw32 = (t_hi - t_lo) * kBRW;
   for (int i = threadIdx.x; i < nw32; i += NT) bits[i] = 0u;
 
@@ -953,13 +962,33 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
     }
     visit_staged<T, NT, U>(st, nseg, tot, set_bits, CR);
   } else {
+    auto acc_direct = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
+      int32_t r[U];
+      T a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        r[u] = -1;
+        if (idx[u] >= 0) {
+          r[u] = rowsA[idx[u]] - row0;
+          a[u] = valA[idx[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (r[u] >= 0) {
+          atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+          SR::acc(&vals[r[u]], SR::mul(a[u], bvs[u]));
+        }
+    };
+    if (direct) __syncthreads();  // the value window is initialised before anyone accumulates
     for (int64_t eb = b0; eb < b1; eb += NT) {
       int64_t base;
       int len;
       T bv;
       load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
       tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
-      visit_staged<T, NT, U>(st, nseg, tot, set_bits);
+      if (direct) visit_staged<T, NT, U>(st, nseg, tot, acc_direct);
+      else visit_staged<T, NT, U>(st, nseg, tot, set_bits);
     }
   }
   __syncthreads();
@@ -985,6 +1014,17 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
     if (threadIdx.x == 0 && total != cnt) atomicOr(err, 128);
   }
   __syncthreads();
+  if (direct) {  // values already sit at their rows: one thread per row, stores stay coalesced
+    for (int r = threadIdx.x; r < ninit; r += NT) {
+      if (!((bits[r >> 5] >> (r & 31)) & 1u)) continue;
+      const int q = rank_of(r);
+      if (q < cnt) {
+        irC[out + q] = (int64_t)row0 + r;
+        valC[out + q] = SR::out(vals[r]);
+      }
+    }
+    return;
+  }
   // sweep 2: accumulate values at their ranks
   if (single_chunk) {
 #pragma unroll
