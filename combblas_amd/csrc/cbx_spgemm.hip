@@ -1578,7 +1578,8 @@ constexpr int kSymNT = 512;
 constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass (2^20 rows)
 // unit tier
 constexpr int kSymUnitNT = 1024;
-constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 32;    // <= 4096 outputs, <= 128K rows
+// light units span at most 16 blocks: ~53 KB of LDS, three workgroups per CU
+constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 16;    // <= 4096 outputs, <= 64K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
 // learned units: one workgroup shape for all (512 threads, 4096-entry value
 // window, 32 row blocks), cut to target * 4096 predicted outputs
