@@ -35,7 +35,7 @@ GRID = {
     "hash_max_flop": [1024, 2048, 4096],
     "hash_mid_flop": [512, 1024],
     "tiny_flop": [128, 256],
-    "heavy_nnz": [65536, 131072],
+    "heavy_nnz": [65536, 131072, 524288, 2097152],
 }
 LEARNED_GRID = {"learned_target": [0.6, 0.75, 0.9]}
 
