@@ -35,9 +35,18 @@ GRID = {
     "hash_max_flop": [1024, 2048, 4096],
     "hash_mid_flop": [512, 1024],
     "tiny_flop": [128, 256],
-    "heavy_nnz": [65536, 131072, 524288, 2097152],
+    "unit_split_nnz": [65536, 131072, 262144, 524288],
 }
-LEARNED_GRID = {"learned_target": [0.6, 0.75, 0.9]}
+# the learned single pass adds its own knobs; the hash tiers are swept more
+# coarsely there to keep the product of the grids small
+LEARNED_GRID = {
+    "hash_max_flop": [1024, 2048],
+    "hash_mid_flop": [1024],
+    "tiny_flop": [256],
+    "unit_split_nnz": [131072, 262144, 524288],
+    "learned_target": [0.75, 0.9],
+    "heavy_nnz": [32768, 65536, 131072],
+}
 
 
 def device_name() -> str:
@@ -69,7 +78,7 @@ def tune(name: str, reps: int = 2, log=print) -> dict:
         from combblas_amd import predictor
         cb.set_predictor(predictor.load())
     base = cb.get_config()
-    grid = dict(GRID, **(LEARNED_GRID if mode == "learned" else {}))
+    grid = LEARNED_GRID if mode == "learned" else GRID
     keys = list(grid)
     results = []
     ref_nnz = None

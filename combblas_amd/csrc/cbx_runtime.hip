@@ -309,7 +309,7 @@ extern "C" cbx_status cbx_spgemm_set_predictor(const float* params) {
 
 // ------------------------------------------------------------- tunables
 static std::mutex g_config_mu;
-static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 0.75f, 0.15f};
+static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f};
 
 SpgemmConfig cbx::current_config() {
   SpgemmConfig c;
@@ -319,6 +319,7 @@ SpgemmConfig cbx::current_config() {
     c.hash_mid_flop = g_config.hash_mid_flop;
     c.tiny_flop = g_config.tiny_flop;
     c.heavy_nnz = g_config.heavy_nnz;
+    c.unit_split_nnz = g_config.unit_split_nnz;
     c.learned_target = g_config.learned_target;
     c.capacity_margin = g_config.capacity_margin;
   }
@@ -342,6 +343,8 @@ extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
   CBX_RETURN_IF(cfg->hash_max_flop < cfg->hash_mid_flop || cfg->hash_max_flop > 4096, CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: hash_max_flop must be in [hash_mid_flop, 4096]");
   CBX_RETURN_IF(cfg->heavy_nnz < 4096, CBX_ERR_INVALID, "cbx_spgemm_set_config: heavy_nnz must be >= 4096");
+  CBX_RETURN_IF(cfg->unit_split_nnz < 4096, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: unit_split_nnz must be >= 4096");
   CBX_RETURN_IF(!(cfg->learned_target > 0.05f && cfg->learned_target <= 4.0f), CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: learned_target must be in (0.05, 4]");
   CBX_RETURN_IF(!(cfg->capacity_margin >= 0.0f && cfg->capacity_margin <= 4.0f), CBX_ERR_INVALID,

@@ -19,8 +19,12 @@ struct SpgemmConfig {
   // take one wave each (the tiny tier).
   int64_t hash_mid_flop = 1024;
   int64_t tiny_flop = 256;
-  // Unit tier: columns whose output exceeds heavy_nnz use the large units.
+  // Learned mode: columns predicted to exceed heavy_nnz outputs are computed
+  // exactly (symbolic, then two-phase units).
   int64_t heavy_nnz = 65536;
+  // Unit tier: columns whose exact output exceeds unit_split_nnz use the
+  // large (16384-output) units.
+  int64_t unit_split_nnz = 131072;
   // Learned mode: device parameters of the nnz predictor (null = not loaded),
   // unit size target as a fraction of the value window, and the headroom of
   // the output capacity over the predicted total.

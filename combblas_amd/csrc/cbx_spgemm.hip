@@ -1204,8 +1204,8 @@ __device__ __forceinline__ int cut_units(int col, float pred, float flop, int lo
 
 __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, int64_t small_max,
                                 const float* __restrict__ logpred, const int32_t* __restrict__ blo,
-                                const int32_t* __restrict__ bhi, const float* __restrict__ w, float target_heavy,
-                                float target_light, float heavy_pred, float fcap, int rblk, int64_t m,
+                                const int32_t* __restrict__ bhi, const float* __restrict__ w, float target,
+                                float heavy_pred, float fcap, int rblk, int64_t m,
                                 const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
                                 UnitL* __restrict__ units, float* __restrict__ pred_nnz,
                                 uint8_t* __restrict__ heavy_flag) {
@@ -1230,7 +1230,6 @@ __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, 
     }
     return;
   }
-  const float target = target_light;
   if (!units) {
     ucount[j] = cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, nullptr);
     pred_nnz[j] = pred;
@@ -1651,11 +1650,10 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   k_col_blocks<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, blo, bhi);
   // units in column order: count, scan, write
   const float target = cfg.learned_target * (float)kLearnK;
-  const float target_light = target;
   const float heavy_pred = (float)cfg.heavy_nnz;
   const float fcap = (float)kLearnFlopCap;
   k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                  target_light, heavy_pred, fcap, kLearnRB, m, nullptr, ucount,
+                                                  heavy_pred, fcap, kLearnRB, m, nullptr, ucount,
                                                   nullptr, pred_nnz, heavy_flag);
   CBX_HIP(hipMemsetAsync(ucount + nzcB, 0, sizeof(int64_t), s));
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ucount, ubase, (int)(nzcB + 1), s));
@@ -1692,7 +1690,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
     CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
     CBX_HIP(hipMemsetAsync(hucnt, 0, 2 * sizeof(int), s));
     k_sym_units<kSymUnitNT, 4><<<(unsigned)nheavy, kSymUnitNT, 0, s>>>(
-        hcols, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.heavy_nnz, nnzcol,
+        hcols, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
         units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, err);
     CBX_HIP(hipGetLastError());
   }
@@ -1712,7 +1710,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_status, std::max<int64_t>(nunits, 1), s, &status));
   if (nunits)
     k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                    target_light, heavy_pred, fcap, kLearnRB, m, ubase, nullptr,
+                                                    heavy_pred, fcap, kLearnRB, m, ubase, nullptr,
                                                     units, nullptr, nullptr);
   // output capacity from the predictions (plus the exact hash-tier sizes)
   const int64_t cap = hash_total + (int64_t)(pred_sum * (1.0 + cfg.capacity_margin)) + (1 << 20);
@@ -1881,8 +1879,9 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
                                                        s));
   CBX_HIP(hipcub::DeviceReduce::Sum(tmp, tmp_bytes, flop, bounds + 5, (int)nzcB, s));
 
-  // class thresholds: units > small_max ; hash8k > mid ; hash2k > tiny ; tiny (one wave) > 0 ; heavy > heavy_nnz
-  int64_t hthr[5] = {cfg.small_flop_max, cfg.hash_mid_flop, cfg.tiny_flop, 0, cfg.heavy_nnz};
+  // class thresholds: units > small_max ; hash8k > mid ; hash2k > tiny ; tiny (one wave) > 0 ;
+  // large units possible > unit_split_nnz
+  int64_t hthr[5] = {cfg.small_flop_max, cfg.hash_mid_flop, cfg.tiny_flop, 0, cfg.unit_split_nnz};
   CBX_HIP(hipMemcpyAsync(thr, hthr, sizeof(hthr), hipMemcpyHostToDevice, s));
   k_class_bounds<<<1, 64, 0, s>>>(flop_s, nzcB, thr, 5, bounds);
   int64_t hb[6];
@@ -1890,7 +1889,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_HIP(hipMemsetAsync(nnzcol, 0, sizeof(int64_t) * (nzcB + 1), s));
   CBX_HIP(hipStreamSynchronize(s));
   const int64_t n_dense = hb[0], n_h8 = hb[1] - hb[0], n_h2 = hb[2] - hb[1], n_h256 = hb[3] - hb[2];
-  const int64_t n_heavy_max = hb[4];  // columns whose flops (an upper bound of nnz) exceed heavy_nnz
+  const int64_t n_heavy_max = hb[4];  // columns whose flops (an upper bound of nnz) exceed unit_split_nnz
   const int64_t total_flop = hb[5];
   // one workgroup per column (hash tiers, symbolic units): a dispatch holds
   // fewer than 2^32 work-items
@@ -1948,7 +1947,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
     CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
     k_sym_units<kSymUnitNT, 4><<<(unsigned)n_dense, kSymUnitNT, 0, s>>>(
-        order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.heavy_nnz, nnzcol,
+        order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
         units_l, ucnt, (int)cap_l, units_h, ucnt + 1, (int)cap_h, err);
   } else if (n_dense) {
     k_sym_dense<kSymNT, kSymRBW><<<(unsigned)n_dense, kSymNT, 0, s>>>(order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, G,

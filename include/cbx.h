@@ -173,15 +173,18 @@ enum {
  * sizes the reference hard-codes: the heap/hash switch cr < 2 at
  * mtSpGEMM.h:311 and minHashTableSize / hashScale at :366-367).  Picked per
  * workload by combblas_amd/autotune.py; defaults {4096, 1024, 256, 65536,
- * 0.75, 0.15}. */
+ * 131072, 0.75, 0.15}. */
 typedef struct cbx_spgemm_config {
   int64_t hash_max_flop;   /* columns with flop_j <= this use the LDS hash tier (<= 4096) */
   int64_t hash_mid_flop;   /* ... with flop_j > this use the 8192-slot table, else the 2048-slot one (<= 1024) */
   int64_t tiny_flop;       /* columns with flop_j <= this take one wave each, sorted in registers (<= 256) */
-  int64_t heavy_nnz;       /* unit tier: columns with more outputs use the 16384-output units (>= 4096) */
-  float learned_target;    /* learned mode: predicted outputs per unit as a fraction of the unit size (16384 for
-                              columns predicted above heavy_nnz, else 4096); above 1 units are cut larger than
-                              their value window and re-swept in windows (exact, slower) */
+  int64_t heavy_nnz;       /* learned mode: columns predicted to hold more outputs are computed exactly (symbolic
+                              pass, then two-phase units) after the single pass (>= 4096) */
+  int64_t unit_split_nnz;  /* unit tier: columns with more (exact) outputs use the 16384-output units, the
+                              others the 4096-output ones (>= 4096) */
+  float learned_target;    /* learned mode: predicted outputs per unit as a fraction of the 4096-output value
+                              window; above 1 units are cut larger than their window and re-swept in windows
+                              (exact, slower) */
   float capacity_margin;   /* learned mode: output capacity headroom over the predicted total */
 } cbx_spgemm_config;
 
