@@ -43,9 +43,9 @@ LEARNED_GRID = {
     "hash_max_flop": [1024, 2048],
     "hash_mid_flop": [1024],
     "tiny_flop": [256],
-    "unit_split_nnz": [131072, 262144, 524288],
+    "unit_split_nnz": [131072, 262144],
     "learned_target": [0.75, 0.9],
-    "heavy_nnz": [32768, 65536, 131072],
+    "heavy_nnz": [8192, 16384, 32768, 65536],
 }
 
 
