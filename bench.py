@@ -349,7 +349,7 @@ def main() -> int:
             "units": units, "overflow_units": ls[-1]["overflow_cols"],
             "overflow_pct": round(100.0 * ls[-1]["overflow_cols"] / max(units, 1), 3),
             "fallback_steps": int(sum(x["learned_fallback"] for x in ls)),
-            "predicted_vs_actual_unit_tier_nnz": round(ls[-1]["predicted_nnz"] / max(ls[-1]["nnz_main"], 1), 4),
+            "predicted_vs_actual_nnz": round(ls[-1]["predicted_nnz"] / max(ls[-1]["actual_nnz_predicted"], 1), 4),
             "non_learned_baseline": baseline}
 
     out = {

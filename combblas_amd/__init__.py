@@ -57,7 +57,8 @@ class cbx_dcsc(C.Structure):
 class cbx_spgemm_stats(C.Structure):
     _fields_ = [
         ("flops", C.c_int64), ("nnz", C.c_int64), ("nzc", C.c_int64),
-        ("overflow_cols", C.c_int64), ("predicted_nnz", C.c_int64), ("learned_fallback", C.c_int64),
+        ("overflow_cols", C.c_int64), ("predicted_nnz", C.c_int64), ("actual_nnz_predicted", C.c_int64),
+        ("learned_fallback", C.c_int64),
         ("units_light", C.c_int64), ("units_heavy", C.c_int64), ("nnz_main", C.c_int64),
         ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_alloc", C.c_double),
         ("ms_numeric", C.c_double), ("ms_main_kernel", C.c_double), ("ms_total", C.c_double),

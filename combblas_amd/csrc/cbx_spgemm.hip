@@ -1593,6 +1593,7 @@ inline unsigned tiny_grid(int64_t ncols) {  // 4 columns per workgroup, grid-str
 
 struct LearnedInfo {
   int64_t nnzC = 0, units = 0, overflow_units = 0, hash_total = 0, heavy_units = 0;
+  int64_t exact_total = 0;  // outputs of the predicted-heavy columns (computed exactly)
   double pred_sum = 0;
   float kernel_ms = 0;  // the single-pass numeric kernel alone
   bool over_capacity = false;
@@ -1706,6 +1707,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   info.units = nunits;
   info.pred_sum = pred_sum;
   info.hash_total = hash_only;
+  info.exact_total = hash_total - hash_only;
   CBX_TRY(dalloc(b_units, std::max<int64_t>(nunits, 1), s, &units));
   CBX_TRY(dalloc(b_status, std::max<int64_t>(nunits, 1), s, &status));
   if (nunits)
@@ -2045,6 +2047,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     stats->nnz_main = nnz_main;
     stats->overflow_cols = linfo.overflow_units;
     stats->predicted_nnz = (int64_t)linfo.pred_sum;
+    stats->actual_nnz_predicted = learned_done ? linfo.nnzC - linfo.hash_total - linfo.exact_total : 0;
     stats->learned_fallback = learned && !learned_done;
   }
   return CBX_OK;

@@ -200,7 +200,9 @@ typedef struct cbx_spgemm_stats {
   int64_t nzc;            /* non-empty output columns */
   int64_t overflow_cols;  /* learned mode: units whose exact size exceeded the value window and were
                              re-run exactly in windows of ranks (caught by a device counter) */
-  int64_t predicted_nnz;  /* learned mode: sum of the predicted sizes of the unit-tier columns */
+  int64_t predicted_nnz;  /* learned mode: sum of the predicted sizes of the columns the single pass
+                             sized by prediction */
+  int64_t actual_nnz_predicted; /* learned mode: their exact sizes (the product's outputs in those columns) */
   int64_t learned_fallback; /* learned mode: 1 if the predicted output capacity was exceeded and the
                                product was recomputed by the exact two-phase path */
   int64_t units_light;    /* row-range units of the numeric pass (small/large) */

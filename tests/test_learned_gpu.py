@@ -110,6 +110,11 @@ def test_learned_trained_model_s16():
     assert st["learned_fallback"] == 0
     want, _ = oracle.spgemm(A, A)
     bit_exact(got, want)
+    # the columns the single pass sized by prediction: their exact total is a
+    # part of the product, and the trained network's total lands near it
+    assert st["units_light"] > 0
+    assert 0 < st["actual_nnz_predicted"] <= st["nnz"]
+    assert 0.5 < st["predicted_nnz"] / st["actual_nnz_predicted"] < 2.0
 
 
 @pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
