@@ -227,6 +227,34 @@ def bench_galerkin(cb, count: int = 256, lo: int = 16, hi: int = 96, reps: int =
             "rowsums_and_symmetric_pattern_all": rowsums_ok}
 
 
+def bench_spmv(cb, scale: int, ef: int, reps: int = 20) -> dict:
+    """dcsc_gespmv (y += A*x, dense x) on the SpGEMM workload's matrix, PlusTimes<int64>, x and y
+    resident on the device; checked against numpy outside the timed region."""
+    A = cb.gen_rmat(scale, ef)
+    dA = A.to_device()
+    rng = np.random.default_rng(7)
+    x = rng.integers(-9, 10, A.n).astype(np.int64)
+    dx = cb.DeviceArray.from_host(x)
+    dy = cb.DeviceArray.from_host(np.zeros(A.m, np.int64))
+    cb.spmv_dense_device(dA, dx, dy)  # warm-up
+    cb.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        cb.spmv_dense_device(dA, dx, dy)
+    cb.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / reps
+    got = dy.to_host()
+    want = np.zeros(A.m, np.int64)
+    np.add.at(want, A.ir, A.val * x[A.col_ids()])
+    exact = bool(np.array_equal(got, want * (reps + 1)))
+    alg = 16 * A.jc.size + 8 + 16 * A.nnz + 8 * A.jc.size + 16 * A.m  # A, the x entries used, y read+written
+    for d in (dx, dy, dA):
+        d.release()
+    return {"workload": f"dense SpMV y += A*x (dcsc_gespmv), R-MAT s{scale} ef{ef}, PlusTimes<int64>, x and y in HBM",
+            "ms_per_spmv": round(ms, 3), "nnz_a": A.nnz, "alg_bytes": alg,
+            "gbps_algorithmic": round(alg / (ms * 1e-3) / 1e9, 1), "exact_after_all_reps": exact}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,7 +265,7 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["both", "two_phase", "learned"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
     ap.add_argument("--no-extra", action="store_true", help="skip the BFS, MCL and Galerkin workloads")
-    ap.add_argument("--only", choices=["bfs", "mcl", "galerkin"], default=None,
+    ap.add_argument("--only", choices=["bfs", "mcl", "galerkin", "spmv"], default=None,
                     help="run only this workload and print its JSON (for profiling)")
     ap.add_argument("--copy-gib", type=float, default=4.0,
                     help="size of a device-to-device copy timed before the workload: the achievable HBM rate, and "
@@ -255,7 +283,8 @@ def main() -> int:
         raise SystemExit("no GPU visible")
     if args.only:
         fn = {"bfs": lambda: bench_bfs(cb, args.scale, args.edgefactor), "mcl": lambda: bench_mcl(cb),
-              "galerkin": lambda: bench_galerkin(cb)}[args.only]
+              "galerkin": lambda: bench_galerkin(cb),
+              "spmv": lambda: bench_spmv(cb, args.scale, args.edgefactor)}[args.only]
         print(json.dumps({args.only: fn()}))
         return 0
     copy = None
@@ -374,7 +403,8 @@ def main() -> int:
     cb.set_config(**default_cfg)  # the other workloads run with the library defaults
     if not args.no_extra:
         for name, fn in (("bfs", lambda: bench_bfs(cb, args.scale, args.edgefactor)),
-                         ("mcl", lambda: bench_mcl(cb)), ("galerkin", lambda: bench_galerkin(cb))):
+                         ("mcl", lambda: bench_mcl(cb)), ("galerkin", lambda: bench_galerkin(cb)),
+                         ("spmv", lambda: bench_spmv(cb, args.scale, args.edgefactor))):
             log(f"{name} ...")
             try:
                 out[name] = fn()

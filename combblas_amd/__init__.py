@@ -388,18 +388,30 @@ def spgemm(A: DeviceDcsc, B: DeviceDcsc, semiring: str = "plus_times", mode: str
     return DeviceDcsc(out), stats
 
 
+_VEC_DTYPES = {np.dtype(np.int64): DTYPES["int64"], np.dtype(np.float64): DTYPES["float64"],
+               np.dtype(np.float32): DTYPES["float32"]}
+
+
+def spmv_dense_device(A: DeviceDcsc, x: "DeviceArray", y: "DeviceArray", semiring: str = "plus_times") -> None:
+    """dcsc_gespmv on device-resident vectors: y += A*x in place (x: A.n entries, y: A.m, one dtype)."""
+    if x.dtype != y.dtype or x.dtype not in _VEC_DTYPES:
+        raise ValueError("x and y must share an int64, float64 or float32 dtype")
+    if x.n != A.n or y.n != A.m:
+        raise ValueError(f"vector lengths {x.n}, {y.n} do not match A ({A.m} x {A.n})")
+    _check(lib().cbx_spmv_dense(C.byref(A.s), x.ptr, y.ptr, SEMIRINGS[semiring], _VEC_DTYPES[x.dtype], None))
+
+
 def spmv_dense(A: DeviceDcsc, x: np.ndarray, y: np.ndarray, semiring: str = "plus_times") -> np.ndarray:
     """dcsc_gespmv on the device: returns y accumulated with A*x (host arrays in and out)."""
     x = np.ascontiguousarray(x)
-    dt = {np.dtype(np.int64): DTYPES["int64"], np.dtype(np.float64): DTYPES["float64"],
-          np.dtype(np.float32): DTYPES["float32"]}[x.dtype]
     dx = DeviceArray.from_host(x)
     dy = DeviceArray.from_host(np.ascontiguousarray(y, dtype=x.dtype))
-    _check(lib().cbx_spmv_dense(C.byref(A.s), dx.ptr, dy.ptr, SEMIRINGS[semiring], dt, None))
-    out = dy.to_host()
-    dx.release()
-    dy.release()
-    return out
+    try:
+        spmv_dense_device(A, dx, dy, semiring)
+        return dy.to_host()
+    finally:
+        dx.release()
+        dy.release()
 
 
 def get_config() -> dict:

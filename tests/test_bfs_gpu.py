@@ -105,3 +105,31 @@ def test_spmv_dense_matches_oracle(semiring, dtype):
     else:
         tol = 1e-12 if xd == np.float64 else 1e-5
         assert np.allclose(got, want, rtol=tol, atol=0)
+
+
+def test_spmv_dense_device_accumulates_in_place():
+    """Device-resident vectors: two calls accumulate twice (y += A*x); vectors
+    of the wrong length or mixed dtypes are refused before any launch."""
+    rng = np.random.default_rng(9)
+    A = cb.gen_rmat(12, 8, seed=5)
+    A.val = rng.integers(-9, 10, A.nnz).astype(np.int64)
+    dA = A.to_device()
+    x = rng.integers(-20, 20, A.n).astype(np.int64)
+    dx = cb.DeviceArray.from_host(x)
+    dy = cb.DeviceArray.from_host(np.zeros(A.m, np.int64))
+    try:
+        cb.spmv_dense_device(dA, dx, dy)
+        cb.spmv_dense_device(dA, dx, dy)
+        want = oracle.spmv_dense(A, x, np.zeros(A.m, np.int64), "plus_times")
+        assert np.array_equal(dy.to_host(), 2 * want)
+        short = cb.DeviceArray.from_host(np.zeros(A.m - 1, np.int64))
+        with pytest.raises(ValueError):
+            cb.spmv_dense_device(dA, dx, short)
+        short.release()
+        fy = cb.DeviceArray.from_host(np.zeros(A.m, np.float64))
+        with pytest.raises(ValueError):
+            cb.spmv_dense_device(dA, dx, fy)
+        fy.release()
+    finally:
+        dx.release()
+        dy.release()
