@@ -9,6 +9,7 @@
 // check is exact only for integer-valued semirings.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -44,28 +45,67 @@ __global__ void k_col_checksum(const int64_t* __restrict__ cp, const int64_t* __
 }
 
 // Same rules as the SpGEMM input validation (rows strictly increasing within
-// a column and below m, jc strictly increasing and below n, cp consistent).
-__global__ void k_check_dcsc(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
-                             const int64_t* __restrict__ ir, int64_t nzc, int64_t nnz, int64_t nrows, int64_t ncols,
-                             int* __restrict__ bad) {
-  const int64_t col = global_wave();
-  if (col >= nzc) return;
-  const int lane = lane_id();
-  const int64_t b = cp[col], e = cp[col + 1];
-  bool err = false;
-  if (lane == 0) {
-    err |= b < 0 || e <= b || e > nnz || jc[col] < 0 || jc[col] >= ncols;
-    if (col + 1 < nzc) err |= jc[col + 1] <= jc[col];
-    if (col == 0) err |= b != 0;
-    if (col == nzc - 1) err |= e != nnz;
+// a column and below m, jc strictly increasing and below n, cp consistent),
+// in two fully parallel passes.  Per entry: rows in range, and every adjacent
+// pair with ir[e+1] <= ir[e] (a descent) is counted.  Per column: cp and jc,
+// and the descents that fall on a column start (ir[cp[q]] <= ir[cp[q]-1]).
+// With cp valid, the column starts are exactly the positions where a descent
+// is allowed, so the rows ascend within every column iff both counts agree.
+// cnt[0] = violations, cnt[1] = descents, cnt[2] = descents at column starts.
+// Counts are summed per workgroup before one atomic each: same-address device
+// atomics serialise, so there are few of them (at most kCheckBlocks).
+constexpr int kCheckNT = 256, kCheckBlocks = 1024;
+
+__device__ __forceinline__ void add_block_counts(long long bad, long long desc, unsigned long long* bad_out,
+                                                 unsigned long long* desc_out) {
+  __shared__ long long s[2][kCheckNT / kWave];
+  bad = wave_sum_ll(bad);
+  desc = wave_sum_ll(desc);
+  if (lane_id() == 0) {
+    s[0][wave_id()] = bad;
+    s[1][wave_id()] = desc;
   }
-  if (!(b < 0 || e < b || e > nnz))
-    for (int64_t i = b + lane; i < e; i += kWave) {
-      const int64_t r = ir[i];
-      err |= r < 0 || r >= nrows;
-      if (i + 1 < e) err |= ir[i + 1] <= r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long b = 0, d = 0;
+    for (int w = 0; w < kCheckNT / kWave; ++w) {
+      b += s[0][w];
+      d += s[1][w];
     }
-  if (err) atomicOr(bad, 1);
+    if (b) atomicAdd(bad_out, (unsigned long long)b);
+    if (d) atomicAdd(desc_out, (unsigned long long)d);
+  }
+}
+
+__global__ __launch_bounds__(kCheckNT) void k_check_entries(const int64_t* __restrict__ ir, int64_t nnz, int64_t nrows,
+                                unsigned long long* __restrict__ cnt) {
+  long long bad = 0, desc = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += stride) {
+    const int64_t r = ir[e];
+    bad += r < 0 || r >= nrows;
+    if (e + 1 < nnz) desc += ir[e + 1] <= r;
+  }
+  add_block_counts(bad, desc, &cnt[0], &cnt[1]);
+}
+
+__global__ __launch_bounds__(kCheckNT) void k_check_columns(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
+                                const int64_t* __restrict__ ir, int64_t nzc, int64_t nnz, int64_t ncols,
+                                unsigned long long* __restrict__ cnt) {
+  long long bad = 0, desc = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nzc; q += stride) {
+    const int64_t b = cp[q], e = cp[q + 1], j = jc[q];
+    bool err = b < 0 || e <= b || e > nnz || j < 0 || j >= ncols;
+    if (q + 1 < nzc) err |= jc[q + 1] <= j;
+    if (q == 0) err |= b != 0;
+    if (q == nzc - 1) err |= e != nnz;
+    bad += err;
+    // a column start inside the entries (a malformed cp elsewhere may still
+    // put b at 0: then there is nothing before it to compare with)
+    if (!err && q > 0 && b > 0) desc += ir[b] <= ir[b - 1];
+  }
+  add_block_counts(bad, desc, &cnt[0], &cnt[2]);
 }
 
 }  // namespace cbx
@@ -96,15 +136,26 @@ extern "C" cbx_status cbx_dcsc_check(const cbx_dcsc* X) {
   CBX_RETURN_IF(!X, CBX_ERR_INVALID, "cbx_dcsc_check: null argument");
   CBX_RETURN_IF(X->location != CBX_DEVICE, CBX_ERR_INVALID, "cbx_dcsc_check: expected device memory");
   if (X->nzc == 0) return X->nnz == 0 ? CBX_OK : set_error(CBX_ERR_INVALID, "cbx_dcsc_check: nnz without columns");
-  int* d = nullptr;
-  CBX_HIP(hipMalloc(&d, sizeof(int)));
-  CBX_HIP(hipMemset(d, 0, sizeof(int)));
-  k_check_dcsc<<<wave_grid(X->nzc), 256>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->m, X->n, d);
-  int bad = 0;
-  hipError_t e = hipMemcpy(&bad, d, sizeof(int), hipMemcpyDeviceToHost);
-  (void)hipFree(d);
+  const cbx_status init = init_device_once();
+  if (init != CBX_OK) return init;
+  // stream-ordered scratch from the pool (a plain hipMalloc/hipFree pair costs
+  // more than the check itself on a matrix of a few million entries)
+  const hipStream_t s0 = nullptr;
+  unsigned long long* d = nullptr;
+  CBX_HIP(hipMallocAsync((void**)&d, 3 * sizeof(unsigned long long), s0));
+  CBX_HIP(hipMemsetAsync(d, 0, 3 * sizeof(unsigned long long), s0));
+  auto blocks = [](int64_t n) {
+    return (unsigned)std::min<int64_t>(std::max<int64_t>((n + kCheckNT - 1) / kCheckNT, 1), kCheckBlocks);
+  };
+  k_check_columns<<<blocks(X->nzc), kCheckNT, 0, s0>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->n, d);
+  if (X->nnz > 0) k_check_entries<<<blocks(X->nnz), kCheckNT, 0, s0>>>(X->ir, X->nnz, X->m, d);
+  unsigned long long h[3] = {0, 0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s0);
+  (void)hipFreeAsync(d, s0);
+  if (e == hipSuccess) e = hipStreamSynchronize(s0);
   CBX_HIP(e);
-  return bad ? set_error(CBX_ERR_INVALID, "cbx_dcsc_check: malformed DCSC") : CBX_OK;
+  return h[0] || h[1] != h[2] ? set_error(CBX_ERR_INVALID, "cbx_dcsc_check: malformed DCSC") : CBX_OK;
 }
 
 extern "C" cbx_status cbx_dcsc_structure_to_host(const cbx_dcsc* X, int64_t* cp_host, int64_t* jc_host) {

@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 
 #include "cbx_device.h"
 #include "cbx_flatten.h"
@@ -475,9 +476,24 @@ extern "C" void cbx_bfs_plan_destroy(cbx_bfs_plan_s* p) { bfs_plan_free(p); }
 
 // ------------------------------------------------------ SpMV, dense vector
 // dcsc_gespmv (Friends.h:63-77): y[i] = add(y[i], multiply(A(i,j), x[j])) for
-// every stored entry, y accumulated in place.  One wave per stored column
-// (x[j] read once), the semiring's add applied with a global atomic; a
-// boolean A contributes x[j] itself.
+// every stored entry, y accumulated in place; a boolean A contributes x[j]
+// itself.
+//
+// Bucketed form (m <= 2^24): the entries are cut into tiles of kSvTile; pass 1
+// counts each tile's entries per 4096-row block, a scan over (block, tile)
+// gives every tile a slot range per block, pass 2 writes each product with its
+// row offset into its block's bucket, and pass 3 accumulates each block's
+// bucket in LDS (chunks of kSvChunk entries per workgroup) and updates y once
+// per row: a plain read-modify-write when one workgroup owns the block, the
+// semiring's device atomic when a hot block is split over several.  Device
+// atomics per entry (the direct form below, kept for taller matrices and small
+// products) serialise on the hot rows of skewed matrices.
+constexpr int kSvLB = 12, kSvBR = 1 << kSvLB;  // rows per block
+constexpr int kSvMaxBlocks = 4096;            // bucketed form while m <= 2^24
+constexpr int kSvTile = 4096, kSvNT = 256;    // entries and threads per counting / scattering workgroup
+constexpr int64_t kSvChunk = 32768;           // bucket entries per accumulating workgroup
+constexpr int64_t kSvMinNnz = 1 << 16;        // below this the direct form is cheaper
+
 template <int K, class T, bool BOOLA>
 __global__ void k_spmv_dense(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
                              const int64_t* __restrict__ ir, const void* __restrict__ valA_, int64_t nzc,
@@ -494,15 +510,185 @@ __global__ void k_spmv_dense(const int64_t* __restrict__ cp, const int64_t* __re
   }
 }
 
+// pass 1: hist[b * ntiles + t] = entries of tile t in row block b
+__global__ __launch_bounds__(kSvNT) void k_svb_count(const int64_t* __restrict__ ir, int64_t nnz, int nblk,
+                                                     int64_t ntiles, int64_t* __restrict__ hist) {
+  __shared__ int h[kSvMaxBlocks];
+  const int64_t t = blockIdx.x;
+  for (int b = threadIdx.x; b < nblk; b += kSvNT) h[b] = 0;
+  __syncthreads();
+  const int64_t e0 = t * kSvTile, e1 = min<int64_t>(nnz, e0 + kSvTile);
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += kSvNT) atomicAdd(&h[ir[e] >> kSvLB], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < nblk; b += kSvNT) hist[(int64_t)b * ntiles + t] = h[b];
+}
+
+// largest q in [lo, hi] with c[q] <= e (c ascending, c[lo] <= e)
+template <class P>
+__device__ __forceinline__ int64_t last_le(const P* c, int64_t lo, int64_t hi, int64_t e) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (c[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// pass 2: products into their blocks' buckets (row offset in the block, value)
+template <int K, class T, bool BOOLA>
+__global__ __launch_bounds__(kSvNT) void k_svb_scatter(const int64_t* __restrict__ cp,
+                                                       const int64_t* __restrict__ jc,
+                                                       const int64_t* __restrict__ ir,
+                                                       const void* __restrict__ valA_, int64_t nzc, int64_t nnz,
+                                                       const T* __restrict__ x, int nblk, int64_t ntiles,
+                                                       const int64_t* __restrict__ offs, uint16_t* __restrict__ brow,
+                                                       T* __restrict__ bval) {
+  using SR = Sem<K, T>;
+  __shared__ int h[kSvMaxBlocks];
+  __shared__ int64_t scp[kSvTile + 1];  // cp of the columns the tile spans (columns are non-empty)
+  __shared__ int64_t s_q[2];
+  const int64_t t = blockIdx.x;
+  const int64_t e0 = t * kSvTile, e1 = min<int64_t>(nnz, e0 + kSvTile);
+  for (int b = threadIdx.x; b < nblk; b += kSvNT) h[b] = 0;
+  if (threadIdx.x < 2) s_q[threadIdx.x] = last_le(cp, 0, nzc - 1, threadIdx.x == 0 ? e0 : e1 - 1);
+  __syncthreads();
+  const int64_t q0 = s_q[0], q1 = s_q[1];
+  const int nq = (int)(q1 - q0 + 2);  // <= kSvTile + 1
+  for (int i = threadIdx.x; i < nq; i += kSvNT) scp[i] = cp[q0 + i];
+  __syncthreads();
+  int64_t ql = 0;  // column of this thread's previous entry, relative to q0 (entries ascend)
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += kSvNT) {
+    ql = last_le(scp, ql, nq - 2, e);
+    const T xj = x[jc[q0 + ql]];
+    T prod;
+    if constexpr (BOOLA) prod = xj;
+    else prod = (T)SR::out(SR::mul(((const T*)valA_)[e], xj));
+    const int64_t r = ir[e];
+    const int b = (int)(r >> kSvLB);
+    const int64_t pos = offs[(int64_t)b * ntiles + t] + atomicAdd(&h[b], 1);
+    brow[pos] = (uint16_t)(r & (kSvBR - 1));
+    bval[pos] = prod;
+  }
+}
+
+// chunk table: cstart[b] = chunks of the blocks before b (cstart[nblk] = all);
+// one workgroup, kSvChunkNT threads with kSvMaxBlocks / kSvChunkNT blocks each
+constexpr int kSvChunkNT = 1024, kSvChunkPT = kSvMaxBlocks / kSvChunkNT;
+__global__ __launch_bounds__(kSvChunkNT) void k_svb_chunks(const int64_t* __restrict__ offs, int nblk,
+                                                           int64_t ntiles, int64_t nnz,
+                                                           int64_t* __restrict__ cstart) {
+  __shared__ long long wsum[kSvChunkNT / kWave];
+  int64_t c[kSvChunkPT];
+  long long sum = 0;
+#pragma unroll
+  for (int k = 0; k < kSvChunkPT; ++k) {
+    const int b = threadIdx.x * kSvChunkPT + k;
+    c[k] = 0;
+    if (b < nblk) {
+      const int64_t lo = offs[(int64_t)b * ntiles], hi = b + 1 < nblk ? offs[(int64_t)(b + 1) * ntiles] : nnz;
+      c[k] = (hi - lo + kSvChunk - 1) / kSvChunk;
+    }
+    sum += c[k];
+  }
+  long long total;
+  long long ex = block_excl_scan_ll<kSvChunkNT>(sum, wsum, total);
+#pragma unroll
+  for (int k = 0; k < kSvChunkPT; ++k) {
+    const int b = threadIdx.x * kSvChunkPT + k;
+    if (b < nblk) cstart[b] = ex;
+    ex += c[k];
+  }
+  if (threadIdx.x == 0) cstart[nblk] = total;
+}
+
+// pass 3: one workgroup per chunk of a block's bucket
+template <int K, class T>
+__global__ __launch_bounds__(kSvNT) void k_svb_accumulate(const int64_t* __restrict__ offs,
+                                                          const int64_t* __restrict__ cstart, int nblk,
+                                                          int64_t ntiles, int64_t nnz, int64_t m,
+                                                          const uint16_t* __restrict__ brow,
+                                                          const T* __restrict__ bval, T* __restrict__ y) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  __shared__ Acc acc[kSvBR];
+  const int64_t g = blockIdx.x;
+  if (g >= cstart[nblk]) return;
+  const int b = (int)last_le(cstart, 0, nblk - 1, g);  // blocks without chunks share cstart with the next
+  const int64_t lo = offs[(int64_t)b * ntiles], hi = b + 1 < nblk ? offs[(int64_t)(b + 1) * ntiles] : nnz;
+  const int64_t c = g - cstart[b], nch = cstart[b + 1] - cstart[b];
+  const int64_t i0 = lo + c * kSvChunk, i1 = min<int64_t>(hi, i0 + kSvChunk);
+  for (int r = threadIdx.x; r < kSvBR; r += kSvNT) acc[r] = SR::ident();
+  __syncthreads();
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kSvNT) SR::acc(&acc[brow[i]], (Acc)bval[i]);
+  __syncthreads();
+  const int64_t row0 = (int64_t)b * kSvBR;
+  using Bits = typename std::conditional<sizeof(Acc) == 8, unsigned long long, unsigned int>::type;
+  const Bits id = __builtin_bit_cast(Bits, SR::ident());
+  for (int r = threadIdx.x; r < kSvBR && row0 + r < m; r += kSvNT) {
+    const Acc a = acc[r];
+    if (nch == 1) {
+      y[row0 + r] = (T)SR::out(SR::add((Acc)y[row0 + r], a));
+    } else if (__builtin_bit_cast(Bits, a) != id) {  // the identity changes nothing
+      GAcc<K, T>::acc(&y[row0 + r], (T)SR::out(a));
+    }
+  }
+}
+
 template <int K, class T>
 static cbx_status spmv_dense_typed(const cbx_dcsc* A, bool boolA, const T* x, T* y, hipStream_t s) {
   if (A->nzc == 0) return CBX_OK;
-  if (boolA)
-    k_spmv_dense<K, T, true><<<wave_grid(A->nzc), 256, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, x, y);
-  else
-    k_spmv_dense<K, T, false><<<wave_grid(A->nzc), 256, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, x, y);
-  CBX_HIP(hipGetLastError());
-  CBX_HIP(hipStreamSynchronize(s));
+  const int64_t nnz = A->nnz, m = A->m;
+  const int64_t nblk64 = (m + kSvBR - 1) / kSvBR;
+  if (nnz < kSvMinNnz || nblk64 > kSvMaxBlocks) {
+    if (boolA)
+      k_spmv_dense<K, T, true><<<wave_grid(A->nzc), 256, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, x, y);
+    else
+      k_spmv_dense<K, T, false><<<wave_grid(A->nzc), 256, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, x, y);
+    CBX_HIP(hipGetLastError());
+    CBX_HIP(hipStreamSynchronize(s));
+    return CBX_OK;
+  }
+  const int nblk = (int)nblk64;
+  const int64_t ntiles = (nnz + kSvTile - 1) / kSvTile;
+  const int64_t nh = (int64_t)nblk * ntiles;
+  const int64_t max_chunks = nnz / kSvChunk + nblk;
+  CBX_RETURN_IF(ntiles >= (1ll << 31) || max_chunks >= (1ll << 31) || nh >= (1ll << 31), CBX_ERR_LIMIT,
+                "cbx_spmv_dense: matrix too large for one launch");
+  int64_t *hist = nullptr, *offs = nullptr, *cstart = nullptr;
+  uint16_t* brow = nullptr;
+  T* bval = nullptr;
+  void* tmp = nullptr;
+  size_t tb = 0;
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist, offs, (int)nh, s));
+  auto cleanup = [&]() {
+    for (void* p : {(void*)hist, (void*)offs, (void*)cstart, (void*)brow, (void*)bval, tmp})
+      if (p) (void)hipFreeAsync(p, s);
+  };
+  if (hipMallocAsync(&hist, sizeof(int64_t) * nh, s) != hipSuccess ||
+      hipMallocAsync(&offs, sizeof(int64_t) * nh, s) != hipSuccess ||
+      hipMallocAsync(&cstart, sizeof(int64_t) * (nblk + 1), s) != hipSuccess ||
+      hipMallocAsync(&brow, sizeof(uint16_t) * nnz, s) != hipSuccess ||
+      hipMallocAsync(&bval, sizeof(T) * nnz, s) != hipSuccess ||
+      hipMallocAsync(&tmp, std::max<size_t>(tb, 16), s) != hipSuccess) {
+    cleanup();
+    return set_error(CBX_ERR_NOMEM, "cbx_spmv_dense: workspace allocation failed");
+  }
+  k_svb_count<<<(unsigned)ntiles, kSvNT, 0, s>>>(A->ir, nnz, nblk, ntiles, hist);
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, hist, offs, (int)nh, s);
+  if (e == hipSuccess) {
+    if (boolA)
+      k_svb_scatter<K, T, true><<<(unsigned)ntiles, kSvNT, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, nnz, x, nblk,
+                                                                 ntiles, offs, brow, bval);
+    else
+      k_svb_scatter<K, T, false><<<(unsigned)ntiles, kSvNT, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, nnz, x,
+                                                                  nblk, ntiles, offs, brow, bval);
+    k_svb_chunks<<<1, kSvChunkNT, 0, s>>>(offs, nblk, ntiles, nnz, cstart);
+    k_svb_accumulate<K, T><<<(unsigned)max_chunks, kSvNT, 0, s>>>(offs, cstart, nblk, ntiles, nnz, m, brow, bval, y);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess) return set_error(CBX_ERR_HIP, "cbx_spmv_dense: %s", hipGetErrorString(e));
   return CBX_OK;
 }
 

@@ -133,3 +133,28 @@ def test_spmv_dense_device_accumulates_in_place():
     finally:
         dx.release()
         dy.release()
+
+
+@pytest.mark.parametrize("shape", ["rmat_s16", "tall"])
+def test_spmv_dense_bucketed_and_direct_forms(shape):
+    """Both device forms of dcsc_gespmv: the bucketed one (R-MAT s16: ~1 M
+    entries in 16 row blocks, the hot block split over several accumulating
+    workgroups) and the direct atomic one (m above 2^24 rows)."""
+    rng = np.random.default_rng(21)
+    if shape == "rmat_s16":
+        A = cb.gen_rmat(16, 16, seed=3)
+    else:
+        m, n, k = (1 << 24) + 4096, 5000, 200_000
+        key = np.unique(rng.integers(0, n, k) * m + rng.integers(0, m, k))
+        A = cb.Dcsc.from_coo(m, n, key % m, key // m, np.ones(key.size, np.int64), "int64")
+    A.val = rng.integers(-9, 10, A.nnz).astype(np.int64)
+    dA = A.to_device()
+    x = rng.integers(-20, 20, A.n).astype(np.int64)
+    for sr in ("plus_times", "min_plus", "select_max"):
+        y = rng.integers(-50, 50, A.m).astype(np.int64)
+        assert np.array_equal(cb.spmv_dense(dA, x, y, sr), oracle.spmv_dense(A, x, y, sr))
+    Af = A.astype("float64")
+    Af.val = rng.uniform(0.5, 2.0, A.nnz)
+    xf, yf = rng.uniform(0.5, 2.0, A.n), rng.uniform(0.5, 2.0, A.m)
+    got = cb.spmv_dense(Af.to_device(), xf, yf, "plus_times")
+    assert np.allclose(got, oracle.spmv_dense(Af, xf, yf, "plus_times"), rtol=1e-12, atol=0)

@@ -178,6 +178,47 @@ def test_invalid_input_rejected():
         gpu_spgemm(A, cb.gen_rmat(9, 4))  # inner dimensions differ
 
 
+def test_device_structure_check_cases():
+    """cbx_dcsc_check accepts a valid DCSC (rows descend only where a column
+    starts) and rejects each malformation: rows out of order or repeated
+    inside a column, a row out of range, jc not ascending, cp not starting at
+    0, and an empty column in cp."""
+    A = cb.gen_rmat(10, 8)
+    A.to_device().check()
+    c = int(np.argmax(np.diff(A.cp) >= 2))
+    b0 = int(A.cp[c])
+
+    def variant(edit):
+        X = cb.Dcsc(A.m, A.n, A.cp.copy(), A.jc.copy(), A.ir.copy(), A.val.copy())
+        edit(X)
+        return X
+
+    def swap_rows(X):
+        X.ir[b0], X.ir[b0 + 1] = X.ir[b0 + 1], X.ir[b0]
+
+    def repeat_row(X):
+        X.ir[b0 + 1] = X.ir[b0]
+
+    def row_out_of_range(X):
+        X.ir[-1] = X.m
+
+    def jc_unsorted(X):
+        X.jc[0], X.jc[1] = X.jc[1], X.jc[0]
+
+    def cp_offset(X):
+        X.cp[0] = 1
+
+    def empty_column(X):
+        X.cp[1] = X.cp[0]
+
+    for edit in (swap_rows, repeat_row, row_out_of_range, jc_unsorted, cp_offset, empty_column):
+        dX = variant(edit).to_device()
+        with pytest.raises(cb.CbxError) as ei:
+            dX.check()
+        assert ei.value.status == cb.ERR_INVALID, edit.__name__
+        dX.release()
+
+
 @pytest.mark.parametrize("scale,ef", [(18, 16)])
 def test_full_size_linear_checksums(scale, ef):
     """Large R-MAT product checked without materialising it on the host:
