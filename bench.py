@@ -122,25 +122,37 @@ def run_spgemm(cb, A, dA, mode: str, steps: int, warmup: int) -> dict:
 
 
 def bench_bfs(cb, scale: int, ef: int, nroots: int = 64) -> dict:
+    """TopDownBFS's parents from 64 seeded roots, each level top-down (SpMSpV)
+    or bottom-up as the plan chooses; the same roots top-down only for
+    comparison.  Both give the same parents (the tests check it)."""
     G = cb.gen_rmat(scale, ef, symmetrize=True, no_self_loops=True, pattern=True).astype("bool")
     deg = np.zeros(G.n, np.int64)
     deg[G.jc] = np.diff(G.cp)
     plan = cb.BfsPlan(G.to_device())
     roots = cb.bfs_roots(G, nroots)
-    plan.run(int(roots[0]), copy=False)
-    ms, teps, levels = [], [], []
-    for r in roots:
-        par, st = plan.run(int(r), copy=True)
-        edges = int(deg[par >= 0].sum()) // 2  # undirected edges in the reached component
-        ms.append(st["ms"])
-        levels.append(st["levels"])
-        teps.append(edges / (st["ms"] * 1e-3))
+    out = {"workload": f"BFS (TopDownBFS parents), symmetrised R-MAT s{scale} ef{ef} without self-loops, "
+                       f"{len(roots)} roots (seed 4242), one captured graph per level and direction"}
+    for direction in ("optimizing", "top_down"):
+        plan.set_direction(direction)
+        plan.run(int(roots[0]), copy=False)
+        ms, teps, levels, bu = [], [], [], []
+        for r in roots:
+            par, st = plan.run(int(r), copy=True)
+            edges = int(deg[par >= 0].sum()) // 2  # undirected edges in the reached component
+            ms.append(st["ms"])
+            levels.append(st["levels"])
+            bu.append(st["bottom_up_levels"])
+            teps.append(edges / (st["ms"] * 1e-3))
+        hm = len(teps) / sum(1.0 / t for t in teps)
+        res = {"gteps_harmonic_mean": round(hm / 1e9, 3), "ms_per_bfs_mean": round(float(np.mean(ms)), 3),
+               "levels_mean": round(float(np.mean(levels)), 2)}
+        if direction == "optimizing":
+            res["bottom_up_levels_mean"] = round(float(np.mean(bu)), 2)
+            out.update(res)
+        else:
+            out["top_down_only"] = res
     plan.release()
-    hm = len(teps) / sum(1.0 / t for t in teps)
-    return {"workload": f"top-down BFS, symmetrised R-MAT s{scale} ef{ef} without self-loops, SelectMax SpMSpV, "
-                        f"{len(roots)} roots (seed 4242), one captured graph per level",
-            "gteps_harmonic_mean": round(hm / 1e9, 3), "ms_per_bfs_mean": round(float(np.mean(ms)), 3),
-            "levels_mean": round(float(np.mean(levels)), 2)}
+    return out
 
 
 def bench_mcl(cb, n: int = 1_000_000, reps: int = 2) -> dict:

@@ -75,7 +75,7 @@ MLP_PARAMS = 64 * 6 + 64 + 64 * 64 + 64 + 64 + 1 + 12
 
 
 class cbx_bfs_stats(C.Structure):
-    _fields_ = [("levels", C.c_int32), ("pad", C.c_int32), ("reached", C.c_int64), ("ms", C.c_double)]
+    _fields_ = [("levels", C.c_int32), ("bottom_up_levels", C.c_int32), ("reached", C.c_int64), ("ms", C.c_double)]
 
 
 class cbx_mcl_stats(C.Structure):
@@ -154,6 +154,8 @@ def lib() -> C.CDLL:
         L.cbx_bfs_run.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, P(cbx_bfs_stats)]
         L.cbx_bfs_plan_destroy.argtypes = [C.c_void_p]
         L.cbx_bfs_plan_destroy.restype = None
+        L.cbx_bfs_plan_set_direction.argtypes = [C.c_void_p, C.c_int32]
+        L.cbx_bfs_plan_set_direction.restype = C.c_int32
         L.cbx_spgemm_symbolic.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p, C.c_void_p]
         L.cbx_spgemm_set_predictor.argtypes = [C.c_void_p]
         L.cbx_spgemm_set_predictor.restype = C.c_int32
@@ -575,20 +577,29 @@ def spmspv(A: DeviceDcsc, xind: np.ndarray, xval: np.ndarray, semiring: str = "s
     return dyi.to_host(ny.value), dyv.to_host(ny.value)
 
 
-class BfsPlan:
-    """Top-down BFS plan over a device adjacency matrix (square, symmetric)."""
+BFS_DIRECTIONS = {"optimizing": 0, "top_down": 1, "bottom_up": 2}
 
-    def __init__(self, A: DeviceDcsc):
+
+class BfsPlan:
+    """BFS plan over a device adjacency matrix (square, symmetric): TopDownBFS's
+    parents, with each level run top-down or bottom-up (same result)."""
+
+    def __init__(self, A: DeviceDcsc, direction: str = "optimizing"):
         h = C.c_void_p()
         _check(lib().cbx_bfs_plan_create(C.byref(A.s), C.byref(h)))
         self.h = h.value
         self.n = A.n
         self.parents = DeviceArray(A.n, np.int64)
+        self.set_direction(direction)
+
+    def set_direction(self, direction: str) -> None:
+        """"optimizing" (per level, the default), "top_down" or "bottom_up"."""
+        _check(lib().cbx_bfs_plan_set_direction(self.h, BFS_DIRECTIONS[direction]))
 
     def run(self, root: int, copy: bool = True) -> tuple[np.ndarray | None, dict]:
         st = cbx_bfs_stats()
         _check(lib().cbx_bfs_run(self.h, int(root), self.parents.ptr, C.byref(st)))
-        stats = {"levels": st.levels, "reached": st.reached, "ms": st.ms}
+        stats = {"levels": st.levels, "bottom_up_levels": st.bottom_up_levels, "reached": st.reached, "ms": st.ms}
         return (self.parents.to_host() if copy else None), stats
 
     def release(self) -> None:

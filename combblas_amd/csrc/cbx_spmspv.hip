@@ -283,8 +283,11 @@ __global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __
   }
 }
 
-__global__ void k_bfs_zero(int32_t* __restrict__ c) {
-  if (threadIdx.x == 0) *c = 0;
+__global__ void k_bfs_zero(int32_t* __restrict__ c, unsigned long long* __restrict__ edges) {
+  if (threadIdx.x == 0) {
+    *c = 0;
+    *edges = 0ull;
+  }
 }
 
 // One level's SpMV(A, fringe) with SelectMax<bool,int64> fused with the
@@ -339,15 +342,119 @@ __global__ __launch_bounds__(NT) void k_bfs_expand(const int64_t* __restrict__ b
   }
 }
 
-// parents.Set(fringe) (FullyDistVec.cpp:1036) and the visited mark.
-__global__ void k_bfs_finalize(const int32_t* __restrict__ f, const int32_t* __restrict__ count,
-                               const long long* __restrict__ cand, int64_t* __restrict__ parents,
-                               uint32_t* __restrict__ visited) {
+// parents.Set(fringe) (FullyDistVec.cpp:1036) and the visited mark; also
+// sums the new frontier's degrees (the host's direction choice), one atomic
+// per workgroup.
+constexpr int kBfsFinNT = 256;
+__global__ __launch_bounds__(kBfsFinNT) void k_bfs_finalize(const int32_t* __restrict__ f,
+                                                            const int32_t* __restrict__ count,
+                                                            const long long* __restrict__ cand,
+                                                            const int64_t* __restrict__ beg,
+                                                            const int64_t* __restrict__ end,
+                                                            int64_t* __restrict__ parents,
+                                                            uint32_t* __restrict__ visited,
+                                                            unsigned long long* __restrict__ edges) {
+  __shared__ long long ws[kBfsFinNT / kWave];
   const int nf = *count;
+  long long deg = 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x) {
     const int32_t i = f[t];
     parents[i] = cand[i];
     atomicOr(&visited[i >> 5], 1u << (i & 31));
+    deg += end[i] - beg[i];
+  }
+  deg = wave_sum_ll(deg);
+  if (lane_id() == 0) ws[wave_id()] = deg;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long d = 0;
+    for (int w = 0; w < kBfsFinNT / kWave; ++w) d += ws[w];
+    if (d) atomicAdd(edges, (unsigned long long)d);
+  }
+}
+
+// Bottom-up form of the same level (for large frontiers): the frontier as a
+// bitmap, then every unvisited vertex scans its neighbours from the largest
+// index down and stops at the first one in the frontier -- the largest
+// frontier neighbour, the value SelectMax's atomic max gives top-down, so
+// both forms produce the same parents and next frontier.
+__global__ void k_bfs_fbits_clear(uint32_t* __restrict__ fbits, int64_t nw) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < nw) fbits[i] = 0u;
+}
+
+__global__ void k_bfs_fbits_set(const int32_t* __restrict__ f, const int32_t* __restrict__ count,
+                                uint32_t* __restrict__ fbits) {
+  const int nf = *count;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x)
+    atomicOr(&fbits[f[t] >> 5], 1u << (f[t] & 31));
+}
+
+// Vertices of degree above kBfsSerialDeg are scanned by the whole wave, 64
+// neighbours per step from the top (hubs would stall a lane for thousands of
+// steps); the first step with a hit gives the largest frontier neighbour.
+constexpr int64_t kBfsSerialDeg = 64;
+
+__global__ __launch_bounds__(256) void k_bfs_bottom_up(const int64_t* __restrict__ beg,
+                                                       const int64_t* __restrict__ end,
+                                                       const int32_t* __restrict__ rows, int64_t n,
+                                                       const uint32_t* __restrict__ fbits,
+                                                       const uint32_t* __restrict__ visited,
+                                                       long long* __restrict__ cand, int32_t* __restrict__ fout,
+                                                       int32_t* __restrict__ fout_count) {
+  const int lane = lane_id();
+  for (int64_t v0 = blockIdx.x * (int64_t)blockDim.x; v0 < n; v0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = v0 + threadIdx.x;  // the loop bound is uniform per wave (ballots below)
+    bool push = false, heavy = false;
+    int64_t b = 0, e = 0;
+    if (v < n && !((visited[v >> 5] >> (v & 31)) & 1u)) {
+      b = beg[v];
+      e = end[v];
+      if (e - b > kBfsSerialDeg) {
+        heavy = true;
+      } else {
+        for (int64_t i = e - 1; i >= b; --i) {
+          const int32_t u = rows[i];
+          if ((fbits[u >> 5] >> (u & 31)) & 1u) {
+            cand[v] = u;
+            push = true;
+            break;
+          }
+        }
+      }
+    }
+    uint64_t hv = __ballot(heavy);
+    while (hv) {  // the wave's high-degree vertices, one at a time
+      const int h = __ffsll((unsigned long long)hv) - 1;
+      hv &= hv - 1;
+      const int64_t hb = __shfl(b, h, kWave), he = __shfl(e, h, kWave);
+      for (int64_t top = he; top > hb; top -= kWave) {
+        const int64_t i = top - 1 - lane;  // lane 0 holds the largest index of the step
+        int32_t u = -1;
+        if (i >= hb) {
+          u = rows[i];
+          if (!((fbits[u >> 5] >> (u & 31)) & 1u)) u = -1;
+        }
+        const uint64_t hit = __ballot(u >= 0);
+        if (hit) {
+          const int first = __ffsll((unsigned long long)hit) - 1;
+          const int32_t par = __shfl(u, first, kWave);
+          if (lane == h) {
+            cand[v] = par;
+            push = true;
+          }
+          break;
+        }
+      }
+    }
+    const uint64_t mask = __ballot(push);
+    if (mask) {
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      int basei = 0;
+      if (lane == leader) basei = atomicAdd(fout_count, __popcll(mask));
+      basei = __shfl(basei, leader, kWave);
+      if (push) fout[basei + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)v;
+    }
   }
 }
 
@@ -364,20 +471,26 @@ struct cbx_bfs_plan_s {
   int32_t *rows = nullptr, *f[2] = {nullptr, nullptr}, *counts = nullptr;
   long long* cand = nullptr;
   uint32_t* visited = nullptr;
-  int32_t* host_count = nullptr;  // pinned
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  unsigned long long* edges = nullptr;  // degree sums of the two frontier buffers
+  uint32_t* fbits = nullptr;             // frontier bitmap of a bottom-up level
+  int32_t* host_count = nullptr;  // pinned, 32 B: next frontier size; [2..3] its degree sum; [4..7] root's range
+  int direction = CBX_BFS_DIRECTION_OPTIMIZING;
+  // [direction (0 top-down, 1 bottom-up)][frontier parity]
+  hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
 };
 
 static void bfs_plan_free(cbx_bfs_plan_s* p) {
   if (!p) return;
-  for (int i = 0; i < 2; ++i) {
-    if (p->exec[i]) (void)hipGraphExecDestroy(p->exec[i]);
-    if (p->graph[i]) (void)hipGraphDestroy(p->graph[i]);
+  for (int d = 0; d < 2; ++d)
+    for (int i = 0; i < 2; ++i) {
+      if (p->exec[d][i]) (void)hipGraphExecDestroy(p->exec[d][i]);
+      if (p->graph[d][i]) (void)hipGraphDestroy(p->graph[d][i]);
+    }
+  for (int i = 0; i < 2; ++i)
     if (p->f[i]) (void)hipFree(p->f[i]);
-  }
   for (void* q : {(void*)p->beg, (void*)p->end, (void*)p->parents, (void*)p->rows, (void*)p->counts, (void*)p->cand,
-                  (void*)p->visited})
+                  (void*)p->visited, (void*)p->edges, (void*)p->fbits})
     if (q) (void)hipFree(q);
   if (p->host_count) (void)hipHostFree(p->host_count);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -386,6 +499,7 @@ static void bfs_plan_free(cbx_bfs_plan_s* p) {
 
 constexpr int kBfsNT = 256;
 constexpr unsigned kBfsGrid = 2048;
+constexpr int64_t kBfsAlpha = 256;
 
 extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** out) {
   CBX_RETURN_IF(!A || !out, CBX_ERR_INVALID, "cbx_bfs_plan_create: null argument");
@@ -410,28 +524,42 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
       hipMalloc(&p->parents, 8 * std::max<int64_t>(n, 1)) || hipMalloc(&p->rows, 4 * std::max<int64_t>(p->nnz, 1)) ||
       hipMalloc(&p->f[0], 4 * std::max<int64_t>(n, 1)) || hipMalloc(&p->f[1], 4 * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->counts, 16) || hipMalloc(&p->cand, 8 * std::max<int64_t>(n, 1)) ||
-      hipMalloc(&p->visited, 4 * p->nw) || hipHostMalloc(&p->host_count, 16, hipHostMallocDefault))
+      hipMalloc(&p->visited, 4 * p->nw) || hipMalloc(&p->edges, 2 * sizeof(unsigned long long)) ||
+      hipMalloc(&p->fbits, 4 * p->nw) || hipHostMalloc(&p->host_count, 32, hipHostMallocDefault))
     return fail("allocation");
   hipStream_t s = p->stream;
   if (hipMemsetAsync(p->beg, 0, 8 * n, s) || hipMemsetAsync(p->end, 0, 8 * n, s)) return fail("memset");
   k_bfs_dense_cp<<<nb(A->nzc, 256), 256, 0, s>>>(A->cp, A->jc, A->nzc, n, p->beg, p->end);
   k_bfs_rows32<<<nb(A->nnz, 256), 256, 0, s>>>(A->ir, A->nnz, p->rows);
   if (hipStreamSynchronize(s) != hipSuccess) return fail("setup kernels");
-  // one graph per frontier parity: clear the next count, expand, finalize,
-  // copy the next count to pinned host memory
-  for (int par = 0; par < 2; ++par) {
-    int32_t* cin = p->counts + par;
-    int32_t* cout = p->counts + (1 - par);
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail("capture begin");
-    k_bfs_zero<<<1, 64, 0, s>>>(cout);
-    k_bfs_expand<kBfsNT, 4><<<kBfsGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->f[par], cin, p->visited, p->cand,
-                                                       p->f[1 - par], cout);
-    k_bfs_finalize<<<1024, 256, 0, s>>>(p->f[1 - par], cout, p->cand, p->parents, p->visited);
-    (void)hipMemcpyAsync(p->host_count, cout, sizeof(int32_t), hipMemcpyDeviceToHost, s);
-    if (hipStreamEndCapture(s, &p->graph[par]) != hipSuccess) return fail("capture end");
-    if (hipGraphInstantiate(&p->exec[par], p->graph[par], nullptr, nullptr, 0) != hipSuccess)
-      return fail("graph instantiate");
-  }
+  // one graph per (direction, frontier parity): clear the next count, the
+  // level's kernels, finalize, copy the next count and degree sum to pinned
+  // host memory
+  const unsigned bu_grid = (unsigned)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 8192);
+  for (int dir = 0; dir < 2; ++dir)
+    for (int par = 0; par < 2; ++par) {
+      int32_t* cin = p->counts + par;
+      int32_t* cout = p->counts + (1 - par);
+      unsigned long long* eout = p->edges + (1 - par);
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail("capture begin");
+      k_bfs_zero<<<1, 64, 0, s>>>(cout, eout);
+      if (dir == 0) {
+        k_bfs_expand<kBfsNT, 4><<<kBfsGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->f[par], cin, p->visited,
+                                                           p->cand, p->f[1 - par], cout);
+      } else {
+        k_bfs_fbits_clear<<<nb(p->nw, 256), 256, 0, s>>>(p->fbits, p->nw);
+        k_bfs_fbits_set<<<256, 256, 0, s>>>(p->f[par], cin, p->fbits);
+        k_bfs_bottom_up<<<bu_grid, 256, 0, s>>>(p->beg, p->end, p->rows, n, p->fbits, p->visited, p->cand,
+                                                p->f[1 - par], cout);
+      }
+      k_bfs_finalize<<<256, kBfsFinNT, 0, s>>>(p->f[1 - par], cout, p->cand, p->beg, p->end, p->parents, p->visited,
+                                               eout);
+      (void)hipMemcpyAsync(p->host_count, cout, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+      (void)hipMemcpyAsync(p->host_count + 2, eout, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+      if (hipStreamEndCapture(s, &p->graph[dir][par]) != hipSuccess) return fail("capture end");
+      if (hipGraphInstantiate(&p->exec[dir][par], p->graph[dir][par], nullptr, nullptr, 0) != hipSuccess)
+        return fail("graph instantiate");
+    }
   *out = p;
   return CBX_OK;
 }
@@ -446,13 +574,27 @@ extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* pare
   CBX_HIP(hipEventRecord(e0, s));
   k_bfs_reset<<<nb(std::max(p->n, p->nw), 256), 256, 0, s>>>(p->n, p->cand, p->parents, p->visited, p->nw, p->f[0],
                                                              p->counts, root);
-  int levels = 0;
-  int64_t reached = 0;
+  // direction per level: bottom-up while the frontier's edges exceed 1/256 of
+  // the edges still unexplored.  (Beamer's rule compares the two with a factor
+  // 14 for equal per-edge costs; here a top-down edge costs a device atomic on
+  // the target, some 20-40 times a bottom-up edge, which is a streamed read.)
+  int levels = 0, bottom_up_levels = 0;
+  int64_t reached = 0, mu = p->nnz;
+  int64_t* hdeg = reinterpret_cast<int64_t*>(p->host_count + 4);  // pinned: beg[root], end[root]
+  CBX_HIP(hipMemcpyAsync(hdeg, p->beg + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipMemcpyAsync(hdeg + 1, p->end + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  int64_t fe = hdeg[1] - hdeg[0];  // edges of the current frontier
+  bool bottom_up = p->direction == CBX_BFS_BOTTOM_UP;
   for (int par = 0;; par ^= 1) {
-    CBX_HIP(hipGraphLaunch(p->exec[par], s));
+    if (p->direction == CBX_BFS_DIRECTION_OPTIMIZING) bottom_up = fe > mu / kBfsAlpha;
+    CBX_HIP(hipGraphLaunch(p->exec[bottom_up ? 1 : 0][par], s));
     CBX_HIP(hipStreamSynchronize(s));
     ++levels;
+    bottom_up_levels += bottom_up;
     const int32_t next = *(volatile int32_t*)p->host_count;
+    fe = (int64_t) * (volatile unsigned long long*)(p->host_count + 2);
+    mu -= fe;
     reached += next;
     if (next == 0 || levels > p->n + 1) break;
   }
@@ -465,9 +607,19 @@ extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* pare
   (void)hipEventDestroy(e1);
   if (stats) {
     stats->levels = levels;
+    stats->bottom_up_levels = bottom_up_levels;
     stats->reached = reached;
     stats->ms = ms;
   }
+  return CBX_OK;
+}
+
+extern "C" cbx_status cbx_bfs_plan_set_direction(cbx_bfs_plan_s* p, cbx_bfs_direction direction) {
+  CBX_RETURN_IF(!p, CBX_ERR_INVALID, "cbx_bfs_plan_set_direction: null plan");
+  CBX_RETURN_IF(direction != CBX_BFS_DIRECTION_OPTIMIZING && direction != CBX_BFS_TOP_DOWN &&
+                    direction != CBX_BFS_BOTTOM_UP,
+                CBX_ERR_INVALID, "cbx_bfs_plan_set_direction: unknown direction %d", (int)direction);
+  p->direction = direction;
   return CBX_OK;
 }
 

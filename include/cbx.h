@@ -314,13 +314,28 @@ typedef struct cbx_bfs_plan_s* cbx_bfs_plan;
 
 typedef struct cbx_bfs_stats {
   int32_t levels;   /* SpMV iterations (TopDownBFS's "Number iterations") */
-  int32_t pad;
+  int32_t bottom_up_levels; /* of which run bottom-up (see cbx_bfs_plan_set_direction) */
   int64_t reached;  /* vertices that received a parent */
   double ms;        /* device time of the whole BFS */
 } cbx_bfs_stats;
 
 /* A: device DCSC, square, symmetric pattern (values ignored). */
 cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan* plan);
+
+/* How each level is computed.  Top-down is TopDownBFS's SpMV: every frontier
+ * vertex offers itself to its unvisited neighbours (atomic max).  Bottom-up
+ * has every unvisited vertex scan its neighbours from the largest index down
+ * and stop at the first frontier vertex: the same largest frontier neighbour,
+ * so parents and levels are identical in all three settings.  The default,
+ * CBX_BFS_DIRECTION_OPTIMIZING, chooses per level: bottom-up while the
+ * frontier's edges exceed 1/256 of the unexplored edges.  No counterpart in
+ * the reference, which runs top-down only (TopDownBFS.cpp:442-448). */
+typedef enum cbx_bfs_direction {
+  CBX_BFS_DIRECTION_OPTIMIZING = 0,
+  CBX_BFS_TOP_DOWN = 1,
+  CBX_BFS_BOTTOM_UP = 2
+} cbx_bfs_direction;
+cbx_status cbx_bfs_plan_set_direction(cbx_bfs_plan plan, cbx_bfs_direction direction);
 /* parents: device int64[n], overwritten. */
 cbx_status cbx_bfs_run(cbx_bfs_plan plan, int64_t root, int64_t* parents, cbx_bfs_stats* stats);
 void cbx_bfs_plan_destroy(cbx_bfs_plan plan);

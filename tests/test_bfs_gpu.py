@@ -158,3 +158,28 @@ def test_spmv_dense_bucketed_and_direct_forms(shape):
     xf, yf = rng.uniform(0.5, 2.0, A.n), rng.uniform(0.5, 2.0, A.m)
     got = cb.spmv_dense(Af.to_device(), xf, yf, "plus_times")
     assert np.allclose(got, oracle.spmv_dense(Af, xf, yf, "plus_times"), rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_bfs_directions_give_identical_parents(scale):
+    """Top-down only, bottom-up only and the per-level switch give the same
+    parents (the largest frontier neighbour) and level counts, all equal to
+    the oracle; the switching plan runs some levels each way at s16."""
+    A = graph(scale)
+    dA = A.to_device()
+    plans = {d: cb.BfsPlan(dA, d) for d in ("top_down", "bottom_up", "optimizing")}
+    for root in cb.bfs_roots(A, 4, seed=100 + scale):
+        want, it = oracle.bfs(A, int(root))
+        for d, plan in plans.items():
+            got, st = plan.run(int(root))
+            assert st["levels"] == it and np.array_equal(got, want), d
+            if d == "top_down":
+                assert st["bottom_up_levels"] == 0
+            elif d == "bottom_up":
+                assert st["bottom_up_levels"] == st["levels"]
+            elif scale == 16 and st["reached"] > A.n // 4:
+                assert 0 < st["bottom_up_levels"] < st["levels"]
+    for plan in plans.values():
+        plan.release()
+    with pytest.raises(KeyError):
+        cb.BfsPlan(dA, "sideways")
