@@ -56,27 +56,6 @@ __global__ void k_col_checksum(const int64_t* __restrict__ cp, const int64_t* __
 // atomics serialise, so there are few of them (at most kCheckBlocks).
 constexpr int kCheckNT = 256, kCheckBlocks = 1024;
 
-__device__ __forceinline__ void add_block_counts(long long bad, long long desc, unsigned long long* bad_out,
-                                                 unsigned long long* desc_out) {
-  __shared__ long long s[2][kCheckNT / kWave];
-  bad = wave_sum_ll(bad);
-  desc = wave_sum_ll(desc);
-  if (lane_id() == 0) {
-    s[0][wave_id()] = bad;
-    s[1][wave_id()] = desc;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long b = 0, d = 0;
-    for (int w = 0; w < kCheckNT / kWave; ++w) {
-      b += s[0][w];
-      d += s[1][w];
-    }
-    if (b) atomicAdd(bad_out, (unsigned long long)b);
-    if (d) atomicAdd(desc_out, (unsigned long long)d);
-  }
-}
-
 __global__ __launch_bounds__(kCheckNT) void k_check_entries(const int64_t* __restrict__ ir, int64_t nnz, int64_t nrows,
                                 unsigned long long* __restrict__ cnt) {
   long long bad = 0, desc = 0;
@@ -86,7 +65,7 @@ __global__ __launch_bounds__(kCheckNT) void k_check_entries(const int64_t* __res
     bad += r < 0 || r >= nrows;
     if (e + 1 < nnz) desc += ir[e + 1] <= r;
   }
-  add_block_counts(bad, desc, &cnt[0], &cnt[1]);
+  block_add_counts<kCheckNT>(bad, desc, &cnt[0], &cnt[1]);
 }
 
 __global__ __launch_bounds__(kCheckNT) void k_check_columns(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
@@ -105,7 +84,7 @@ __global__ __launch_bounds__(kCheckNT) void k_check_columns(const int64_t* __res
     // put b at 0: then there is nothing before it to compare with)
     if (!err && q > 0 && b > 0) desc += ir[b] <= ir[b - 1];
   }
-  add_block_counts(bad, desc, &cnt[0], &cnt[2]);
+  block_add_counts<kCheckNT>(bad, desc, &cnt[0], &cnt[2]);
 }
 
 }  // namespace cbx

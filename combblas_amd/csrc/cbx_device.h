@@ -119,4 +119,29 @@ __device__ __forceinline__ long long block_sum_ll(long long v, long long* wsum) 
   return t;
 }
 
+// Adds two per-thread counts into *pa and *pb with one device atomic each per
+// workgroup (same-address device atomics serialise, so reduce first).
+// Contains a barrier: every thread of the workgroup must call it.
+template <int NT>
+__device__ __forceinline__ void block_add_counts(long long a, long long b, unsigned long long* pa,
+                                                 unsigned long long* pb) {
+  __shared__ long long s[2][NT / kWave];
+  a = wave_sum_ll(a);
+  b = wave_sum_ll(b);
+  if (lane_id() == 0) {
+    s[0][wave_id()] = a;
+    s[1][wave_id()] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long x = 0, y = 0;
+    for (int w = 0; w < NT / kWave; ++w) {
+      x += s[0][w];
+      y += s[1][w];
+    }
+    if (x) atomicAdd(pa, (unsigned long long)x);
+    if (y) atomicAdd(pb, (unsigned long long)y);
+  }
+}
+
 }  // namespace cbx

@@ -238,32 +238,62 @@ __global__ void k_set_last(int64_t* cp, const int64_t* nsel, const int64_t* tota
 }
 
 // Input validation (the reference's preconditions, mtSpGEMM.h / dcsc.h):
-// cp starts at 0, ends at nnz and never decreases; jc strictly increasing and
-// below ncols; rows of every column strictly increasing and below nrows.
-// One wave per stored column.  Sets bit `flagbit` of *err on a violation.
-__global__ void k_validate(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
-                           const int64_t* __restrict__ ir, int64_t nzc, int64_t nnz, int64_t nrows,
-                           int64_t ncols, int flagbit, int* __restrict__ err) {
-  const int64_t col = global_wave();
-  if (col >= nzc) return;
-  const int lane = lane_id();
-  const int64_t b = cp[col], e = cp[col + 1];
-  bool bad = false;
-  if (lane == 0) {
-    bad |= b < 0 || e < b || e > nnz;
-    bad |= jc[col] < 0 || jc[col] >= ncols;
-    if (col + 1 < nzc) bad |= jc[col + 1] <= jc[col];
-    if (col == 0) bad |= b != 0;
-    if (col == nzc - 1) bad |= e != nnz;
+// cp starts at 0, ends at nnz and never decreases (empty columns allowed); jc
+// strictly increasing and below ncols; rows of every column strictly
+// increasing and below nrows.  Two fully parallel passes: per entry, rows in
+// range and every adjacent descent ir[e+1] <= ir[e] counted; per column, cp
+// and jc, and the descents at the positions where a column begins (each
+// distinct start counted once).  With cp valid those are the only places a
+// descent is allowed, so the rows ascend within columns iff the two counts
+// agree.  cnt[0] violations, cnt[1] descents, cnt[2] descents at starts;
+// k_validate_flag then sets bit `flagbit` of *err.
+constexpr int kValNT = 256, kValBlocks = 1024;
+
+__global__ __launch_bounds__(kValNT) void k_validate_cols(const int64_t* __restrict__ cp,
+                                                          const int64_t* __restrict__ jc,
+                                                          const int64_t* __restrict__ ir, int64_t nzc, int64_t nnz,
+                                                          int64_t ncols, unsigned long long* __restrict__ cnt) {
+  long long bad = 0, desc = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nzc; q += stride) {
+    const int64_t b = cp[q], e = cp[q + 1], j = jc[q];
+    bool err = b < 0 || e < b || e > nnz || j < 0 || j >= ncols;
+    if (q + 1 < nzc) err |= jc[q + 1] <= j;
+    if (q == 0) err |= b != 0;
+    if (q == nzc - 1) err |= e != nnz;
+    bad += err;
+    if (!err && q > 0 && b > 0 && b < nnz && cp[q - 1] < b) desc += ir[b] <= ir[b - 1];
   }
-  if (!(b < 0 || e < b || e > nnz)) {
-    for (int64_t i = b + lane; i < e; i += kWave) {
-      const int64_t r = ir[i];
-      bad |= r < 0 || r >= nrows;
-      if (i + 1 < e) bad |= ir[i + 1] <= r;
-    }
+  block_add_counts<kValNT>(bad, desc, &cnt[0], &cnt[2]);
+}
+
+__global__ __launch_bounds__(kValNT) void k_validate_entries(const int64_t* __restrict__ ir, int64_t nnz,
+                                                             int64_t nrows, unsigned long long* __restrict__ cnt) {
+  long long bad = 0, desc = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += stride) {
+    const int64_t r = ir[e];
+    bad += r < 0 || r >= nrows;
+    if (e + 1 < nnz) desc += ir[e + 1] <= r;
   }
-  if (bad) atomicOr(err, 1 << flagbit);
+  block_add_counts<kValNT>(bad, desc, &cnt[0], &cnt[1]);
+}
+
+__global__ void k_validate_flag(const unsigned long long* __restrict__ cnt, int flagbit, int* __restrict__ err) {
+  if (threadIdx.x == 0 && (cnt[0] || cnt[1] != cnt[2])) atomicOr(err, 1 << flagbit);
+}
+
+// Launches the validation of X on s; cnt: 3 device counters (cleared here).
+static cbx_status launch_validate(const cbx_dcsc* X, int flagbit, unsigned long long* cnt, int* err, hipStream_t s) {
+  auto blocks = [](int64_t n) {
+    return (unsigned)std::min<int64_t>(std::max<int64_t>((n + kValNT - 1) / kValNT, 1), kValBlocks);
+  };
+  CBX_HIP(hipMemsetAsync(cnt, 0, 3 * sizeof(unsigned long long), s));
+  k_validate_cols<<<blocks(X->nzc), kValNT, 0, s>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->n, cnt);
+  k_validate_entries<<<blocks(X->nnz), kValNT, 0, s>>>(X->ir, X->nnz, X->m, cnt);
+  k_validate_flag<<<1, 64, 0, s>>>(cnt, flagbit, err);
+  CBX_HIP(hipGetLastError());
+  return CBX_OK;
 }
 
 // --------------------------------------------------------- symbolic: hash
@@ -1815,9 +1845,10 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const int G = kSymRBW * 32 / TR;  // tiles per symbolic pass of the streaming tier
 
   DevBuf b_colmap, b_rows, b_bpos, b_flop, b_order, b_flop_s, b_order_s, b_P, b_nnz, b_cp, b_tmp, b_thr, b_bounds,
-      b_flags, b_nsel, b_err, b_ul, b_uh, b_ucnt;
+      b_flags, b_nsel, b_err, b_vcnt, b_ul, b_uh, b_ucnt;
   int32_t *colmap, *rowsA, *bpos, *order, *order_s, *P;
   int *err, *ucnt;
+  unsigned long long* vcnt;
   int64_t *flop, *flop_s, *nnzcol, *cpC, *thr, *bounds, *nsel;
   uint8_t* flags;
   Unit *units_l = nullptr, *units_h = nullptr;
@@ -1835,6 +1866,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_TRY(dalloc(b_flags, nzcB, s, &flags));
   CBX_TRY(dalloc(b_nsel, 2, s, &nsel));
   CBX_TRY(dalloc(b_err, 1, s, &err));
+  CBX_TRY(dalloc(b_vcnt, 6, s, &vcnt));
   CBX_TRY(dalloc(b_ucnt, 2, s, &ucnt));
   CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
   CBX_HIP(hipMemsetAsync(ucnt, 0, 2 * sizeof(int), s));
@@ -1843,10 +1875,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const T* valB = (const T*)B->val;
 
   // --- validate both operands before any indexed access
-  if (!cfg.trusted_a)
-    k_validate<<<wave_grid(nzcA), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
-  if (!cfg.trusted_b)
-    k_validate<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
+  if (!cfg.trusted_a) CBX_TRY(launch_validate(A, 4, vcnt, err, s));
+  if (!cfg.trusted_b) CBX_TRY(launch_validate(B, 5, vcnt + 3, err, s));
   {
     int herr = 0;
     CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2064,8 +2094,11 @@ cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, h
   CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
   CBX_TRY(dalloc(b_err, 1, s, &err));
   CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
-  k_validate<<<wave_grid(nzcA), 256, 0, s>>>(A->cp, A->jc, A->ir, nzcA, nnzA, A->m, A->n, 4, err);
-  k_validate<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, B->jc, B->ir, nzcB, nnzB, B->m, B->n, 5, err);
+  DevBuf b_vcnt;
+  unsigned long long* vcnt;
+  CBX_TRY(dalloc(b_vcnt, 6, s, &vcnt));
+  CBX_TRY(launch_validate(A, 4, vcnt, err, s));
+  CBX_TRY(launch_validate(B, 5, vcnt + 3, err, s));
   int herr = 0;
   CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipStreamSynchronize(s));

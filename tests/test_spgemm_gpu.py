@@ -178,6 +178,30 @@ def test_invalid_input_rejected():
         gpu_spgemm(A, cb.gen_rmat(9, 4))  # inner dimensions differ
 
 
+def test_operand_validation_with_empty_columns():
+    """The product's operand validation accepts explicitly empty columns in cp
+    (a descent of rows where a column starts after an empty one is legal) and
+    still rejects a descent inside a column."""
+    m = 64
+    # columns 3, 7 (empty), 9 of a 64 x 16 matrix; column 9 starts below column 3's last row
+    cp = np.array([0, 3, 3, 6], np.int64)
+    jc = np.array([3, 7, 9], np.int64)
+    ir = np.array([10, 20, 30, 1, 2, 40], np.int64)
+    val = np.array([1, 2, 3, 4, 5, 6], np.int64)
+    A = cb.Dcsc(m, 16, cp, jc, ir, val)
+    B = cb.gen_rmat(4, 4)  # 16 x 16
+    got, _ = gpu_spgemm(A, B)
+    keep = cb.Dcsc(m, 16, np.array([0, 3, 6], np.int64), np.array([3, 9], np.int64), ir, val)
+    want, _ = oracle.spgemm(keep, B)
+    assert np.array_equal(got.ir, want.ir) and np.array_equal(got.val, want.val)
+    assert np.array_equal(got.cp, want.cp) and np.array_equal(got.jc, want.jc)
+    bad_ir = ir.copy()
+    bad_ir[4] = 0  # 1 then 0 inside column 9
+    with pytest.raises(cb.CbxError) as ei:
+        gpu_spgemm(cb.Dcsc(m, 16, cp, jc, bad_ir, val), B)
+    assert ei.value.status == cb.ERR_INVALID
+
+
 def test_device_structure_check_cases():
     """cbx_dcsc_check accepts a valid DCSC (rows descend only where a column
     starts) and rejects each malformation: rows out of order or repeated
