@@ -130,10 +130,18 @@ size_t cache_idle_bytes() {
 
 void release_cache() {
   Cache& c = cache();
-  std::lock_guard<std::mutex> g(c.mu);
-  for (auto& kv : c.idle) (void)hipFree(kv.second);
-  c.idle.clear();
-  c.idle_bytes = 0;
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    for (auto& kv : c.idle) (void)hipFree(kv.second);
+    c.idle.clear();
+    c.idle_bytes = 0;
+  }
+  // and the stream pool's reserve (scratch freed by finished calls), so that
+  // a large hipMalloc can use it
+  int dev = 0;
+  hipMemPool_t pool;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+    (void)hipMemPoolTrimTo(pool, 0);
 }
 
 }  // namespace cbx

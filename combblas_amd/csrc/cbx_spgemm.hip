@@ -109,6 +109,7 @@ struct Unit {
   int32_t blk_hi;  // one past the last block
   int32_t cnt;     // outputs in the unit
   int32_t off;     // offset of the unit's first output within the column
+  int32_t slot;    // the column's row-map slot (its workgroup in the symbolic launch)
 };
 
 // Segment of B entry e restricted to P tiles [t_lo, t_hi) (or the whole A
@@ -777,7 +778,8 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
                                                   int kmax_h, int rblk_h, int64_t heavy_thr,
                                                   int64_t* __restrict__ nnzcol, Unit* __restrict__ units_l,
                                                   int* __restrict__ n_l, int cap_l, Unit* __restrict__ units_h,
-                                                  int* __restrict__ n_h, int cap_h, int* __restrict__ err) {
+                                                  int* __restrict__ n_h, int cap_h, uint32_t* __restrict__ rmap,
+                                                  int64_t rm_cols, int* __restrict__ err) {
   constexpr int BMW = 32768;          // 2^20 rows per bitmap pass
   constexpr int BPP = BMW / kBRW;     // 256 blocks per pass
   __shared__ uint32_t bits[BMW];
@@ -814,6 +816,13 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
       });
     }
     __syncthreads();
+    // the column's row map (its presence bitmap, S blocks of kBRW words) for
+    // the numeric pass, which then need not rebuild it; kept for the first
+    // rm_cols columns of the launch (the only ones that can form large units)
+    if (rmap && blockIdx.x < rm_cols) {
+      uint32_t* dst = rmap + (int64_t)blockIdx.x * S * kBRW + (int64_t)t_lo * kBRW;
+      for (int i = threadIdx.x; i < nblk * kBRW; i += NT) dst[i] = bits[i];
+    }
     // per-block counts: one wave per block, two words per lane
     for (int b = wave_id(); b < nblk; b += NT / kWave) {
       const uint32_t* w = bits + b * kBRW;
@@ -859,6 +868,7 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
           su[nu].blk_hi = t_lo + lo + 1;
           su[nu].cnt = c;
           su[nu].off = unit_off;
+          su[nu].slot = (int32_t)blockIdx.x;
           unit_off += c;
           ++nu;
         }
@@ -1081,6 +1091,131 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
   // emit: values in rank order (coalesced); then the rows, staged in rank
   // order in the value array's LDS (every Acc is at least 4 bytes) so that
   // they too leave in full, coalesced lines
+  for (int i = threadIdx.x; i < cnt; i += NT) valC[out + i] = SR::out(vals[i]);
+  __syncthreads();
+  uint32_t* rstage = reinterpret_cast<uint32_t*>(vals);
+  for (int w = threadIdx.x; w < nw64; w += NT) {
+    uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+    int q = pre[w];
+    while (word) {
+      const int b = __ffsll((unsigned long long)word) - 1;
+      word &= word - 1;
+      if (q < cnt) rstage[q] = (uint32_t)(w * 64 + b);
+      ++q;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < cnt; i += NT) irC[out + i] = (int64_t)row0 + rstage[i];
+}
+
+// Numeric pass of the unit tier with the symbolic pass's row map: the unit's
+// presence bitmap is loaded (coalesced) instead of rebuilt, ranks come from
+// its prefix popcounts, and one sweep gathers every product once and
+// accumulates it at its rank (at its row for a unit whose rows fit the value
+// window).
+template <int K, class T, int NT, int U, int KMAX, int RBLK>
+__global__ __launch_bounds__(NT) void k_num_units_rm(const Unit* __restrict__ units, const uint32_t* __restrict__ rmap,
+                                                     const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+                                                     const T* __restrict__ valB, const int64_t* __restrict__ cpA,
+                                                     const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
+                                                     const int32_t* __restrict__ P, int S, int64_t rm_cols,
+                                                     const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
+                                                     T* __restrict__ valC, int* __restrict__ err) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int NW32 = RBLK * kBRW;
+  constexpr int NW64 = NW32 / 2;
+  constexpr int WPT = (NW64 + NT - 1) / NT;
+  __shared__ Acc vals[KMAX];
+  __shared__ uint32_t bits[NW32];
+  __shared__ uint16_t pre[NW64];
+  __shared__ SegStage<T, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  const Unit un = units[blockIdx.x];
+  const int col = un.col;
+  const int t_lo = un.blk_lo, t_hi = un.blk_hi;
+  const int32_t row0 = t_lo * kBR;
+  const int cnt = un.cnt;
+  const int64_t out = cpC[col] + un.off;
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  const bool direct = (t_hi - t_lo) * kBR <= KMAX;
+  if (cnt > KMAX || t_hi - t_lo > RBLK || t_lo < 0 || t_hi > S || un.slot < 0 || un.slot >= rm_cols) {
+    if (threadIdx.x == 0) atomicOr(err, 64);
+    return;
+  }
+  const int ninit = direct ? (t_hi - t_lo) * kBR : cnt;
+  for (int i = threadIdx.x; i < ninit; i += NT) vals[i] = SR::ident();
+  const int nw32 = (t_hi - t_lo) * kBRW;
+  const uint32_t* src = rmap + (int64_t)un.slot * S * kBRW + (int64_t)t_lo * kBRW;
+  for (int i = threadIdx.x; i < nw32; i += NT) bits[i] = src[i];
+  __syncthreads();
+  const int nw64 = nw32 / 2;
+  {
+    int local[WPT];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+      const int w = threadIdx.x * WPT + q;
+      const int c = w < nw64 ? __popc(bits[2 * w]) + __popc(bits[2 * w + 1]) : 0;
+      local[q] = sum;
+      sum += c;
+    }
+    long long total;
+    const int ex = (int)block_excl_scan_ll<NT>(sum, wsum, total);
+#pragma unroll
+    for (int q = 0; q < WPT; ++q) {
+      const int w = threadIdx.x * WPT + q;
+      if (w < nw64) pre[w] = (uint16_t)(ex + local[q]);
+    }
+    if (threadIdx.x == 0 && total != cnt) atomicOr(err, 128);
+  }
+  __syncthreads();
+  auto rank_of = [&](int32_t r) {
+    const int w = r >> 6, b = r & 63;
+    const uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+    return (int)pre[w] + __popcll(word & ((1ull << b) - 1ull));
+  };
+  auto accumulate = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
+    int32_t r[U];
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      r[u] = -1;
+      if (idx[u] >= 0) {
+        r[u] = rowsA[idx[u]] - row0;
+        a[u] = valA[idx[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r[u] < 0) continue;
+      const int q = direct ? r[u] : rank_of(r[u]);
+      if (q < (direct ? ninit : cnt)) SR::acc(&vals[q], SR::mul(a[u], bvs[u]));
+    }
+  };
+  for (int64_t eb = b0; eb < b1; eb += NT) {
+    int64_t base;
+    int len;
+    T bv;
+    load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+    int nseg;
+    const int tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+    visit_staged<T, NT, U>(st, nseg, tot, accumulate);
+  }
+  __syncthreads();
+  if (direct) {  // values sit at their rows: one thread per row, stores stay coalesced
+    for (int r = threadIdx.x; r < ninit; r += NT) {
+      if (!((bits[r >> 5] >> (r & 31)) & 1u)) continue;
+      const int q = rank_of(r);
+      if (q < cnt) {
+        irC[out + q] = (int64_t)row0 + r;
+        valC[out + q] = SR::out(vals[r]);
+      }
+    }
+    return;
+  }
+  // values in rank order (coalesced); then the rows, staged in rank order in
+  // the value array's LDS (every Acc is at least 4 bytes)
   for (int i = threadIdx.x; i < cnt; i += NT) valC[out + i] = SR::out(vals[i]);
   __syncthreads();
   uint32_t* rstage = reinterpret_cast<uint32_t*>(vals);
@@ -1607,6 +1742,7 @@ constexpr int kSymNT = 512;
 constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass (2^20 rows)
 // unit tier
 constexpr int kSymUnitNT = 1024;
+constexpr double kRowMapShare = 0.25;  // of free device memory, at most, for the unit tier's row maps
 // light units span at most 16 blocks: ~53 KB of LDS, three workgroups per CU
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 16;    // <= 4096 outputs, <= 64K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
@@ -1722,7 +1858,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
     CBX_HIP(hipMemsetAsync(hucnt, 0, 2 * sizeof(int), s));
     k_sym_units<kSymUnitNT, 4><<<(unsigned)nheavy, kSymUnitNT, 0, s>>>(
         hcols, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
-        units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, err);
+        units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, nullptr, 0, err);
     CBX_HIP(hipGetLastError());
   }
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
@@ -1973,6 +2109,26 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (!learned_done) {
   // --- symbolic
   int64_t cap_l = 0, cap_h = 0;
+  // row maps (presence bitmaps, 512 B per 4096 rows) of the columns that can
+  // form large units (flop above unit_split_nnz; they lead the flop order),
+  // when they fit a share of the free memory: the large units then load them
+  // instead of rebuilding them in a counting sweep.  From the caching
+  // allocator, so repeated products reuse the block.
+  struct RowMap {
+    uint32_t* p = nullptr;
+    ~RowMap() { dev_free(p); }
+  } rm;
+  uint32_t*& rmap = rm.p;
+  const int64_t rm_cols = std::min<int64_t>(n_heavy_max, n_dense);
+  if (use_units && rm_cols > 0) {
+    const int64_t rm_words = rm_cols * (int64_t)S * kBRW;
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) == hipSuccess &&
+        4.0 * (double)rm_words <= kRowMapShare * (double)(freeb + cache_idle_bytes()) &&
+        dev_alloc((void**)&rmap, sizeof(uint32_t) * rm_words) != CBX_OK)
+      rmap = nullptr;
+    (void)hipGetLastError();
+  }
   if (use_units && n_dense) {
     cap_l = std::max<int64_t>(1, std::min<int64_t>(n_dense * (int64_t)S, 1ll << 30));
     cap_h = std::max<int64_t>(1, std::min<int64_t>(n_heavy_max * (int64_t)S, 1ll << 30));
@@ -1980,7 +2136,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
     k_sym_units<kSymUnitNT, 4><<<(unsigned)n_dense, kSymUnitNT, 0, s>>>(
         order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
-        units_l, ucnt, (int)cap_l, units_h, ucnt + 1, (int)cap_h, err);
+        units_l, ucnt, (int)cap_l, units_h, ucnt + 1, (int)cap_h, rmap, rm_cols, err);
   } else if (n_dense) {
     k_sym_dense<kSymNT, kSymRBW><<<(unsigned)n_dense, kSymNT, 0, s>>>(order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, G,
                                                                        TR, nnzcol);
@@ -2005,10 +2161,22 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (symbolic_out) return CBX_OK;
 
   // --- output allocation (cached: repeated products reuse the arrays)
-  CBX_TRY(dev_alloc((void**)&C->cp, sizeof(int64_t) * (nzcB + 1)));
-  CBX_TRY(dev_alloc((void**)&C->jc, sizeof(int64_t) * std::max<int64_t>(nzcB, 1)));
-  CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * std::max<int64_t>(nnzC, 1)));
-  CBX_TRY(dev_alloc(&C->val, sizeof(T) * std::max<int64_t>(nnzC, 1)));
+  auto alloc_out = [&]() -> cbx_status {
+    if (!C->cp) CBX_TRY(dev_alloc((void**)&C->cp, sizeof(int64_t) * (nzcB + 1)));
+    if (!C->jc) CBX_TRY(dev_alloc((void**)&C->jc, sizeof(int64_t) * std::max<int64_t>(nzcB, 1)));
+    if (!C->ir) CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * std::max<int64_t>(nnzC, 1)));
+    if (!C->val) CBX_TRY(dev_alloc(&C->val, sizeof(T) * std::max<int64_t>(nnzC, 1)));
+    return CBX_OK;
+  };
+  cbx_status ast = alloc_out();
+  if (ast != CBX_OK && rmap) {  // give the row maps back and use the two-sweep numeric pass
+    CBX_HIP(hipStreamSynchronize(s));
+    dev_free(rmap);
+    rmap = nullptr;
+    release_cache();
+    ast = alloc_out();
+  }
+  CBX_TRY(ast);
   C->nnz = nnzC;
   tm.mark(3, s);
 
@@ -2016,7 +2184,14 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32), CBX_ERR_LIMIT,
                 "cbx_spgemm: too many units for a single launch");
   if (use_units) {
-    if (hu[1])
+    // large units: one sweep over the row map when it was kept (their
+    // products exceed what the two-sweep kernel holds in registers); small
+    // units keep the two-sweep kernel, whose second sweep mostly replays
+    // registers
+    if (hu[1] && rmap)
+      k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+          units_h, rmap, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, rm_cols, cpC, C->ir, (T*)C->val, err);
+    else if (hu[1])
       k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
           units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err);
     if (hu[0])
