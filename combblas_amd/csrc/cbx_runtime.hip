@@ -317,7 +317,7 @@ extern "C" cbx_status cbx_spgemm_set_predictor(const float* params) {
 
 // ------------------------------------------------------------- tunables
 static std::mutex g_config_mu;
-static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f};
+static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f};
 
 SpgemmConfig cbx::current_config() {
   SpgemmConfig c;
@@ -330,6 +330,7 @@ SpgemmConfig cbx::current_config() {
     c.unit_split_nnz = g_config.unit_split_nnz;
     c.learned_target = g_config.learned_target;
     c.capacity_margin = g_config.capacity_margin;
+    c.row_map_share = g_config.row_map_share;
   }
   c.predictor = predictor_params();
   return c;
@@ -357,6 +358,8 @@ extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
                 "cbx_spgemm_set_config: learned_target must be in (0.05, 4]");
   CBX_RETURN_IF(!(cfg->capacity_margin >= 0.0f && cfg->capacity_margin <= 4.0f), CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: capacity_margin must be in [0, 4]");
+  CBX_RETURN_IF(!(cfg->row_map_share >= 0.0f && cfg->row_map_share <= 1.0f), CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: row_map_share must be in [0, 1]");
   std::lock_guard<std::mutex> g(g_config_mu);
   g_config = *cfg;
   return CBX_OK;

@@ -34,6 +34,8 @@ struct SpgemmConfig {
   // are valid by construction and skip the device validation pass.
   bool trusted_a = false, trusted_b = false;
   float capacity_margin = 0.15f;
+  // Two-phase mode: share of the free memory the large units' row maps may take (0: none).
+  float row_map_share = 0.25f;
 };
 
 // The process-wide tunables (cbx_spgemm_set_config) with the installed

@@ -1742,7 +1742,6 @@ constexpr int kSymNT = 512;
 constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass (2^20 rows)
 // unit tier
 constexpr int kSymUnitNT = 1024;
-constexpr double kRowMapShare = 0.25;  // of free device memory, at most, for the unit tier's row maps
 // light units span at most 16 blocks: ~53 KB of LDS, three workgroups per CU
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 16;    // <= 4096 outputs, <= 64K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
@@ -2113,18 +2112,19 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   // form large units (flop above unit_split_nnz; they lead the flop order),
   // when they fit a share of the free memory: the large units then load them
   // instead of rebuilding them in a counting sweep.  From the caching
-  // allocator, so repeated products reuse the block.
+  // allocator, so repeated products reuse the block.  (cfg.row_map_share of
+  // the free memory at most; 0 turns them off.)
   struct RowMap {
     uint32_t* p = nullptr;
     ~RowMap() { dev_free(p); }
   } rm;
   uint32_t*& rmap = rm.p;
   const int64_t rm_cols = std::min<int64_t>(n_heavy_max, n_dense);
-  if (use_units && rm_cols > 0) {
+  if (use_units && rm_cols > 0 && cfg.row_map_share > 0.0f) {
     const int64_t rm_words = rm_cols * (int64_t)S * kBRW;
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) == hipSuccess &&
-        4.0 * (double)rm_words <= kRowMapShare * (double)(freeb + cache_idle_bytes()) &&
+        4.0 * (double)rm_words <= (double)cfg.row_map_share * (double)(freeb + cache_idle_bytes()) &&
         dev_alloc((void**)&rmap, sizeof(uint32_t) * rm_words) != CBX_OK)
       rmap = nullptr;
     (void)hipGetLastError();

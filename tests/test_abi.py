@@ -62,12 +62,13 @@ def test_tunables_validated_without_gpu():
     base = cb.get_config()
     try:
         assert base["unit_split_nnz"] == 131072 and base["heavy_nnz"] == 65536
+        assert abs(base["row_map_share"] - 0.25) < 1e-6
         got = cb.set_config(hash_max_flop=2048, heavy_nnz=131072, unit_split_nnz=262144)
         assert got["hash_max_flop"] == 2048 and cb.get_config()["heavy_nnz"] == 131072
         assert cb.get_config()["unit_split_nnz"] == 262144
         for bad in ({"tiny_flop": 500}, {"hash_mid_flop": 2000}, {"hash_max_flop": 8192}, {"heavy_nnz": 10},
                     {"unit_split_nnz": 100}, {"learned_target": 0.0}, {"learned_target": 5.0},
-                    {"capacity_margin": -1.0}):
+                    {"capacity_margin": -1.0}, {"row_map_share": 1.5}, {"row_map_share": -0.1}):
             with pytest.raises(cb.CbxError):
                 cb.set_config(**bad)
         with pytest.raises(ValueError):

@@ -178,6 +178,26 @@ def test_invalid_input_rejected():
         gpu_spgemm(A, cb.gen_rmat(9, 4))  # inner dimensions differ
 
 
+@pytest.mark.parametrize("split,share", [(4096, 0.25), (4096, 0.0), (131072, 0.25)])
+def test_large_units_with_and_without_row_maps(split, share):
+    """Columns above unit_split_nnz outputs take the 16384-output units: with
+    the symbolic pass's row maps (share > 0) they are computed in one sweep,
+    without them (share 0) in the counting + accumulating sweeps.  Both are
+    bit-exact against the oracle; the low split makes large units at s16."""
+    A = cb.gen_rmat(16, 16)
+    base = cb.get_config()
+    try:
+        cb.set_config(unit_split_nnz=split, row_map_share=share)
+        got, st = gpu_spgemm(A, A)
+    finally:
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, A)
+    if split == 4096:
+        assert st["units_heavy"] > 0
+    assert np.array_equal(got.cp, want.cp) and np.array_equal(got.jc, want.jc)
+    assert np.array_equal(got.ir, want.ir) and np.array_equal(got.val, want.val)
+
+
 def test_operand_validation_with_empty_columns():
     """The product's operand validation accepts explicitly empty columns in cp
     (a descent of rows where a column starts after an empty one is legal) and
