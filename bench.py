@@ -267,6 +267,45 @@ def bench_spmv(cb, scale: int, ef: int, reps: int = 20) -> dict:
             "gbps_algorithmic": round(alg / (ms * 1e-3) / 1e9, 1), "exact_after_all_reps": exact}
 
 
+def bench_merge(cb, scale: int = 17, ef: int = 16, nstages: int = 4, reps: int = 3) -> dict:
+    """SUMMA's stage merge (MultiwayMerge<PlusTimes<int64>>) on one GPU: the
+    local products of nstages stages of R-MAT A*A, A(:, K_s) * B(K_s, :),
+    merged on the device; the merged product is checked against A*A with the
+    full-size column checksums outside the timed region."""
+    from combblas_amd.checks import product_matches
+    A = cb.gen_rmat(scale, ef)
+    cuts = np.linspace(0, A.n, nstages + 1).astype(np.int64)
+    parts = []
+    for s in range(nstages):
+        As = A.column_range(int(cuts[s]), int(cuts[s + 1])).to_device()
+        Bs = A.row_range(int(cuts[s]), int(cuts[s + 1])).to_device()
+        C, _ = cb.spgemm(As, Bs)
+        parts.append(C)
+        As.release()
+        Bs.release()
+    M = cb.merge(parts)  # warm-up
+    M.release()
+    cb.synchronize()
+    t = time.perf_counter()
+    for i in range(reps):
+        M = cb.merge(parts)
+        if i + 1 < reps:
+            M.release()
+    cb.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / reps
+    exact = product_matches(M, A, A)
+    nnz_in = sum(p.nnz for p in parts)
+    alg = sum(16 * p.nnz + 16 * p.nzc + 8 for p in parts) + 16 * M.nnz + 16 * M.nzc + 8
+    out = {"workload": f"stage merge of {nstages} SUMMA stage products of R-MAT s{scale} ef{ef} A*A, "
+                       "PlusTimes<int64>, DCSC in HBM",
+           "ms_per_merge": round(ms, 3), "nnz_in": nnz_in, "nnz_out": M.nnz, "alg_bytes": alg,
+           "gbps_algorithmic": round(alg / (ms * 1e-3) / 1e9, 1), "merged_equals_product": exact}
+    M.release()
+    for p in parts:
+        p.release()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,7 +316,7 @@ def main() -> int:
     ap.add_argument("--mode", default="both", choices=["both", "two_phase", "learned"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
     ap.add_argument("--no-extra", action="store_true", help="skip the BFS, MCL and Galerkin workloads")
-    ap.add_argument("--only", choices=["bfs", "mcl", "galerkin", "spmv"], default=None,
+    ap.add_argument("--only", choices=["bfs", "mcl", "galerkin", "spmv", "merge"], default=None,
                     help="run only this workload and print its JSON (for profiling)")
     ap.add_argument("--copy-gib", type=float, default=4.0,
                     help="size of a device-to-device copy timed before the workload: the achievable HBM rate, and "
@@ -296,7 +335,8 @@ def main() -> int:
     if args.only:
         fn = {"bfs": lambda: bench_bfs(cb, args.scale, args.edgefactor), "mcl": lambda: bench_mcl(cb),
               "galerkin": lambda: bench_galerkin(cb),
-              "spmv": lambda: bench_spmv(cb, args.scale, args.edgefactor)}[args.only]
+              "spmv": lambda: bench_spmv(cb, args.scale, args.edgefactor),
+              "merge": lambda: bench_merge(cb)}[args.only]
         print(json.dumps({args.only: fn()}))
         return 0
     copy = None
@@ -416,7 +456,8 @@ def main() -> int:
     if not args.no_extra:
         for name, fn in (("bfs", lambda: bench_bfs(cb, args.scale, args.edgefactor)),
                          ("mcl", lambda: bench_mcl(cb)), ("galerkin", lambda: bench_galerkin(cb)),
-                         ("spmv", lambda: bench_spmv(cb, args.scale, args.edgefactor))):
+                         ("spmv", lambda: bench_spmv(cb, args.scale, args.edgefactor)),
+                         ("merge", lambda: bench_merge(cb))):
             log(f"{name} ...")
             try:
                 out[name] = fn()

@@ -125,6 +125,8 @@ def lib() -> C.CDLL:
         L.cbx_gen_aggregation.argtypes = [C.c_int32, C.c_int32, P(cbx_dcsc)]
         L.cbx_gen_aggregation.restype = C.c_int32
         L.cbx_dcsc_transpose.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p]
+        L.cbx_dcsc_merge.argtypes = [C.c_int32, P(cbx_dcsc), C.c_int32, P(cbx_dcsc), C.c_void_p]
+        L.cbx_dcsc_merge.restype = C.c_int32
         L.cbx_dcsc_transpose.restype = C.c_int32
         L.cbx_spgemm_batched.argtypes = [C.c_int64, P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, P(cbx_dcsc),
                                          P(cbx_spgemm_stats), C.c_void_p]
@@ -257,6 +259,24 @@ class Dcsc:
     def col_ids(self) -> np.ndarray:
         """Column index of every stored entry."""
         return np.repeat(self.jc, np.diff(self.cp))
+
+    def column_range(self, lo: int, hi: int) -> "Dcsc":
+        """The columns lo..hi-1 at the full shape (others empty): A(:, K) of a SUMMA stage."""
+        keep = (self.jc >= lo) & (self.jc < hi)
+        cnt = np.diff(self.cp)[keep]
+        first = self.cp[:-1][keep]
+        idx = (np.repeat(first - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt) + np.arange(cnt.sum())
+               if cnt.size else np.zeros(0, np.int64))
+        return Dcsc(self.m, self.n, np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64), self.jc[keep].copy(),
+                    self.ir[idx].copy(), self.val[idx].copy(), self.dtype)
+
+    def row_range(self, lo: int, hi: int) -> "Dcsc":
+        """The rows lo..hi-1 at the full shape (others empty): B(K, :) of a SUMMA stage."""
+        sel = (self.ir >= lo) & (self.ir < hi)
+        cols = self.col_ids()[sel]
+        jc, cnt = np.unique(cols, return_counts=True)
+        return Dcsc(self.m, self.n, np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64), jc.astype(np.int64),
+                    self.ir[sel].copy(), self.val[sel].copy(), self.dtype)
 
     def to_dense(self) -> np.ndarray:
         d = np.zeros((self.m, self.n), dtype=NP_DTYPES[self.dtype])
@@ -452,6 +472,15 @@ def transpose(X: DeviceDcsc, stream: int | None = None) -> DeviceDcsc:
     """SpDCCols::Transpose on the device."""
     out = cbx_dcsc()
     _check(lib().cbx_dcsc_transpose(C.byref(X.s), C.byref(out), stream))
+    return DeviceDcsc(out)
+
+
+def merge(parts: list[DeviceDcsc], semiring: str = "plus_times", stream: int | None = None) -> DeviceDcsc:
+    """MultiwayMerge<SR> on the device: the semiring sum of same-shape parts (SUMMA's stage merge)."""
+    if not parts:
+        raise ValueError("merge needs at least one part")
+    out = cbx_dcsc()
+    _check(lib().cbx_dcsc_merge(len(parts), _desc_array(parts), SEMIRINGS[semiring], C.byref(out), stream))
     return DeviceDcsc(out)
 
 

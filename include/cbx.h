@@ -392,6 +392,16 @@ cbx_status cbx_mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, double hard_thre
  * (rows sorted within columns). */
 cbx_status cbx_dcsc_transpose(const cbx_dcsc* X, cbx_dcsc* out, void* stream);
 
+/* Stage merge: MultiwayMerge<SR>(parts, m, n) (MultiwayMerge.h:428-541, and
+ * MultiwayMergeHash with sorted=true, :553) followed by SpDCCols(SpTuples)
+ * (SpDCCols.cpp:109-184).  out = the sum under the semiring's add of count
+ * device DCSC parts of one shape and dtype (an array of count structs), as a
+ * new device DCSC; entries with equal (row, col) are combined, none is
+ * dropped.  Integer semirings are exact; floating-point sums of duplicates
+ * are formed in part order (the reference's heap order may round
+ * differently).  At most 2^31 - 1 entries over all parts. */
+cbx_status cbx_dcsc_merge(int32_t count, const cbx_dcsc* parts, cbx_semiring semiring, cbx_dcsc* out, void* stream);
+
 /* count independent products C[i] = A[i]*B[i] (arrays of count device
  * descriptors, one dtype) computed together: the batch is packed into one
  * block-diagonal product, so every SpGEMM kernel is launched once for the

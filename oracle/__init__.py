@@ -100,6 +100,8 @@ def _load_ref() -> C.CDLL:
         L.ref_spmv_dense.restype = C.c_int32
         L.ref_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc)]
         L.ref_mcl_prune.restype = C.c_int32
+        L.ref_multiway_merge.argtypes = [C.c_int32, P(cbx_dcsc), C.c_int32, P(cbx_dcsc)]
+        L.ref_multiway_merge.restype = C.c_int32
         L.ref_free.argtypes = [P(cbx_dcsc)]
         L.ref_free.restype = None
         _ref = L
@@ -187,6 +189,60 @@ def ref_spgemm(A: Dcsc, B: Dcsc, semiring: str = "plus_times", mode: str = "hybr
     if st != 0:
         raise RuntimeError(f"ref_spgemm failed with status {st}")
     return Dcsc.from_struct(out, L.ref_free)
+
+
+def ref_merge(parts: list[Dcsc], semiring: str = "plus_times") -> Dcsc:
+    """The reference's MultiwayMerge<SR> (MultiwayMerge.h:428) over the parts'
+    tuples, returned as DCSC (only where oracle/_ref was built)."""
+    L = _load_ref()
+    keep = [p._cstruct() for p in parts]
+    arr = (cbx_dcsc * len(parts))(*[k[0] for k in keep])
+    out = cbx_dcsc()
+    st = L.ref_multiway_merge(len(parts), arr, SEMIRINGS[semiring], C.byref(out))
+    if st != 0:
+        raise RuntimeError(f"ref_multiway_merge failed with status {st}")
+    return Dcsc.from_struct(out, L.ref_free)
+
+
+def merge(parts: list[Dcsc], semiring: str = "plus_times") -> Dcsc:
+    """Restatement of MultiwayMerge<SR> (MultiwayMerge.h:428-541, SerialMerge
+    :193-250): the parts' entries in column-major order, equal (row, col)
+    combined with SR::add, nothing dropped (explicit zeros stay).  Ties are
+    added in part order; the reference pops them in heap order, which changes
+    only floating-point rounding."""
+    if not parts:
+        raise ValueError("merge needs at least one part")
+    m, n, dt = parts[0].m, parts[0].n, parts[0].dtype
+    for p in parts:
+        if (p.m, p.n, p.dtype) != (m, n, dt):
+            raise ValueError("parts differ in shape or dtype")
+    rows = np.concatenate([p.ir for p in parts]).astype(np.int64)
+    cols = np.concatenate([p.col_ids() for p in parts]).astype(np.int64)
+    vals = np.concatenate([p.val for p in parts])
+    order = np.lexsort((rows, cols))  # stable: ties keep part order
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    if rows.size == 0:
+        return Dcsc(m, n, np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0, np.int64), vals, dt)
+    head = np.ones(rows.size, bool)
+    head[1:] = (rows[1:] != rows[:-1]) | (cols[1:] != cols[:-1])
+    starts = np.nonzero(head)[0]
+    if dt == "bool":
+        v = np.logical_or.reduceat(vals.astype(bool), starts).astype(vals.dtype)
+    elif semiring == "plus_times":
+        if dt == "int64":  # two's-complement wrap, like the reference's int64 adds
+            v = np.add.reduceat(vals.view(np.uint64), starts).view(np.int64)
+        else:
+            v = np.add.reduceat(vals, starts)
+    elif semiring == "min_plus":
+        v = np.minimum.reduceat(vals, starts)
+    elif semiring == "select_max":
+        v = np.maximum.reduceat(vals, starts)
+    else:
+        raise ValueError(semiring)
+    r, c = rows[starts], cols[starts]
+    jc, cnt = np.unique(c, return_counts=True)
+    cp = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+    return Dcsc(m, n, cp, jc.astype(np.int64), r, v, dt)
 
 
 def ref_symbolic(A: Dcsc, B: Dcsc) -> tuple[np.ndarray, np.ndarray]:

@@ -124,6 +124,25 @@ void export_dcsc(SpDCCols<int64_t, NT>& M, cbx_dtype dt, cbx_dcsc* C) {
   std::memcpy(v, d->numx, sizeof(NT) * nnz);
 }
 
+// MultiwayMerge<SR> (MultiwayMerge.h:428) of the parts' column-sorted tuples.
+template <class SR, class NT, class ST>
+int merge(int32_t count, const cbx_dcsc* parts, cbx_dcsc* C) {
+  ensure_mpi();
+  std::vector<SpDCCols<int64_t, NT>*> mats;
+  std::vector<SpTuples<int64_t, NT>*> tups;
+  for (int32_t i = 0; i < count; ++i) {
+    mats.push_back(make_mat<NT, ST>(parts[i]));
+    tups.push_back(new SpTuples<int64_t, NT>(*mats.back()));
+  }
+  const int64_t m = count ? parts[0].m : 0, n = count ? parts[0].n : 0;
+  SpTuples<int64_t, NT>* T = MultiwayMerge<SR>(tups, m, n, false);
+  export_mat<NT, ST>(*T, m, n, count ? parts[0].dtype : CBX_I64, C);
+  if (!(count == 1 && T == tups[0])) delete T;
+  for (auto* t : tups) delete t;
+  for (auto* x : mats) delete x;
+  return 0;
+}
+
 // MCLPruneRecoverySelect (ParFriends.h:185) on a 1x1 process grid.
 template <class NT>
 int mcl_prune(const cbx_dcsc* A, double thr, int64_t select, int64_t recover, double pct, cbx_dcsc* C) {
@@ -169,6 +188,35 @@ int ref_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semiring, int32_t m
         case CBX_I64: return run<SelectMaxSRing<int64_t, int64_t>, int64_t, int64_t>(A, B, mode, C);
         case CBX_F64: return run<SelectMaxSRing<double, double>, double, double>(A, B, mode, C);
         case CBX_F32: return run<SelectMaxSRing<float, float>, float, float>(A, B, mode, C);
+      }
+      break;
+  }
+  return -1;
+}
+
+int ref_multiway_merge(int32_t count, const cbx_dcsc* parts, int32_t semiring, cbx_dcsc* C) {
+  if (count < 1) return CBX_ERR_INVALID;
+  switch (semiring) {
+    case CBX_SR_PLUS_TIMES:
+      switch (parts[0].dtype) {
+        case CBX_I64: return merge<PlusTimesSRing<int64_t, int64_t>, int64_t, int64_t>(count, parts, C);
+        case CBX_F64: return merge<PlusTimesSRing<double, double>, double, double>(count, parts, C);
+        case CBX_F32: return merge<PlusTimesSRing<float, float>, float, float>(count, parts, C);
+        case CBX_BOOL: return merge<PlusTimesSRing<bool, bool>, bool, uint8_t>(count, parts, C);
+      }
+      break;
+    case CBX_SR_MIN_PLUS:
+      switch (parts[0].dtype) {
+        case CBX_I64: return merge<MinPlusSRing<int64_t, int64_t>, int64_t, int64_t>(count, parts, C);
+        case CBX_F64: return merge<MinPlusSRing<double, double>, double, double>(count, parts, C);
+        case CBX_F32: return merge<MinPlusSRing<float, float>, float, float>(count, parts, C);
+      }
+      break;
+    case CBX_SR_SELECT_MAX:
+      switch (parts[0].dtype) {
+        case CBX_I64: return merge<SelectMaxSRing<int64_t, int64_t>, int64_t, int64_t>(count, parts, C);
+        case CBX_F64: return merge<SelectMaxSRing<double, double>, double, double>(count, parts, C);
+        case CBX_F32: return merge<SelectMaxSRing<float, float>, float, float>(count, parts, C);
       }
       break;
   }

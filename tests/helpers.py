@@ -74,3 +74,13 @@ def mcl_columns(seed: int, ncols: int = 240, m: int = 6000, dtype: str = "float3
 # every branch on small columns.
 MCL_PARAMS = [(1e-4, 1000, 0, 0.9), (1e-4, 1100, 1400, 0.9), (1e-3, 50, 100, 0.5), (0.0, 10, 0, 0.9),
               (1e-4, 0, 300, 0.95), (2e-4, 200, 0, 0.0)]
+
+
+
+def stage_slices(A, B, nstages: int):
+    """(A(:, K_s), B(K_s, :)) for nstages contiguous ranges K_s of the inner
+    dimension, at the full shape: the operands of SUMMA's stages on one process."""
+    import numpy as _np
+    cuts = _np.linspace(0, A.n, nstages + 1).astype(_np.int64)
+    return [(A.column_range(int(cuts[s]), int(cuts[s + 1])), B.row_range(int(cuts[s]), int(cuts[s + 1])))
+            for s in range(nstages)]

@@ -1,0 +1,53 @@
+"""Device stage merge (cbx_dcsc_merge, MultiwayMerge<SR>) against the oracle:
+integer, bool, min-plus and select-max bit-exact, FP within 1e-12 relative
+(duplicates summed in another order); and the SUMMA identity on the device."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import combblas_amd as cb
+import oracle
+from helpers import stage_slices
+from test_merge_oracle import random_part, same
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("semiring,dtype", [("plus_times", "int64"), ("min_plus", "int64"), ("select_max", "int64"),
+                                            ("plus_times", "bool"), ("plus_times", "float64"),
+                                            ("min_plus", "float64"), ("select_max", "float64")])
+def test_merge_matches_oracle(semiring, dtype):
+    rng = np.random.default_rng(12)
+    for k in (1, 3, 8):
+        parts = [random_part(rng, 5000, 700, 30000, dtype) for _ in range(k)]
+        parts.append(random_part(rng, 5000, 700, 0, dtype))
+        dparts = [p.to_device() for p in parts]
+        got = cb.merge(dparts, semiring).to_host()
+        want = oracle.merge(parts, semiring)
+        same(got, want, rtol=1e-12 if (dtype == "float64" and semiring == "plus_times") else 0.0)
+
+
+def test_stage_products_merge_to_the_product_on_device():
+    """Local products of 5 SUMMA stages, merged on the device, equal the
+    product computed at once (PlusTimes<int64>, R-MAT s14)."""
+    A = cb.gen_rmat(14, 8, seed=7)
+    dA = A.to_device()
+    full, _ = cb.spgemm(dA, dA)
+    parts = []
+    for As, Bs in stage_slices(A, A, 5):
+        C, _ = cb.spgemm(As.to_device(), Bs.to_device())
+        parts.append(C)
+    got = cb.merge(parts).to_host()
+    same(got, full.to_host())
+
+
+def test_merge_edge_cases():
+    rng = np.random.default_rng(2)
+    empty = [random_part(rng, 50, 40, 0).to_device() for _ in range(3)]
+    got = cb.merge(empty).to_host()
+    assert got.nnz == 0 and got.nzc == 0 and (got.m, got.n) == (50, 40)
+    with pytest.raises(cb.CbxError):
+        cb.merge([random_part(rng, 50, 40, 10).to_device(), random_part(rng, 50, 41, 10).to_device()])
+    with pytest.raises(ValueError):
+        cb.merge([])
