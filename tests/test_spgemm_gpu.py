@@ -198,6 +198,29 @@ def test_large_units_with_and_without_row_maps(split, share):
     assert np.array_equal(got.ir, want.ir) and np.array_equal(got.val, want.val)
 
 
+@pytest.mark.parametrize("share", [0.25, 0.0])
+def test_large_units_on_tall_matrices(share):
+    """m above 2^20 rows: the symbolic pass runs several bitmap passes and
+    writes the row map pass by pass; large units (split lowered to 4096) read
+    slices of it.  Bit-exact against the oracle with and without row maps."""
+    rng = np.random.default_rng(31)
+    m, n, k = (1 << 21) + 5000, 2000, 300
+    key = np.unique(rng.integers(0, n, 400_000) * m + rng.integers(0, m, 400_000))
+    A = cb.Dcsc.from_coo(m, n, key % m, key // m, rng.integers(-4, 5, key.size).astype(np.int64), "int64")
+    keyb = np.unique(rng.integers(0, k, 15_000) * n + rng.integers(0, n, 15_000))
+    B = cb.Dcsc.from_coo(n, k, keyb % n, keyb // n, rng.integers(-4, 5, keyb.size).astype(np.int64), "int64")
+    base = cb.get_config()
+    try:
+        cb.set_config(unit_split_nnz=4096, row_map_share=share)
+        got, st = gpu_spgemm(A, B)
+    finally:
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, B)
+    assert st["units_heavy"] > 0
+    assert np.array_equal(got.cp, want.cp) and np.array_equal(got.jc, want.jc)
+    assert np.array_equal(got.ir, want.ir) and np.array_equal(got.val, want.val)
+
+
 def test_operand_validation_with_empty_columns():
     """The product's operand validation accepts explicitly empty columns in cp
     (a descent of rows where a column starts after an empty one is legal) and
