@@ -127,6 +127,9 @@ def lib() -> C.CDLL:
         L.cbx_dcsc_transpose.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_void_p]
         L.cbx_dcsc_merge.argtypes = [C.c_int32, P(cbx_dcsc), C.c_int32, P(cbx_dcsc), C.c_void_p]
         L.cbx_dcsc_merge.restype = C.c_int32
+        L.cbx_dcsc_from_coo.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                        C.c_int32, C.c_int32, P(cbx_dcsc), C.c_void_p]
+        L.cbx_dcsc_from_coo.restype = C.c_int32
         L.cbx_dcsc_transpose.restype = C.c_int32
         L.cbx_spgemm_batched.argtypes = [C.c_int64, P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_int32, P(cbx_dcsc),
                                          P(cbx_spgemm_stats), C.c_void_p]
@@ -481,6 +484,20 @@ def merge(parts: list[DeviceDcsc], semiring: str = "plus_times", stream: int | N
         raise ValueError("merge needs at least one part")
     out = cbx_dcsc()
     _check(lib().cbx_dcsc_merge(len(parts), _desc_array(parts), SEMIRINGS[semiring], C.byref(out), stream))
+    return DeviceDcsc(out)
+
+
+def from_coo_device(m: int, n: int, rows: "DeviceArray", cols: "DeviceArray", vals: "DeviceArray | None" = None,
+                    dtype: str = "int64", semiring: str = "plus_times", remove_loops: bool = False,
+                    stream: int | None = None) -> DeviceDcsc:
+    """Edge list to DCSC on the device (SpTuples' Graph500 constructor / RemoveDuplicates):
+    duplicates combined with the semiring's add, optional self-loop removal, vals None = ones."""
+    if rows.n != cols.n or (vals is not None and vals.n != rows.n):
+        raise ValueError("rows, cols and vals must have the same length")
+    out = cbx_dcsc()
+    _check(lib().cbx_dcsc_from_coo(int(m), int(n), rows.ptr, cols.ptr, vals.ptr if vals is not None else None,
+                                   rows.n, DTYPES[dtype], SEMIRINGS[semiring], int(bool(remove_loops)),
+                                   C.byref(out), stream))
     return DeviceDcsc(out)
 
 

@@ -70,6 +70,74 @@ __global__ void k_merge_close(int64_t* __restrict__ cp, int64_t nzc, int64_t nnz
   if (threadIdx.x == 0 && blockIdx.x == 0) cp[nzc] = nnz;  // after the exclusive scan of the column counts
 }
 
+using ColIt = hipcub::TransformInputIterator<int64_t, KeyCol, const uint64_t*>;
+
+// Temporary bytes of reduce_assemble for up to N sorted entries.
+template <int K, class T>
+hipError_t assemble_tmp_bytes(int N, hipStream_t s, size_t& bytes) {
+  size_t t2 = 0, t3 = 0, t4 = 0;
+  uint64_t* k = nullptr;
+  T* v = nullptr;
+  int64_t* c = nullptr;
+  hipError_t e = hipcub::DeviceReduce::ReduceByKey(nullptr, t2, k, k, v, v, c, SemAdd<K, T>(), N, s);
+  if (e == hipSuccess) e = hipcub::DeviceRunLengthEncode::Encode(nullptr, t3, ColIt(k, KeyCol()), c, c, c, N, s);
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, t4, c, c, N, s);
+  bytes = std::max(t2, std::max(t3, t4));
+  return e;
+}
+
+// From N (col, row)-sorted keys and values to a new device DCSC in *out:
+// equal keys combined with the semiring's add (into ukeys/uvals, N entries
+// each), columns from the run lengths of the keys' column halves (counts: N
+// entries; nruns: 2).  out's shape and dtype are set by the caller.
+template <int K, class T>
+cbx_status reduce_assemble(const uint64_t* keys, const T* vals, int N, uint64_t* ukeys, T* uvals, int64_t* nruns,
+                           int64_t* counts, void* tmp, size_t tb, cbx_dcsc* out, hipStream_t s, const char* who) {
+  hipError_t e = hipSuccess;
+  int64_t nnz = 0;  // no entries (every one a dropped loop): an empty matrix
+  if (N > 0) {
+    e = hipcub::DeviceReduce::ReduceByKey(tmp, tb, keys, ukeys, vals, uvals, nruns, SemAdd<K, T>(), N, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&nnz, nruns, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) return set_error(CBX_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
+  cbx_status st = CBX_OK;
+  if ((st = dev_alloc((void**)&out->jc, sizeof(int64_t) * std::max<int64_t>(nnz, 1))) != CBX_OK ||
+      (st = dev_alloc((void**)&out->ir, sizeof(int64_t) * std::max<int64_t>(nnz, 1))) != CBX_OK ||
+      (st = dev_alloc(&out->val, sizeof(T) * std::max<int64_t>(nnz, 1))) != CBX_OK) {
+    cbx_dcsc_free(out);
+    return st;
+  }
+  int64_t nzc = 0;
+  if (nnz > 0) {
+    e = hipcub::DeviceRunLengthEncode::Encode(tmp, tb, ColIt(ukeys, KeyCol()), out->jc, counts, nruns + 1, (int)nnz,
+                                              s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&nzc, nruns + 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e == hipSuccess && (st = dev_alloc((void**)&out->cp, sizeof(int64_t) * (nzc + 1))) != CBX_OK) {
+    cbx_dcsc_free(out);
+    return st;
+  }
+  if (e == hipSuccess && nzc > 0) e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, out->cp, (int)nzc, s);
+  if (e == hipSuccess) {
+    k_merge_close<<<1, 64, 0, s>>>(out->cp, nzc, nnz);
+    if (nnz > 0) {
+      k_merge_rows<<<(unsigned)((nnz + kMergeNT - 1) / kMergeNT), kMergeNT, 0, s>>>(ukeys, nnz, out->ir);
+      e = hipMemcpyAsync(out->val, uvals, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s);
+    }
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    cbx_dcsc_free(out);
+    return set_error(CBX_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
+  }
+  out->nnz = nnz;
+  out->nzc = nzc;
+  return CBX_OK;
+}
+
 template <int K, class T>
 cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipStream_t s) {
   const int64_t m = parts[0].m, n = parts[0].n;
@@ -103,14 +171,10 @@ cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipS
   T *v0 = nullptr, *v1 = nullptr;
   int64_t *nruns = nullptr, *counts = nullptr;
   void* tmp = nullptr;
-  size_t tb = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+  size_t tb = 0, t1 = 0, t2 = 0;
   CBX_HIP(rocprim::merge(nullptr, t1, k0, k0, k1, v0, v0, v1, (size_t)N, (size_t)N, rocprim::less<uint64_t>(), s));
-  CBX_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, t2, k1, ukeys, v1, v0, nruns, SemAdd<K, T>(), N, s));
-  using ColIt = hipcub::TransformInputIterator<int64_t, KeyCol, const uint64_t*>;
-  CBX_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, t3, ColIt(ukeys, KeyCol()), (int64_t*)nullptr, counts,
-                                                nruns + 1, N, s));  // size query only (ukeys not allocated yet)
-  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, t4, counts, (int64_t*)nullptr, N, s));
-  tb = std::max(std::max(t1, t2), std::max(t3, t4));
+  CBX_HIP((assemble_tmp_bytes<K, T>(N, s, t2)));
+  tb = std::max(t1, t2);
   auto cleanup = [&]() {
     for (void* p : {(void*)k0, (void*)k1, (void*)ukeys, (void*)v0, (void*)v1, (void*)nruns, (void*)counts, tmp})
       if (p) (void)hipFreeAsync(p, s);
@@ -157,49 +221,98 @@ cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipS
     e = hipMemcpyAsync(k1, ksrc, sizeof(uint64_t) * N, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(v1, vsrc, sizeof(T) * N, hipMemcpyDeviceToDevice, s);
   }
+  if (e != hipSuccess) {
+    cleanup();
+    return set_error(CBX_ERR_HIP, "cbx_dcsc_merge: %s", hipGetErrorString(e));
+  }
   // distinct (col, row) keys and their combined values (values reuse v0)
-  if (e == hipSuccess) e = hipcub::DeviceReduce::ReduceByKey(tmp, tb, k1, ukeys, v1, v0, nruns, SemAdd<K, T>(), N, s);
-  int64_t nnz = 0;
-  if (e == hipSuccess) e = hipMemcpyAsync(&nnz, nruns, sizeof(int64_t), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    cleanup();
-    return set_error(CBX_ERR_HIP, "cbx_dcsc_merge: %s", hipGetErrorString(e));
-  }
-  // columns: run lengths of the keys' column halves
-  cbx_status st = CBX_OK;
-  if ((st = dev_alloc((void**)&out->jc, sizeof(int64_t) * std::max<int64_t>(nnz, 1))) != CBX_OK ||
-      (st = dev_alloc((void**)&out->ir, sizeof(int64_t) * std::max<int64_t>(nnz, 1))) != CBX_OK ||
-      (st = dev_alloc(&out->val, sizeof(T) * std::max<int64_t>(nnz, 1))) != CBX_OK) {
-    cleanup();
-    cbx_dcsc_free(out);
-    return st;
-  }
-  e = hipcub::DeviceRunLengthEncode::Encode(tmp, tb, ColIt(ukeys, KeyCol()), out->jc, counts, nruns + 1, (int)nnz, s);
-  int64_t nzc = 0;
-  if (e == hipSuccess) e = hipMemcpyAsync(&nzc, nruns + 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e == hipSuccess && (st = dev_alloc((void**)&out->cp, sizeof(int64_t) * (nzc + 1))) != CBX_OK) {
-    cleanup();
-    cbx_dcsc_free(out);
-    return st;
-  }
-  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, out->cp, (int)nzc, s);
-  if (e == hipSuccess) {
-    k_merge_close<<<1, 64, 0, s>>>(out->cp, nzc, nnz);
-    k_merge_rows<<<(unsigned)((nnz + kMergeNT - 1) / kMergeNT), kMergeNT, 0, s>>>(ukeys, nnz, out->ir);
-    e = hipMemcpyAsync(out->val, v0, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s);
-  }
-  if (e == hipSuccess) e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  const cbx_status st = reduce_assemble<K, T>(k1, v1, N, ukeys, v0, nruns, counts, tmp, tb, out, s, "cbx_dcsc_merge");
   cleanup();
-  if (e != hipSuccess) {
-    cbx_dcsc_free(out);
-    return set_error(CBX_ERR_HIP, "cbx_dcsc_merge: %s", hipGetErrorString(e));
+  return st;
+}
+
+// Edge-list input: keys of COO entries.  bad[0] counts entries out of range,
+// bad[1] the self loops dropped (their key is the sentinel (n << 32), past
+// every valid key, so they sort to the end).  vals may be null (value 1).
+template <class T>
+__global__ __launch_bounds__(kMergeNT) void k_coo_keys(const int64_t* __restrict__ rows,
+                                                       const int64_t* __restrict__ cols,
+                                                       const T* __restrict__ vals, int64_t nnz, int64_t m, int64_t n,
+                                                       int drop_loops, uint64_t* __restrict__ keys,
+                                                       T* __restrict__ out, unsigned long long* __restrict__ bad) {
+  long long oob = 0, loops = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += stride) {
+    const int64_t r = rows[i], c = cols[i];
+    const bool ok = r >= 0 && r < m && c >= 0 && c < n;
+    const bool loop = drop_loops && r == c;
+    oob += !ok;
+    loops += ok && loop;
+    keys[i] = ok && !loop ? ((uint64_t)c << 32) | (uint64_t)r : ((uint64_t)n << 32);
+    out[i] = vals ? vals[i] : (T)1;
   }
-  out->nnz = nnz;
-  out->nzc = nzc;
-  return CBX_OK;
+  block_add_counts<kMergeNT>(oob, loops, &bad[0], &bad[1]);
+}
+
+int bits_for(int64_t v) {  // bits to hold 0..v
+  int b = 1;
+  while (b < 62 && (int64_t(1) << b) <= v) ++b;
+  return b;
+}
+
+template <int K, class T>
+cbx_status coo_typed(int64_t m, int64_t n, const int64_t* rows, const int64_t* cols, const T* vals, int64_t nnz,
+                     int drop_loops, cbx_dcsc* out, hipStream_t s) {
+  const int N = (int)nnz;
+  uint64_t *k0 = nullptr, *k1 = nullptr, *ukeys = nullptr;
+  T *v0 = nullptr, *v1 = nullptr;
+  int64_t *nruns = nullptr, *counts = nullptr;
+  unsigned long long* bad = nullptr;
+  void* tmp = nullptr;
+  size_t tb = 0, t1 = 0, t2 = 0;
+  const int end_bit = 32 + bits_for(n);  // the loop sentinel (col = n) included
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, k0, k1, v0, v1, N, 0, end_bit, s));
+  CBX_HIP((assemble_tmp_bytes<K, T>(N, s, t2)));
+  tb = std::max(t1, t2);
+  auto cleanup = [&]() {
+    for (void* p : {(void*)k0, (void*)k1, (void*)ukeys, (void*)v0, (void*)v1, (void*)nruns, (void*)counts,
+                    (void*)bad, tmp})
+      if (p) (void)hipFreeAsync(p, s);
+  };
+  if (hipMallocAsync((void**)&k0, sizeof(uint64_t) * N, s) || hipMallocAsync((void**)&k1, sizeof(uint64_t) * N, s) ||
+      hipMallocAsync((void**)&ukeys, sizeof(uint64_t) * N, s) || hipMallocAsync((void**)&v0, sizeof(T) * N, s) ||
+      hipMallocAsync((void**)&v1, sizeof(T) * N, s) || hipMallocAsync((void**)&nruns, 2 * sizeof(int64_t), s) ||
+      hipMallocAsync((void**)&counts, sizeof(int64_t) * N, s) ||
+      hipMallocAsync((void**)&bad, 2 * sizeof(unsigned long long), s) || hipMallocAsync(&tmp, std::max<size_t>(tb, 16), s)) {
+    (void)hipGetLastError();
+    cleanup();
+    return set_error(CBX_ERR_NOMEM, "cbx_dcsc_from_coo: workspace allocation failed");
+  }
+  unsigned long long hbad[2] = {0, 0};
+  hipError_t e = hipMemsetAsync(bad, 0, 2 * sizeof(unsigned long long), s);
+  if (e == hipSuccess) {
+    const unsigned g = (unsigned)std::min<int64_t>((nnz + kMergeNT - 1) / kMergeNT, 4096);
+    k_coo_keys<T><<<g, kMergeNT, 0, s>>>(rows, cols, vals, nnz, m, n, drop_loops, k0, v0, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess || hbad[0]) {
+    cleanup();
+    return e != hipSuccess ? set_error(CBX_ERR_HIP, "cbx_dcsc_from_coo: %s", hipGetErrorString(e))
+                           : set_error(CBX_ERR_INVALID, "cbx_dcsc_from_coo: %llu entries outside the %lld x %lld shape",
+                                       hbad[0], (long long)m, (long long)n);
+  }
+  e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, N, 0, end_bit, s);
+  if (e != hipSuccess) {
+    cleanup();
+    return set_error(CBX_ERR_HIP, "cbx_dcsc_from_coo: %s", hipGetErrorString(e));
+  }
+  // the dropped loops sit at the end of the sorted keys
+  const cbx_status st = reduce_assemble<K, T>(k1, v1, N - (int)hbad[1], ukeys, v0, nruns, counts, tmp, tb, out, s,
+                                              "cbx_dcsc_from_coo");
+  cleanup();
+  return st;
 }
 
 }  // namespace
@@ -238,4 +351,52 @@ extern "C" cbx_status cbx_dcsc_merge(int32_t count, const cbx_dcsc* parts, cbx_s
   }
   return set_error(CBX_ERR_UNSUPPORTED, "cbx_dcsc_merge: semiring %d with dtype %d is not provided", (int)semiring,
                    (int)dt);
+}
+
+extern "C" cbx_status cbx_dcsc_from_coo(int64_t m, int64_t n, const int64_t* rows, const int64_t* cols,
+                                        const void* vals, int64_t nnz, cbx_dtype dtype, cbx_semiring semiring,
+                                        int32_t remove_loops, cbx_dcsc* out, void* stream) {
+  CBX_RETURN_IF(!out || (nnz > 0 && (!rows || !cols)), CBX_ERR_INVALID, "cbx_dcsc_from_coo: null argument");
+  CBX_RETURN_IF(m < 0 || n < 0 || nnz < 0, CBX_ERR_INVALID, "cbx_dcsc_from_coo: negative size");
+  CBX_RETURN_IF(m >= (1ll << 31) || n >= (1ll << 31), CBX_ERR_LIMIT, "cbx_dcsc_from_coo: dimensions must be below 2^31");
+  CBX_RETURN_IF(nnz >= (1ll << 31), CBX_ERR_LIMIT, "cbx_dcsc_from_coo: at most 2^31 - 1 entries");
+  cbx_status st = init_device_once();
+  if (st != CBX_OK) return st;
+  std::memset(out, 0, sizeof(*out));
+  out->m = m;
+  out->n = n;
+  out->dtype = dtype;
+  out->location = CBX_DEVICE;
+  hipStream_t s = (hipStream_t)stream;
+  if (nnz == 0) {  // an empty m x n matrix
+    if ((st = dev_alloc((void**)&out->cp, sizeof(int64_t))) != CBX_OK ||
+        (st = dev_alloc((void**)&out->jc, sizeof(int64_t))) != CBX_OK ||
+        (st = dev_alloc((void**)&out->ir, sizeof(int64_t))) != CBX_OK ||
+        (st = dev_alloc(&out->val, sizeof(int64_t))) != CBX_OK) {
+      cbx_dcsc_free(out);
+      return st;
+    }
+    CBX_HIP(hipMemsetAsync(out->cp, 0, sizeof(int64_t), s));
+    CBX_HIP(hipStreamSynchronize(s));
+    return CBX_OK;
+  }
+  const int loops = remove_loops ? 1 : 0;
+  switch (semiring * 8 + dtype) {
+#define CBX_CO(SRK, SRE, T, DT) \
+  case SRE * 8 + DT:            \
+    return coo_typed<SRK, T>(m, n, rows, cols, (const T*)vals, nnz, loops, out, s);
+    CBX_CO(kPlusTimes, CBX_SR_PLUS_TIMES, int64_t, CBX_I64)
+    CBX_CO(kPlusTimes, CBX_SR_PLUS_TIMES, double, CBX_F64)
+    CBX_CO(kPlusTimes, CBX_SR_PLUS_TIMES, float, CBX_F32)
+    CBX_CO(kPlusTimes, CBX_SR_PLUS_TIMES, uint8_t, CBX_BOOL)
+    CBX_CO(kMinPlus, CBX_SR_MIN_PLUS, int64_t, CBX_I64)
+    CBX_CO(kMinPlus, CBX_SR_MIN_PLUS, double, CBX_F64)
+    CBX_CO(kMinPlus, CBX_SR_MIN_PLUS, float, CBX_F32)
+    CBX_CO(kSelectMax, CBX_SR_SELECT_MAX, int64_t, CBX_I64)
+    CBX_CO(kSelectMax, CBX_SR_SELECT_MAX, double, CBX_F64)
+    CBX_CO(kSelectMax, CBX_SR_SELECT_MAX, float, CBX_F32)
+#undef CBX_CO
+  }
+  return set_error(CBX_ERR_UNSUPPORTED, "cbx_dcsc_from_coo: semiring %d with dtype %d is not provided", (int)semiring,
+                   (int)dtype);
 }

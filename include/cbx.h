@@ -402,6 +402,18 @@ cbx_status cbx_dcsc_transpose(const cbx_dcsc* X, cbx_dcsc* out, void* stream);
  * differently).  At most 2^31 - 1 entries over all parts. */
 cbx_status cbx_dcsc_merge(int32_t count, const cbx_dcsc* parts, cbx_semiring semiring, cbx_dcsc* out, void* stream);
 
+/* Edge list to matrix on the device: the Graph500 constructor SpTuples(maxnnz,
+ * m, n, edges, removeloops) (SpTuples.cpp:70-110: values 1, duplicates summed,
+ * self loops dropped) and SpTuples::RemoveDuplicates(BinOp) (SpTuples.cpp:271)
+ * followed by SpDCCols(SpTuples).  rows, cols: device int64 arrays of nnz
+ * entries in any order; vals: device array of nnz values of dtype, or NULL
+ * for all ones.  Entries with equal (row, col) are combined with the
+ * semiring's add; with remove_loops != 0 entries with row == col are
+ * dropped.  Indices outside m x n are an error.  At most 2^31 - 1 entries. */
+cbx_status cbx_dcsc_from_coo(int64_t m, int64_t n, const int64_t* rows, const int64_t* cols, const void* vals,
+                             int64_t nnz, cbx_dtype dtype, cbx_semiring semiring, int32_t remove_loops, cbx_dcsc* out,
+                             void* stream);
+
 /* count independent products C[i] = A[i]*B[i] (arrays of count device
  * descriptors, one dtype) computed together: the batch is packed into one
  * block-diagonal product, so every SpGEMM kernel is launched once for the

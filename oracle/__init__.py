@@ -102,6 +102,8 @@ def _load_ref() -> C.CDLL:
         L.ref_mcl_prune.restype = C.c_int32
         L.ref_multiway_merge.argtypes = [C.c_int32, P(cbx_dcsc), C.c_int32, P(cbx_dcsc)]
         L.ref_multiway_merge.restype = C.c_int32
+        L.ref_graph500_tuples.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int32, P(cbx_dcsc)]
+        L.ref_graph500_tuples.restype = C.c_int32
         L.ref_free.argtypes = [P(cbx_dcsc)]
         L.ref_free.restype = None
         _ref = L
@@ -204,22 +206,12 @@ def ref_merge(parts: list[Dcsc], semiring: str = "plus_times") -> Dcsc:
     return Dcsc.from_struct(out, L.ref_free)
 
 
-def merge(parts: list[Dcsc], semiring: str = "plus_times") -> Dcsc:
-    """Restatement of MultiwayMerge<SR> (MultiwayMerge.h:428-541, SerialMerge
-    :193-250): the parts' entries in column-major order, equal (row, col)
-    combined with SR::add, nothing dropped (explicit zeros stay).  Ties are
-    added in part order; the reference pops them in heap order, which changes
-    only floating-point rounding."""
-    if not parts:
-        raise ValueError("merge needs at least one part")
-    m, n, dt = parts[0].m, parts[0].n, parts[0].dtype
-    for p in parts:
-        if (p.m, p.n, p.dtype) != (m, n, dt):
-            raise ValueError("parts differ in shape or dtype")
-    rows = np.concatenate([p.ir for p in parts]).astype(np.int64)
-    cols = np.concatenate([p.col_ids() for p in parts]).astype(np.int64)
-    vals = np.concatenate([p.val for p in parts])
-    order = np.lexsort((rows, cols))  # stable: ties keep part order
+def _combine(m: int, n: int, rows, cols, vals, dt: str, semiring: str) -> Dcsc:
+    """Entries in column-major order with equal (row, col) combined by the
+    semiring's add (stable: ties in input order), as DCSC."""
+    rows = np.asarray(rows, np.int64)
+    cols = np.asarray(cols, np.int64)
+    order = np.lexsort((rows, cols))
     rows, cols, vals = rows[order], cols[order], vals[order]
     if rows.size == 0:
         return Dcsc(m, n, np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0, np.int64), vals, dt)
@@ -243,6 +235,54 @@ def merge(parts: list[Dcsc], semiring: str = "plus_times") -> Dcsc:
     jc, cnt = np.unique(c, return_counts=True)
     cp = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
     return Dcsc(m, n, cp, jc.astype(np.int64), r, v, dt)
+
+
+def merge(parts: list[Dcsc], semiring: str = "plus_times") -> Dcsc:
+    """Restatement of MultiwayMerge<SR> (MultiwayMerge.h:428-541, SerialMerge
+    :193-250): the parts' entries in column-major order, equal (row, col)
+    combined with SR::add, nothing dropped (explicit zeros stay).  Ties are
+    added in part order; the reference pops them in heap order, which changes
+    only floating-point rounding."""
+    if not parts:
+        raise ValueError("merge needs at least one part")
+    m, n, dt = parts[0].m, parts[0].n, parts[0].dtype
+    for p in parts:
+        if (p.m, p.n, p.dtype) != (m, n, dt):
+            raise ValueError("parts differ in shape or dtype")
+    return _combine(m, n, np.concatenate([p.ir for p in parts]), np.concatenate([p.col_ids() for p in parts]),
+                    np.concatenate([p.val for p in parts]), dt, semiring)
+
+
+def from_coo(m: int, n: int, rows, cols, vals=None, dtype: str = "int64", semiring: str = "plus_times",
+             remove_loops: bool = False) -> Dcsc:
+    """Restatement of the Graph500 SpTuples constructor (SpTuples.cpp:70-110:
+    values 1, duplicates summed, self loops dropped) generalised like
+    SpTuples::RemoveDuplicates(BinOp) (:271): vals None means ones, equal
+    (row, col) combined with the semiring's add, loops dropped on request."""
+    from combblas_amd import NP_DTYPES
+    rows = np.asarray(rows, np.int64)
+    cols = np.asarray(cols, np.int64)
+    vals = np.ones(rows.size, NP_DTYPES[dtype]) if vals is None else np.asarray(vals, NP_DTYPES[dtype])
+    if np.any((rows < 0) | (rows >= m) | (cols < 0) | (cols >= n)):
+        raise ValueError("entries outside the matrix")
+    if remove_loops:
+        keep = rows != cols
+        rows, cols, vals = rows[keep], cols[keep], vals[keep]
+    return _combine(m, n, rows, cols, vals, dtype, semiring)
+
+
+def ref_from_coo_graph500(m: int, n: int, rows, cols, remove_loops: bool = True) -> Dcsc:
+    """The reference's Graph500 constructor SpTuples(maxnnz, m, n, edges,
+    removeloops) (SpTuples.cpp:70), as int64 DCSC (only where oracle/_ref was built)."""
+    L = _load_ref()
+    edges = np.empty(2 * len(rows), np.int64)
+    edges[0::2] = rows
+    edges[1::2] = cols
+    out = cbx_dcsc()
+    st = L.ref_graph500_tuples(int(m), int(n), edges.ctypes.data, len(rows), int(bool(remove_loops)), C.byref(out))
+    if st != 0:
+        raise RuntimeError(f"ref_graph500_tuples failed with status {st}")
+    return Dcsc.from_struct(out, L.ref_free)
 
 
 def ref_symbolic(A: Dcsc, B: Dcsc) -> tuple[np.ndarray, np.ndarray]:

@@ -194,6 +194,16 @@ int ref_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semiring, int32_t m
   return -1;
 }
 
+// SpTuples' Graph500 constructor (SpTuples.cpp:70): values 1, duplicates
+// summed, self loops dropped when removeloops.
+int ref_graph500_tuples(int64_t m, int64_t n, const int64_t* edges, int64_t nedges, int32_t removeloops,
+                        cbx_dcsc* C) {
+  std::vector<int64_t> e(edges, edges + 2 * nedges);
+  SpTuples<int64_t, int64_t> T(nedges, m, n, e, removeloops != 0);
+  export_mat<int64_t, int64_t>(T, m, n, CBX_I64, C);
+  return 0;
+}
+
 int ref_multiway_merge(int32_t count, const cbx_dcsc* parts, int32_t semiring, cbx_dcsc* C) {
   if (count < 1) return CBX_ERR_INVALID;
   switch (semiring) {

@@ -51,3 +51,44 @@ def test_merge_edge_cases():
         cb.merge([random_part(rng, 50, 40, 10).to_device(), random_part(rng, 50, 41, 10).to_device()])
     with pytest.raises(ValueError):
         cb.merge([])
+
+
+@pytest.mark.parametrize("semiring,dtype,loops", [("plus_times", "int64", True), ("plus_times", "int64", False),
+                                                  ("min_plus", "int64", False), ("select_max", "float64", True),
+                                                  ("plus_times", "bool", True), ("plus_times", "float64", False)])
+def test_from_coo_matches_oracle(semiring, dtype, loops):
+    rng = np.random.default_rng(17)
+    m, n, k = 70000, 50000, 400000
+    rows = rng.integers(0, m, k)
+    cols = rng.integers(0, n, k)
+    rows[:5000] = cols[:5000]
+    rows[5000:9000], cols[5000:9000] = rows[:4000], cols[:4000]  # duplicates
+    vals = {"int64": rng.integers(-7, 8, k).astype(np.int64), "float64": rng.uniform(-1, 1, k),
+            "bool": np.ones(k, np.uint8)}[dtype]
+    dv = cb.DeviceArray.from_host(vals) if dtype != "bool" else None
+    dr, dc = cb.DeviceArray.from_host(rows), cb.DeviceArray.from_host(cols)
+    got = cb.from_coo_device(m, n, dr, dc, dv, dtype, semiring, loops).to_host()
+    want = oracle.from_coo(m, n, rows, cols, vals if dtype != "bool" else None, dtype, semiring, loops)
+    same(got, want, rtol=1e-12 if (dtype == "float64" and semiring == "plus_times") else 0.0)
+
+
+def test_from_coo_edge_cases():
+    rows = cb.DeviceArray.from_host(np.array([0, 1, 2], np.int64))
+    cols = cb.DeviceArray.from_host(np.array([0, 1, 2], np.int64))
+    got = cb.from_coo_device(3, 3, rows, cols, None, "int64", "plus_times", True).to_host()
+    assert got.nnz == 0 and got.nzc == 0  # only loops
+    bad = cb.DeviceArray.from_host(np.array([0, 5, 1], np.int64))
+    with pytest.raises(cb.CbxError):
+        cb.from_coo_device(3, 3, bad, cols)
+    empty = cb.DeviceArray.from_host(np.zeros(0, np.int64))
+    assert cb.from_coo_device(4, 4, empty, empty).to_host().nnz == 0
+
+
+def test_from_coo_builds_the_rmat_benchmark_matrix():
+    """The host generator's R-MAT edges (s16 ef16, values 1..9 summed over
+    duplicates) rebuilt on the device give the generator's matrix."""
+    A = cb.gen_rmat(16, 16)
+    rows, cols = A.ir, A.col_ids()
+    dA = cb.from_coo_device(A.m, A.n, cb.DeviceArray.from_host(rows), cb.DeviceArray.from_host(cols),
+                            cb.DeviceArray.from_host(A.val)).to_host()
+    same(dA, A)

@@ -83,3 +83,30 @@ def test_column_and_row_ranges():
         keep = (idx >= lo) & (idx < hi)
         assert np.array_equal(A.column_range(lo, hi).to_dense(), np.where(keep[None, :], d, 0))
         assert np.array_equal(A.row_range(lo, hi).to_dense(), np.where(keep[:, None], d, 0))
+
+
+@pytest.mark.parametrize("remove_loops", [True, False])
+def test_from_coo_matches_reference_graph500_constructor(remove_loops):
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref is not built (needs the reference tree)")
+    rng = np.random.default_rng(8)
+    m = n = 500
+    rows = rng.integers(0, m, 20000)
+    cols = rng.integers(0, n, 20000)
+    rows[:300] = cols[:300]  # self loops
+    same(oracle.from_coo(m, n, rows, cols, remove_loops=remove_loops),
+         oracle.ref_from_coo_graph500(m, n, rows, cols, remove_loops))
+
+
+def test_from_coo_of_concatenated_parts_is_their_merge():
+    """Duplicates combined by the semiring: building from the concatenation of
+    several matrices' entries equals merging them (pinned to the reference
+    MultiwayMerge above), for each semiring."""
+    rng = np.random.default_rng(9)
+    parts = [random_part(rng, 200, 150, 2500) for _ in range(4)]
+    rows = np.concatenate([p.ir for p in parts])
+    cols = np.concatenate([p.col_ids() for p in parts])
+    vals = np.concatenate([p.val for p in parts])
+    perm = rng.permutation(rows.size)  # any input order
+    for sr in ("plus_times", "min_plus", "select_max"):
+        same(oracle.from_coo(200, 150, rows[perm], cols[perm], vals[perm], semiring=sr), oracle.merge(parts, sr))
