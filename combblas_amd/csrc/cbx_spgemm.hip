@@ -370,7 +370,8 @@ template <int K, class T, bool SYM, int NR>
 __device__ __forceinline__ void tiny_finish(int32_t (&key)[kTinyMax / kWave],
                                             typename Sem<K, T>::Acc (&val)[kTinyMax / kWave], int lane, int col,
                                             int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpC,
-                                            int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err) {
+                                            int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err,
+                                            const int64_t* __restrict__ stage_off, int64_t slot) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int32_t kNone = 0x7fffffff;
@@ -432,9 +433,20 @@ __device__ __forceinline__ void tiny_finish(int32_t (&key)[kTinyMax / kWave],
   if constexpr (SYM) {
     if (lane == 0) nnzcol[col] = count;
   } else {
-    const int64_t have = nnzcol[col];  // the column's allocation from the symbolic pass
-    if (lane == 0 && count != have) atomicOr(err, 8);
-    const int64_t out = cpC[col];
+    // the column's allocation from the symbolic pass; or, staged (no symbolic
+    // pass), its slot of flop_j entries in the staging buffer, and the count
+    // is recorded for the copy into place
+    int64_t have, out;
+    if (stage_off) {
+      out = stage_off[slot];
+      have = stage_off[slot + 1] - out;
+      if (lane == 0) nnzcol[col] = count;
+      if (lane == 0 && count > have) atomicOr(err, 8);
+    } else {
+      have = nnzcol[col];
+      if (lane == 0 && count != have) atomicOr(err, 8);
+      out = cpC[col];
+    }
     Acc carry = SR::ident();
     bool open = false;  // a run continues from the previous register
     int before = 0;     // heads in earlier registers
@@ -477,7 +489,8 @@ __global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order,
                                               const T* __restrict__ valB, const int64_t* __restrict__ cpA,
                                               const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
                                               int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpC,
-                                              int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err) {
+                                              int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err,
+                                              const int64_t* __restrict__ stage_off = nullptr) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int kE = kTinyMax / kWave;  // 4 elements per lane
@@ -528,9 +541,27 @@ __global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order,
     }
     if (filled > kTinyMax && lane == 0) atomicOr(err, 4);
     // sort only as many registers as the column fills (wave-uniform)
-    if (filled <= kWave) tiny_finish<K, T, SYM, 1>(key, val, lane, col, nnzcol, cpC, irC, valC, err);
-    else if (filled <= 2 * kWave) tiny_finish<K, T, SYM, 2>(key, val, lane, col, nnzcol, cpC, irC, valC, err);
-    else tiny_finish<K, T, SYM, 4>(key, val, lane, col, nnzcol, cpC, irC, valC, err);
+    if (filled <= kWave) tiny_finish<K, T, SYM, 1>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i);
+    else if (filled <= 2 * kWave)
+      tiny_finish<K, T, SYM, 2>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i);
+    else tiny_finish<K, T, SYM, 4>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i);
+  }
+}
+
+// Staged tiny columns into place: one wave per column copies its nnzcol[col]
+// entries from its staging slot to cpC[col].
+template <class T>
+__global__ void k_tiny_place(const int32_t* __restrict__ order, int64_t first, int64_t ncols,
+                             const int64_t* __restrict__ stage_off, const int64_t* __restrict__ nnzcol,
+                             const int64_t* __restrict__ cpC, const int64_t* __restrict__ sir,
+                             const T* __restrict__ sval, int64_t* __restrict__ irC, T* __restrict__ valC) {
+  const int64_t i = global_wave();
+  if (i >= ncols) return;
+  const int col = order[first + i];
+  const int64_t src = stage_off[i], dst = cpC[col], cnt = nnzcol[col];
+  for (int64_t e = lane_id(); e < cnt; e += kWave) {
+    irC[dst + e] = sir[src + e];
+    valC[dst + e] = sval[src + e];
   }
 }
 
@@ -1980,10 +2011,13 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const int G = kSymRBW * 32 / TR;  // tiles per symbolic pass of the streaming tier
 
   DevBuf b_colmap, b_rows, b_bpos, b_flop, b_order, b_flop_s, b_order_s, b_P, b_nnz, b_cp, b_tmp, b_thr, b_bounds,
-      b_flags, b_nsel, b_err, b_vcnt, b_ul, b_uh, b_ucnt;
+      b_flags, b_nsel, b_err, b_vcnt, b_ul, b_uh, b_ucnt, b_soff, b_sir, b_sval;
   int32_t *colmap, *rowsA, *bpos, *order, *order_s, *P;
   int *err, *ucnt;
   unsigned long long* vcnt;
+  int64_t *stage_off = nullptr, *stage_ir = nullptr;  // staged tiny tier (see the symbolic pass)
+  T* stage_val = nullptr;
+  bool tiny_staged = false;
   int64_t *flop, *flop_s, *nnzcol, *cpC, *thr, *bounds, *nsel;
   uint8_t* flags;
   Unit *units_l = nullptr, *units_h = nullptr;
@@ -2143,7 +2177,33 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   }
   if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
   if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
-  if (n_h256) k_tiny<K, T, true><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr, nullptr, nullptr, err);
+  // tiny tier: its numeric kernel runs now into a staging buffer (a slot of
+  // flop_j entries per column) and records the exact counts; after the
+  // output scan the entries are copied into place.  One gather and sort per
+  // column instead of two (symbolic + numeric), when the slots fit a
+  // quarter of the free memory.
+  if (n_h256) {
+    CBX_TRY(dalloc(b_soff, n_h256 + 1, s, &stage_off));
+    CBX_HIP(hipMemsetAsync(stage_off, 0, sizeof(int64_t), s));
+    CBX_HIP(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, flop_s + hb[2], stage_off + 1, (int)n_h256, s));
+    int64_t slots = 0;
+    CBX_HIP(hipMemcpyAsync(&slots, stage_off + n_h256, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CBX_HIP(hipStreamSynchronize(s));
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) == hipSuccess &&
+        (double)slots * (sizeof(int64_t) + sizeof(T)) <= 0.25 * (double)(freeb + cache_idle_bytes()) &&
+        dalloc(b_sir, std::max<int64_t>(slots, 1), s, &stage_ir) == CBX_OK &&
+        dalloc(b_sval, std::max<int64_t>(slots, 1), s, &stage_val) == CBX_OK)
+      tiny_staged = true;
+    (void)hipGetLastError();
+    if (tiny_staged)
+      k_tiny<K, T, false><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
+                                                           valA, nnzcol, nullptr, stage_ir, stage_val, err,
+                                                           stage_off);
+    else
+      k_tiny<K, T, true><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
+                                                          valA, nnzcol, nullptr, nullptr, nullptr, err);
+  }
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
   if (n_dense) {
@@ -2216,7 +2276,10 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   // the 8192-slot table takes ~100 KB of LDS, so one workgroup per CU: give it 16 waves
   CBX_LAUNCH_HASH(8192, 1024, n_h8, hb[0]);
   CBX_LAUNCH_HASH(2048, kHashNT, n_h2, hb[1]);
-  if (n_h256)
+  if (n_h256 && tiny_staged)
+    k_tiny_place<T><<<wave_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, stage_off, nnzcol, cpC, stage_ir,
+                                                       stage_val, C->ir, (T*)C->val);
+  else if (n_h256)
     k_tiny<K, T, false><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
                                                          valA, nnzcol, cpC, C->ir, (T*)C->val, err);
 #undef CBX_LAUNCH_HASH
