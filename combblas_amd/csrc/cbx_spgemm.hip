@@ -516,16 +516,23 @@ __global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order,
       const int inc = wave_incl_scan(len);
       const int off = filled + inc - len;  // first element of this lane's segment
       const int total = __shfl(inc, kWave - 1, kWave);
+      // lanes that hold B entries in this chunk, and the search's first step:
+      // the largest power of two below that count (all wave-uniform)
+      const int nb = (int)(b1 - eb < kWave ? b1 - eb : kWave);
+      int top = 0;
+      if (nb > 1) top = 1 << (31 - __clz(nb - 1));
 #pragma unroll
       for (int u = 0; u < kE; ++u) {
+        // registers whose 64 elements miss this chunk's range stay as they are
+        if ((u + 1) * kWave <= filled || u * kWave >= filled + total) continue;
         const int e = u * kWave + lane;
-        // last lane s with off_s <= e (segments of length 0 share offsets; the
-        // search lands on the one that holds e when e is in this chunk)
+        // last lane s < nb with off_s <= e (segments of length 0 share
+        // offsets; the search lands on the one that holds e when e is in this
+        // chunk)
         int lo = 0;
-#pragma unroll
-        for (int step = kWave / 2; step > 0; step >>= 1) {
+        for (int step = top; step > 0; step >>= 1) {
           const int o = __shfl(off, lo + step, kWave);
-          if (o <= e) lo += step;
+          if (lo + step < nb && o <= e) lo += step;
         }
         const int o = __shfl(off, lo, kWave);
         const int l = __shfl(len, lo, kWave);
