@@ -806,9 +806,12 @@ __global__ __launch_bounds__(NT) void k_num_dense(const int32_t* __restrict__ or
   }
 }
 
-// Symbolic pass of the unit tier: one workgroup per column; bitmap passes of
-// 2^20 rows.  Writes nnz_j and the column's units.
-template <int NT, int U>
+// Symbolic pass of the unit tier: one workgroup per column.  Units are cut
+// over passes of 256 blocks (2^20 rows); within a pass the presence bitmap is
+// built IBLK blocks at a time (each sub-pass gathers only the products in its
+// rows, through the tile table P), so the LDS bitmap stays small enough for
+// several workgroups per CU.  Writes nnz_j and the column's units.
+template <int NT, int U, int IBLK>
 __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ order, int64_t first,
                                                   const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
                                                   const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
@@ -818,10 +821,11 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
                                                   int* __restrict__ n_l, int cap_l, Unit* __restrict__ units_h,
                                                   int* __restrict__ n_h, int cap_h, uint32_t* __restrict__ rmap,
                                                   int64_t rm_cols, int* __restrict__ err) {
-  constexpr int BMW = 32768;          // 2^20 rows per bitmap pass
-  constexpr int BPP = BMW / kBRW;     // 256 blocks per pass
+  constexpr int BPP = 256;             // blocks per cutting pass (2^20 rows)
+  constexpr int BMW = IBLK * kBRW;     // bitmap words of one sub-pass
+  static_assert(BPP % IBLK == 0, "sub-passes tile the cutting pass");
   __shared__ uint32_t bits[BMW];
-  __shared__ int bpre[BPP];           // inclusive prefix of block counts
+  __shared__ int bpre[BPP];           // block counts, then their inclusive prefix
   __shared__ Unit su[BPP];
   __shared__ SegStage<uint8_t, NT> st;
   __shared__ long long wsum[NT / kWave];
@@ -829,46 +833,51 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
   const int col = order[first + blockIdx.x];
   const int64_t b0 = cpB[col], b1 = cpB[col + 1];
   const int npass = (S + BPP - 1) / BPP;
+  const bool whole = S <= IBLK;  // one sub-pass covers every row: no tile look-ups
   int64_t colcnt = 0;
   int unit_off = 0;  // meaningful in thread 0 only
   for (int sp = 0; sp < npass; ++sp) {
     const int t_lo = sp * BPP, t_hi = min(S, t_lo + BPP), nblk = t_hi - t_lo;
-    const int32_t row0 = t_lo * kBR;
-    for (int i = threadIdx.x; i < nblk * kBRW; i += NT) bits[i] = 0u;
-    // (the staging barrier below orders the clear before any insertion)
-    for (int64_t eb = b0; eb < b1; eb += NT) {
-      int64_t base;
-      int len;
-      uint8_t dummy;
-      load_seg<uint8_t, false>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, npass == 1, nullptr, base, len,
-                               dummy);
-      int nseg;
-      const int tot = stage_chunk<uint8_t, NT>(st, wsum, base, len, 0, nseg);
-      visit_staged<uint8_t, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const uint8_t(&)[U]) {
-        int32_t r[U];
+    for (int i_lo = t_lo; i_lo < t_hi; i_lo += IBLK) {
+      const int i_hi = min(t_hi, i_lo + IBLK), inb = i_hi - i_lo;
+      const int32_t row0 = i_lo * kBR;
+      for (int i = threadIdx.x; i < inb * kBRW; i += NT) bits[i] = 0u;
+      // (the staging barrier below orders the clear before any insertion)
+      for (int64_t eb = b0; eb < b1; eb += NT) {
+        int64_t base;
+        int len;
+        uint8_t dummy;
+        load_seg<uint8_t, false>(eb + threadIdx.x, b1, bpos, cpA, P, S, i_lo, i_hi, whole, nullptr, base, len,
+                                 dummy);
+        int nseg;
+        const int tot = stage_chunk<uint8_t, NT>(st, wsum, base, len, 0, nseg);
+        visit_staged<uint8_t, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const uint8_t(&)[U]) {
+          int32_t r[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
+          for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
-      });
+          for (int u = 0; u < U; ++u)
+            if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+        });
+      }
+      __syncthreads();
+      // the column's row map (its presence bitmap, S blocks of kBRW words)
+      // for the numeric pass, which then need not rebuild it; kept for the
+      // first rm_cols columns of the launch (the only ones that can form
+      // large units)
+      if (rmap && blockIdx.x < rm_cols) {
+        uint32_t* dst = rmap + (int64_t)blockIdx.x * S * kBRW + (int64_t)i_lo * kBRW;
+        for (int i = threadIdx.x; i < inb * kBRW; i += NT) dst[i] = bits[i];
+      }
+      // per-block counts: one wave per block, two words per lane
+      for (int b = wave_id(); b < inb; b += NT / kWave) {
+        const uint32_t* w = bits + b * kBRW;
+        int c = __popc(w[lane_id()]) + __popc(w[lane_id() + kWave]);
+        c = (int)wave_sum_ll(c);
+        if (lane_id() == 0) bpre[i_lo - t_lo + b] = c;
+      }
+      __syncthreads();  // the counts are read, and the next sub-pass clears the bitmap
     }
-    __syncthreads();
-    // the column's row map (its presence bitmap, S blocks of kBRW words) for
-    // the numeric pass, which then need not rebuild it; kept for the first
-    // rm_cols columns of the launch (the only ones that can form large units)
-    if (rmap && blockIdx.x < rm_cols) {
-      uint32_t* dst = rmap + (int64_t)blockIdx.x * S * kBRW + (int64_t)t_lo * kBRW;
-      for (int i = threadIdx.x; i < nblk * kBRW; i += NT) dst[i] = bits[i];
-    }
-    // per-block counts: one wave per block, two words per lane
-    for (int b = wave_id(); b < nblk; b += NT / kWave) {
-      const uint32_t* w = bits + b * kBRW;
-      int c = __popc(w[lane_id()]) + __popc(w[lane_id() + kWave]);
-      c = (int)wave_sum_ll(c);
-      if (lane_id() == 0) bpre[b] = c;
-    }
-    __syncthreads();
     if (wave_id() == 0) {  // inclusive prefix over blocks
       int carry = 0;
       for (int b0i = 0; b0i < nblk; b0i += kWave) {
@@ -1779,7 +1788,7 @@ constexpr int kDenseTR = 32 * kDenseNT;       // rows per numeric tile (16384)
 constexpr int kSymNT = 512;
 constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass (2^20 rows)
 // unit tier
-constexpr int kSymUnitNT = 1024;
+constexpr int kSymUnitNT = 512, kSymUnitIBLK = 64;  // 32 KB bitmap: three workgroups per CU
 // light units span at most 16 blocks: ~53 KB of LDS, three workgroups per CU
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 16;    // <= 4096 outputs, <= 64K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
@@ -1893,7 +1902,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
     CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
     CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
     CBX_HIP(hipMemsetAsync(hucnt, 0, 2 * sizeof(int), s));
-    k_sym_units<kSymUnitNT, 4><<<(unsigned)nheavy, kSymUnitNT, 0, s>>>(
+    k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)nheavy, kSymUnitNT, 0, s>>>(
         hcols, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
         units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, nullptr, 0, err);
     CBX_HIP(hipGetLastError());
@@ -2175,7 +2184,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     cap_h = std::max<int64_t>(1, std::min<int64_t>(n_heavy_max * (int64_t)S, 1ll << 30));
     CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
     CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
-    k_sym_units<kSymUnitNT, 4><<<(unsigned)n_dense, kSymUnitNT, 0, s>>>(
+    k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)n_dense, kSymUnitNT, 0, s>>>(
         order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
         units_l, ucnt, (int)cap_l, units_h, ucnt + 1, (int)cap_h, rmap, rm_cols, err);
   } else if (n_dense) {
