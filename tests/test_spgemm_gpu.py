@@ -221,6 +221,28 @@ def test_large_units_on_tall_matrices(share):
     assert np.array_equal(got.ir, want.ir) and np.array_equal(got.val, want.val)
 
 
+@pytest.mark.parametrize("m", [300_007, (1 << 18) + 1])
+def test_symbolic_bitmap_sub_passes(m):
+    """One 256-block cutting pass whose bitmap is built in 64-block sub-passes,
+    the last one partial (74 blocks: 64 + 10; 65 blocks: 64 + 1): unit counts,
+    row maps and the product are bit-exact against the oracle."""
+    rng = np.random.default_rng(37)
+    n, k = 1500, 200
+    key = np.unique(rng.integers(0, n, 300_000) * m + rng.integers(0, m, 300_000))
+    A = cb.Dcsc.from_coo(m, n, key % m, key // m, rng.integers(-4, 5, key.size).astype(np.int64), "int64")
+    keyb = np.unique(rng.integers(0, k, 12_000) * n + rng.integers(0, n, 12_000))
+    B = cb.Dcsc.from_coo(n, k, keyb % n, keyb // n, rng.integers(-4, 5, keyb.size).astype(np.int64), "int64")
+    base = cb.get_config()
+    try:
+        cb.set_config(unit_split_nnz=4096)
+        got, st = gpu_spgemm(A, B)
+    finally:
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, B)
+    assert st["units_light"] + st["units_heavy"] > 0
+    bit_exact(got, want)
+
+
 def test_operand_validation_with_empty_columns():
     """The product's operand validation accepts explicitly empty columns in cp
     (a descent of rows where a column starts after an empty one is legal) and
