@@ -1896,15 +1896,32 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_HIP(hipMemcpyAsync(&nheavy, nh_dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipStreamSynchronize(s));
   int64_t cap_l = 0, cap_h = 0;
+  // row maps of the exact columns, as in the two-phase path: their large
+  // units then load them instead of a counting sweep (when they fit the
+  // share of free memory; from the caching allocator)
+  struct RowMap {
+    uint32_t* p = nullptr;
+    ~RowMap() { dev_free(p); }
+  } rm;
+  int64_t rm_cols = 0;
   if (nheavy) {
     cap_l = std::max<int64_t>(1, std::min<int64_t>(nheavy * (int64_t)S, 1ll << 30));
     cap_h = cap_l;
     CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
     CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
     CBX_HIP(hipMemsetAsync(hucnt, 0, 2 * sizeof(int), s));
+    if (cfg.row_map_share > 0.0f) {
+      const int64_t rm_words = nheavy * (int64_t)S * kBRW;
+      size_t freeb = 0, totb = 0;
+      if (hipMemGetInfo(&freeb, &totb) == hipSuccess &&
+          4.0 * (double)rm_words <= (double)cfg.row_map_share * (double)(freeb + cache_idle_bytes()) &&
+          dev_alloc((void**)&rm.p, sizeof(uint32_t) * rm_words) == CBX_OK)
+        rm_cols = nheavy;
+      (void)hipGetLastError();
+    }
     k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)nheavy, kSymUnitNT, 0, s>>>(
         hcols, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
-        units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, nullptr, 0, err);
+        units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, rm.p, rm_cols, err);
     CBX_HIP(hipGetLastError());
   }
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
@@ -1928,8 +1945,20 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
                                                     units, nullptr, nullptr);
   // output capacity from the predictions (plus the exact hash-tier sizes)
   const int64_t cap = hash_total + (int64_t)(pred_sum * (1.0 + cfg.capacity_margin)) + (1 << 20);
-  CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * cap));
-  CBX_TRY(dev_alloc(&C->val, sizeof(T) * cap));
+  auto alloc_out = [&]() -> cbx_status {
+    if (!C->ir) CBX_TRY(dev_alloc((void**)&C->ir, sizeof(int64_t) * cap));
+    if (!C->val) CBX_TRY(dev_alloc(&C->val, sizeof(T) * cap));
+    return CBX_OK;
+  };
+  cbx_status ast = alloc_out();
+  if (ast != CBX_OK && rm.p) {  // give the row maps back; the large units then count in a sweep
+    CBX_HIP(hipStreamSynchronize(s));
+    dev_free(rm.p);
+    rm.p = nullptr;
+    release_cache();
+    ast = alloc_out();
+  }
+  CBX_TRY(ast);
   CBX_HIP(hipMemsetAsync(status, 0, sizeof(unsigned long long) * std::max<int64_t>(nunits, 1), s));
   CBX_HIP(hipMemsetAsync(ticket, 0, sizeof(int) * 2, s));
   CBX_HIP(hipMemsetAsync(ovf, 0, sizeof(int) * 2, s));
@@ -1971,7 +2000,11 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32), CBX_ERR_LIMIT,
                 "cbx_spgemm: too many units for a single launch");
   info.heavy_units = hu[0] + hu[1];
-  if (hu[1])
+  if (hu[1] && rm.p)
+    k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+        units_h, rm.p, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, rm_cols, cpC, C->ir,
+        (T*)C->val, err);
+  else if (hu[1])
     k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
         units_h, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err);
   if (hu[0])
