@@ -1740,6 +1740,34 @@ struct DevBuf {
   }
 };
 
+// Large per-call arrays whose size repeats from product to product (the unit
+// lists) come from the caching allocator: a stream-pool allocation of several
+// GB has, now and then, taken over a second when the pool had to grow.  The
+// block returns to the cache at scope exit, once the stream's kernels that
+// read it are done (the synchronisation is free on the normal path, which
+// has already synchronised; an early error return waits for them).
+struct CachedBuf {
+  void* p = nullptr;
+  hipStream_t s = nullptr;
+  ~CachedBuf() {
+    if (!p) return;
+    (void)hipStreamSynchronize(s);
+    dev_free(p);
+  }
+};
+
+template <class X>
+cbx_status calloc_cached(CachedBuf& b, size_t count, hipStream_t s, X** out) {
+  b.s = s;
+  const cbx_status st = dev_alloc(&b.p, std::max<size_t>(count * sizeof(X), 256));
+  if (st != CBX_OK) {
+    b.p = nullptr;
+    return st;
+  }
+  *out = (X*)b.p;
+  return CBX_OK;
+}
+
 template <class X>
 cbx_status dalloc(DevBuf& b, size_t count, hipStream_t s, X** out) {
   b.s = s;
@@ -1822,7 +1850,8 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
                                   int* err, LearnedInfo& info) {
   const int64_t nzcB = B->nzc, m = A->m;
   DevBuf b_feat, b_lp, b_hist, b_w, b_blo, b_bhi, b_uc, b_ub, b_pn, b_units, b_status, b_tick, b_ct, b_ovf, b_hcs, b_sum,
-      b_hflag, b_hcols, b_nh, b_ul, b_uh, b_hu;
+      b_hflag, b_hcols, b_nh, b_hu;
+  CachedBuf b_ul, b_uh;
   uint8_t* heavy_flag;
   int32_t* hcols;
   int64_t* nh_dev;
@@ -1907,8 +1936,8 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   if (nheavy) {
     cap_l = std::max<int64_t>(1, std::min<int64_t>(nheavy * (int64_t)S, 1ll << 30));
     cap_h = cap_l;
-    CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
-    CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
+    CBX_TRY(calloc_cached(b_ul, cap_l, s, &units_l));
+    CBX_TRY(calloc_cached(b_uh, cap_h, s, &units_h));
     CBX_HIP(hipMemsetAsync(hucnt, 0, 2 * sizeof(int), s));
     if (cfg.row_map_share > 0.0f) {
       const int64_t rm_words = nheavy * (int64_t)S * kBRW;
@@ -2060,7 +2089,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   const int G = kSymRBW * 32 / TR;  // tiles per symbolic pass of the streaming tier
 
   DevBuf b_colmap, b_rows, b_bpos, b_flop, b_order, b_flop_s, b_order_s, b_P, b_nnz, b_cp, b_tmp, b_thr, b_bounds,
-      b_flags, b_nsel, b_err, b_vcnt, b_ul, b_uh, b_ucnt, b_soff, b_sir, b_sval;
+      b_flags, b_nsel, b_err, b_vcnt, b_ucnt, b_soff, b_sir, b_sval;
+  CachedBuf b_ul, b_uh;
   int32_t *colmap, *rowsA, *bpos, *order, *order_s, *P;
   int *err, *ucnt;
   unsigned long long* vcnt;
@@ -2215,8 +2245,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (use_units && n_dense) {
     cap_l = std::max<int64_t>(1, std::min<int64_t>(n_dense * (int64_t)S, 1ll << 30));
     cap_h = std::max<int64_t>(1, std::min<int64_t>(n_heavy_max * (int64_t)S, 1ll << 30));
-    CBX_TRY(dalloc(b_ul, cap_l, s, &units_l));
-    CBX_TRY(dalloc(b_uh, cap_h, s, &units_h));
+    CBX_TRY(calloc_cached(b_ul, cap_l, s, &units_l));
+    CBX_TRY(calloc_cached(b_uh, cap_h, s, &units_h));
     k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)n_dense, kSymUnitNT, 0, s>>>(
         order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
         units_l, ucnt, (int)cap_l, units_h, ucnt + 1, (int)cap_h, rmap, rm_cols, err);
