@@ -76,3 +76,28 @@ def test_tunables_validated_without_gpu():
     finally:
         cb.set_config(**base)
     assert cb.get_config() == base
+
+
+def test_stored_tuning_table_is_current():
+    """Every stored choice names exactly the library's tunables, passes its
+    validation, is the fastest same-output point of its own sweep, and is
+    what autotune.apply installs."""
+    from combblas_amd import autotune
+    table = autotune.load_table()
+    assert table["version"] == autotune.TABLE_VERSION and table["entries"]
+    base = cb.get_config()
+    try:
+        for name, e in table["entries"].items():
+            assert name in autotune.WORKLOADS
+            assert set(e["best"]) == set(base), name
+            ok = [r for r in e["sweep"] if r["same_nnz"]]
+            fastest = min(ok, key=lambda r: r["ms"])
+            assert all(e["best"][k] == v for k, v in fastest["config"].items()), name
+            assert abs(e["best_ms"] - fastest["ms"]) < 1e-9
+            cb.set_config(**base)
+            got = autotune.apply(name)
+            assert got is not None
+            for k, v in e["best"].items():
+                assert abs(cb.get_config()[k] - v) <= 1e-6 * max(1.0, abs(v)), (name, k)
+    finally:
+        cb.set_config(**base)
