@@ -6,8 +6,8 @@ and FP32 plus-times must lie within the a-priori bound
     |c_hat - c| <= gamma_k(u) * (|A| |B|)_ij,   gamma_k = k u / (1 - k u),
 with k the column's flop count (the length of the longest possible sum),
 u = 2^-53 for FP64 and 2^-24 for FP32 (FP32 compared against an FP64 oracle
-of the same FP32 inputs).  For positive inputs this is the north star's
-"1e-12 relative per entry" for FP64 at every k below 4.5e3.
+of the same FP32 inputs).  For positive inputs this is within 1e-12 relative
+per entry for FP64 at every k below 4.5e3.
 """
 from __future__ import annotations
 
