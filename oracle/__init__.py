@@ -1,4 +1,4 @@
-"""ORACLE -- test infrastructure only (the checker, never the product).
+"""ORACLE -- test infrastructure only (results are compared against it; never the product).
 
 Python binding of the CPU restatement in ``oracle/cbx_oracle.cpp``
 (``oracle/liboracle.so``) and, when it was built in this checkout, of the

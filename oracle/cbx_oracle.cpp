@@ -1,7 +1,8 @@
 // ORACLE -- test infrastructure only.
 //
 // A CPU restatement of CombBLAS's node-local kernels, used by tests/, by
-// __graft_entry__.smoke() and by bench.py's CPU-baseline leg as the checker.
+// __graft_entry__.smoke() and by bench.py's CPU-baseline leg, only to compare
+// results against.
 // Nothing in the product (combblas_amd/, include/) may call it.
 //
 // Every function cites the reference code it restates (paths relative to the
