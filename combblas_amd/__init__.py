@@ -104,6 +104,8 @@ def lib() -> C.CDLL:
         L.cbx_last_error.restype = C.c_char_p
         L.cbx_device_count.restype = C.c_int32
         L.cbx_device_synchronize.restype = C.c_int32
+        L.cbx_device_name.argtypes = [C.c_char_p, C.c_size_t]
+        L.cbx_device_name.restype = C.c_int32
         L.cbx_release_cached_memory.restype = None
         L.cbx_dcsc_free.argtypes = [P(cbx_dcsc)]
         L.cbx_dcsc_free.restype = None
@@ -187,6 +189,13 @@ def _check(status: int) -> None:
 
 def device_count() -> int:
     return int(lib().cbx_device_count())
+
+
+def device_name() -> str:
+    """Name and architecture of the current device (from the HIP runtime libcbx.so links)."""
+    buf = C.create_string_buffer(256)
+    _check(lib().cbx_device_name(buf, 256))
+    return buf.value.decode(errors="replace")
 
 
 def synchronize() -> None:

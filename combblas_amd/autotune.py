@@ -50,10 +50,10 @@ LEARNED_GRID = {
 
 
 def device_name() -> str:
+    """Label of the device a sweep ran on (through libcbx.so's own HIP runtime)."""
     try:
-        import torch
-        return torch.cuda.get_device_name(0)
-    except Exception:  # noqa: BLE001 - the name is only a label
+        return cb.device_name()
+    except cb.CbxError:
         return "unknown"
 
 

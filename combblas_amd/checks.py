@@ -39,19 +39,28 @@ def product_checksums(A: Dcsc, B: Dcsc, salt: int = 0) -> tuple[np.ndarray, np.n
     return B.jc.copy(), cs
 
 
-def product_matches(C, A: Dcsc, B: Dcsc, salt: int = 0x5EED) -> bool:
+def product_matches(C, A: Dcsc, B: Dcsc, salt: int = 0x5EED, col_nnz: np.ndarray | None = None) -> bool:
     """Device product C (a DeviceDcsc) against the closed form: a valid DCSC
-    whose stored columns and per-column checksums are those of A*B."""
+    whose stored columns and per-column checksums are those of A*B.  With
+    col_nnz (the exact entry count of C(:,j) for every stored column of B,
+    e.g. from cbx_spgemm_symbolic) the per-column entry counts must match
+    too, which a zero-valued spurious or dropped entry would change while
+    leaving the checksum alone."""
     C.check()
     got = C.column_checksums(salt=salt)
-    _, jcC = C.structure()
+    cpC, jcC = C.structure()
     jcB, want = product_checksums(A, B, salt=salt)
     pos = np.searchsorted(jcB, jcC)
     if pos.size and (pos.max() >= jcB.size or not np.array_equal(jcB[pos], jcC)):
         return False
     dropped = np.ones(jcB.size, bool)
     dropped[pos] = False
-    return bool(np.array_equal(got, want[pos]) and not np.any(want[dropped]))
+    ok = bool(np.array_equal(got, want[pos]) and not np.any(want[dropped]))
+    if col_nnz is not None:
+        col_nnz = np.asarray(col_nnz, np.int64)
+        ok = ok and col_nnz.size == jcB.size and bool(np.array_equal(np.diff(cpC), col_nnz[pos])) \
+            and not np.any(col_nnz[dropped])
+    return ok
 
 
 def stencil27_coarse_rowsums(g: int) -> np.ndarray:

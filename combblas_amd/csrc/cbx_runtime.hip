@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -154,6 +155,17 @@ extern "C" int32_t cbx_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
+}
+
+extern "C" cbx_status cbx_device_name(char* buf, size_t len) {
+  CBX_RETURN_IF(!buf || len == 0, CBX_ERR_INVALID, "cbx_device_name: empty buffer");
+  buf[0] = 0;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  CBX_HIP(hipGetDevice(&dev));
+  CBX_HIP(hipGetDeviceProperties(&prop, dev));
+  std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+  return CBX_OK;
 }
 
 extern "C" cbx_status cbx_device_synchronize(void) {

@@ -1756,6 +1756,18 @@ struct CachedBuf {
   }
 };
 
+// Row maps of the symbolic pass (from the caching allocator): returned to
+// the cache once the stream's unit kernels that read them are done.
+struct RowMap {
+  uint32_t* p = nullptr;
+  hipStream_t s = nullptr;
+  ~RowMap() {
+    if (!p) return;
+    (void)hipStreamSynchronize(s);
+    dev_free(p);
+  }
+};
+
 template <class X>
 cbx_status calloc_cached(CachedBuf& b, size_t count, hipStream_t s, X** out) {
   b.s = s;
@@ -1928,10 +1940,8 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   // row maps of the exact columns, as in the two-phase path: their large
   // units then load them instead of a counting sweep (when they fit the
   // share of free memory; from the caching allocator)
-  struct RowMap {
-    uint32_t* p = nullptr;
-    ~RowMap() { dev_free(p); }
-  } rm;
+  RowMap rm;
+  rm.s = s;
   int64_t rm_cols = 0;
   if (nheavy) {
     cap_l = std::max<int64_t>(1, std::min<int64_t>(nheavy * (int64_t)S, 1ll << 30));
@@ -2227,10 +2237,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   // instead of rebuilding them in a counting sweep.  From the caching
   // allocator, so repeated products reuse the block.  (cfg.row_map_share of
   // the free memory at most; 0 turns them off.)
-  struct RowMap {
-    uint32_t* p = nullptr;
-    ~RowMap() { dev_free(p); }
-  } rm;
+  RowMap rm;
+  rm.s = s;
   uint32_t*& rmap = rm.p;
   const int64_t rm_cols = std::min<int64_t>(n_heavy_max, n_dense);
   if (use_units && rm_cols > 0 && cfg.row_map_share > 0.0f) {

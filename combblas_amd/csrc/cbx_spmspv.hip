@@ -262,6 +262,17 @@ __global__ void k_bfs_dense_cp(const int64_t* __restrict__ cp, const int64_t* __
   }
 }
 
+// Number of positions where two int64 arrays differ (the symmetry check of
+// a plan: A's arrays against its transpose's).
+__global__ void k_bfs_count_diff(const int64_t* __restrict__ a, const int64_t* __restrict__ b, int64_t n,
+                                 unsigned long long* __restrict__ cnt) {
+  long long d = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d += a[i] != b[i];
+  d = wave_sum_ll(d);
+  if (lane_id() == 0 && d) atomicAdd(cnt, (unsigned long long)d);
+}
+
 __global__ void k_bfs_rows32(const int64_t* __restrict__ ir, int64_t nnz, int32_t* __restrict__ rows) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < nnz) rows[i] = (int32_t)ir[i];
@@ -475,6 +486,10 @@ struct cbx_bfs_plan_s {
   uint32_t* fbits = nullptr;             // frontier bitmap of a bottom-up level
   int32_t* host_count = nullptr;  // pinned, 32 B: next frontier size; [2..3] its degree sum; [4..7] root's range
   int direction = CBX_BFS_DIRECTION_OPTIMIZING;
+  // Pattern of A equal to that of its transpose.  Bottom-up levels scan column
+  // v of A for frontier rows, which is row v only for a symmetric pattern, so
+  // a plan over a directed graph runs every level top-down.
+  bool symmetric = true;
   // [direction (0 top-down, 1 bottom-up)][frontier parity]
   hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -510,7 +525,37 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
                 "cbx_bfs_plan_create: n and nnz must be below 2^31");
   cbx_status st = cbx_dcsc_check(A);
   if (st != CBX_OK) return st;
+  // symmetric pattern?  (A's structure against its transpose's)
+  bool symmetric = true;
+  {
+    cbx_dcsc T;
+    std::memset(&T, 0, sizeof(T));
+    st = cbx_dcsc_transpose(A, &T, nullptr);
+    if (st != CBX_OK) return st;
+    symmetric = T.nzc == A->nzc && T.nnz == A->nnz;
+    if (symmetric && A->nnz) {
+      unsigned long long* cnt = nullptr;
+      unsigned long long diff = 0;
+      if (hipMalloc(&cnt, sizeof(*cnt)) != hipSuccess) {
+        cbx_dcsc_free(&T);
+        return set_error(CBX_ERR_NOMEM, "cbx_bfs_plan_create: allocation");
+      }
+      (void)hipMemset(cnt, 0, sizeof(*cnt));
+      k_bfs_count_diff<<<1024, 256>>>(A->cp, T.cp, A->nzc + 1, cnt);
+      k_bfs_count_diff<<<1024, 256>>>(A->jc, T.jc, A->nzc, cnt);
+      k_bfs_count_diff<<<1024, 256>>>(A->ir, T.ir, A->nnz, cnt);
+      const hipError_t e = hipMemcpy(&diff, cnt, sizeof(diff), hipMemcpyDeviceToHost);
+      (void)hipFree(cnt);
+      if (e != hipSuccess) {
+        cbx_dcsc_free(&T);
+        return set_error(CBX_ERR_HIP, "cbx_bfs_plan_create: symmetry check: %s", hipGetErrorString(e));
+      }
+      symmetric = diff == 0;
+    }
+    cbx_dcsc_free(&T);
+  }
   auto* p = new cbx_bfs_plan_s();
+  p->symmetric = symmetric;
   p->n = A->n;
   p->nnz = A->nnz;
   p->nw = A->n / 32 + 1;
@@ -587,7 +632,7 @@ extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* pare
   int64_t fe = hdeg[1] - hdeg[0];  // edges of the current frontier
   bool bottom_up = p->direction == CBX_BFS_BOTTOM_UP;
   for (int par = 0;; par ^= 1) {
-    if (p->direction == CBX_BFS_DIRECTION_OPTIMIZING) bottom_up = fe > mu / kBfsAlpha;
+    if (p->direction == CBX_BFS_DIRECTION_OPTIMIZING) bottom_up = p->symmetric && fe > mu / kBfsAlpha;
     CBX_HIP(hipGraphLaunch(p->exec[bottom_up ? 1 : 0][par], s));
     CBX_HIP(hipStreamSynchronize(s));
     ++levels;
@@ -619,6 +664,8 @@ extern "C" cbx_status cbx_bfs_plan_set_direction(cbx_bfs_plan_s* p, cbx_bfs_dire
   CBX_RETURN_IF(direction != CBX_BFS_DIRECTION_OPTIMIZING && direction != CBX_BFS_TOP_DOWN &&
                     direction != CBX_BFS_BOTTOM_UP,
                 CBX_ERR_INVALID, "cbx_bfs_plan_set_direction: unknown direction %d", (int)direction);
+  CBX_RETURN_IF(direction == CBX_BFS_BOTTOM_UP && !p->symmetric, CBX_ERR_INVALID,
+                "cbx_bfs_plan_set_direction: bottom-up levels need a symmetric pattern (this plan's is not)");
   p->direction = direction;
   return CBX_OK;
 }

@@ -76,6 +76,10 @@ const char* cbx_last_error(void);
 /* Number of visible devices (0 when there is no GPU); never fails. */
 int32_t cbx_device_count(void);
 
+/* Name and architecture of the current device, e.g. "AMD Instinct MI355X
+ * (gfx950:sramecc+:xnack-)", written NUL-terminated into buf[len]. */
+cbx_status cbx_device_name(char* buf, size_t len);
+
 /* Block until all work on every stream of the current device has finished. */
 cbx_status cbx_device_synchronize(void);
 
@@ -322,7 +326,10 @@ typedef struct cbx_bfs_stats {
   double ms;        /* device time of the whole BFS */
 } cbx_bfs_stats;
 
-/* A: device DCSC, square, symmetric pattern (values ignored). */
+/* A: device DCSC, square (values ignored); normally the symmetric pattern of
+ * an undirected graph.  The plan compares A's pattern with its transpose's: a
+ * directed (asymmetric) pattern runs every level top-down, and
+ * cbx_bfs_plan_set_direction refuses CBX_BFS_BOTTOM_UP for it. */
 cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan* plan);
 
 /* How each level is computed.  Top-down is TopDownBFS's SpMV: every frontier

@@ -70,15 +70,21 @@ def test_bfs_isolated_root_and_tiny_graph():
     assert np.array_equal(got, want) and st["levels"] == it
 
 
-def test_bfs_rmat_s20_ef16_config():
+@pytest.mark.parametrize("direction", ["top_down", "optimizing"])
+def test_bfs_rmat_s20_ef16_config(direction):
     """BFS workload: symmetrized unweighted R-MAT s20 ef16, self-loops removed;
-    4 of the 64 seeded roots checked against the oracle bit for bit."""
+    4 of the 64 seeded roots (seed 4242) checked against the oracle bit for
+    bit, with the workload's own top-down-only plan and with the per-level
+    direction switch."""
     A = graph(20, 16)
-    plan = cb.BfsPlan(A.to_device())
+    plan = cb.BfsPlan(A.to_device(), direction)
     for root in cb.bfs_roots(A, 64, seed=4242)[:4]:
         got, st = plan.run(int(root))
         want, it = oracle.bfs(A, int(root))
         assert st["levels"] == it and np.array_equal(got, want)
+        if direction == "top_down":
+            assert st["bottom_up_levels"] == 0
+    plan.release()
 
 
 @pytest.mark.parametrize("semiring", ["plus_times", "min_plus", "select_max"])
@@ -183,3 +189,22 @@ def test_bfs_directions_give_identical_parents(scale):
         plan.release()
     with pytest.raises(KeyError):
         cb.BfsPlan(dA, "sideways")
+
+
+def test_bfs_directed_graph_runs_top_down():
+    """A directed pattern (R-MAT s14 without symmetrisation): the plan finds it
+    asymmetric, runs every level top-down even when asked to optimise, and
+    refuses bottom-up levels; parents equal the oracle's SpMV-based BFS."""
+    A = cb.gen_rmat(14, 16, no_self_loops=True, pattern=True).astype("bool")
+    assert not np.array_equal(A.to_dense(), A.to_dense().T)
+    dA = A.to_device()
+    plan = cb.BfsPlan(dA)
+    for root in cb.bfs_roots(A, 4, seed=3):
+        got, st = plan.run(int(root))
+        want, it = oracle.bfs(A, int(root))
+        assert st["levels"] == it and st["bottom_up_levels"] == 0
+        assert np.array_equal(got, want)
+    with pytest.raises(cb.CbxError) as ei:
+        plan.set_direction("bottom_up")
+    assert ei.value.status == cb.ERR_INVALID
+    plan.release()

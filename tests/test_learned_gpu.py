@@ -157,3 +157,24 @@ def test_trained_network_beats_sampling_baseline_s18():
             assert st["learned_fallback"] == 0
         assert product_matches(C, A, A)
         C.release()
+
+
+@pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
+@pytest.mark.parametrize("scale,ef", [(14, 8), (16, 16)])
+@pytest.mark.parametrize("share", [0.25, 0.0])
+def test_learned_heavy_columns_exact(scale, ef, share):
+    """heavy_nnz at its lowest (4096): columns predicted larger than that take
+    the exact path for predicted-heavy columns, with row maps (share > 0) and
+    without them; the whole product is bit-exact against the oracle."""
+    A = cb.gen_rmat(scale, ef)
+    base = cb.get_config()
+    cb.set_predictor(predictor.load())
+    try:
+        cb.set_config(heavy_nnz=4096, row_map_share=share)
+        got, st = learned(A, A)
+    finally:
+        cb.set_config(**base)
+    assert st["learned_fallback"] == 0
+    assert st["units_heavy"] > 0
+    want, _ = oracle.spgemm(A, A)
+    bit_exact(got, want)

@@ -327,3 +327,35 @@ def test_full_size_linear_checksums(scale, ef):
     dropped[pos] = False
     assert not np.any(want[dropped])
     assert stats["units_light"] + stats["units_heavy"] > 0
+
+
+def heavy_column_sample(A: cb.Dcsc, nheavy: int, nrand: int, seed: int) -> cb.Dcsc:
+    """B = the nheavy columns of A with the most products in A*A plus nrand
+    random ones, at A's shape: full-length columns of A*A at a fraction of
+    its cost."""
+    lens = np.zeros(A.n, np.int64)
+    lens[A.jc] = np.diff(A.cp)
+    flop = np.add.reduceat(lens[A.ir], A.cp[:-1])
+    rng = np.random.default_rng(seed)
+    qs = np.unique(np.concatenate([np.argsort(flop)[-nheavy:], rng.choice(A.nzc, nrand, replace=False)]))
+    idx = np.concatenate([np.arange(A.cp[q], A.cp[q + 1]) for q in qs])
+    return cb.Dcsc.from_coo(A.m, A.n, A.ir[idx], np.repeat(A.jc[qs], np.diff(A.cp)[qs]), A.val[idx], A.dtype)
+
+
+@pytest.mark.parametrize("scale", [16, 18])
+def test_fp64_plus_times_long_columns_rtol_1e_12(scale):
+    """FP64 plus-times with positive values on R-MAT columns of 10^5 and more
+    products (the heaviest columns of A*A plus random ones): every entry
+    within 1e-12 relative of the FP64 oracle (sums of positive terms in
+    another order differ by about sqrt(k) * 2^-53 relatively)."""
+    rng = np.random.default_rng(scale)
+    A = cb.gen_rmat(scale, 16).astype("float64")
+    A.val = rng.uniform(0.5, 2.0, A.nnz)
+    B = heavy_column_sample(A, 16, 48, seed=scale)
+    got, st = gpu_spgemm(A, B)
+    want, flops = oracle.spgemm(A, B)
+    same_pattern(got, want)
+    colflop, _ = oracle.symbolic(A, B)
+    assert colflop.max() >= 100_000
+    rel = np.abs(got.val - want.val) / want.val
+    assert rel.max() <= 1e-12, f"max relative error {rel.max():.3g}"
