@@ -48,10 +48,37 @@ def dcsc_bytes(nnz: int, nzc: int, vbytes: int = 8) -> int:
     return 16 * nzc + 8 + (8 + vbytes) * nnz
 
 
-def cpu_baseline(scale: int, ef: int, budget_s: float) -> dict | None:
-    """Runs the CPU leg in a child process (it never touches the GPU)."""
+def host_cpu() -> dict:
+    """CPU model, logical CPUs this process may run on, and the physical cores among them."""
+    model, cores = "unknown", set()
+    avail = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo").read().splitlines() + [""]:
+            if not line.strip():
+                if cur.get("processor") is not None and int(cur["processor"]) in avail:
+                    cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                cur = {}
+                continue
+            k, _, v = line.partition(":")
+            cur[k.strip()] = v.strip()
+            if k.strip() == "model name":
+                model = v.strip()
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": len(avail), "physical_cores": len(cores) or None}
+
+
+def cpu_baseline(scale: int, ef: int, budget_s: float, bfs_roots=None, bfs_edges: int | None = None) -> dict | None:
+    """The reference's kernels on this host's cores, in a child process (it
+    never touches the GPU): LocalHybridSpGEMM<PlusTimes<int64>> alone and with
+    the SpDCCols(SpTuples) conversion on B-column slices of the same product
+    (one warm-up, then the median of three runs per slice), and the
+    multithreaded BFS level loop (RowSplit + generic_gespmv_threaded) over the
+    benchmark's roots."""
+    roots = [int(r) for r in bfs_roots] if bfs_roots is not None else []
     code = f"""
-import json, sys, time
+import json, sys
 sys.path.insert(0, {ROOT!r})
 import numpy as np
 import combblas_amd as cb
@@ -65,43 +92,71 @@ try:
 except Exception:
     kind = "port"
 threads = int(oracle.max_threads())
-secs = 0.0; flops = 0; slices = []
-target = max(total // 400, 1)          # first slice ~0.25% of the work
-for s in range(16):
-    if secs >= {budget_s}:
-        break
-    mid = int(np.searchsorted(cum, (s % 8 + 0.5) / 8 * total))
-    c0 = mid; c1 = int(np.searchsorted(cum, cum[c0] + target)) + 1
-    c1 = min(max(c1, c0 + 1), A.nzc)
+spent = 0.0; flops = 0; t_prod = 0.0; t_dcsc = 0.0; slices = []
+target = max(total // 2000, 1)          # first slice ~0.05% of the work
+def timed(c0, c1):
     B = oracle.column_slice(A, c0, c1)
-    f = int(fl[c0:c1].sum())
     if kind == "reference":
-        t, nnz = oracle.ref_time_hybrid_i64(A, B)
-    else:
-        t, nnz, _ = oracle.time_spgemm(A, A, cols=(c0, c1))
-    secs += t; flops += f; slices.append([c0, c1])
-    if t < {budget_s} / 16:
+        return oracle.ref_time_hybrid_dcsc_i64(A, B)[:2]
+    t = oracle.time_spgemm(A, A, cols=(c0, c1))[0]
+    return t, None
+for s in range(16):
+    if spent >= {budget_s}:
+        break
+    mid = int(np.searchsorted(cum, (s + 0.5) / 16 * total))  # 16 disjoint positions across the work
+    c0 = mid; c1 = int(np.searchsorted(cum, cum[c0] + min(target, total // 32))) + 1
+    c1 = min(max(c1, c0 + 1), A.nzc)
+    runs = [timed(c0, c1) for _ in range(4)]          # one warm-up, then three timed
+    spent += sum(r[1] if r[1] is not None else r[0] for r in runs)
+    t_prod += float(np.median([r[0] for r in runs[1:]]))
+    if runs[0][1] is not None:
+        t_dcsc += float(np.median([r[1] for r in runs[1:]]))
+    flops += int(fl[c0:c1].sum()); slices.append([c0, c1])
+    if runs[1][0] < {budget_s} / 64:
         target *= 2
-print(json.dumps(dict(kind=kind, threads=threads, secs=secs, flops=flops, slices=slices, total=total)))
+out = dict(kind=kind, threads=threads, t_prod=t_prod, t_dcsc=t_dcsc, flops=flops, slices=slices, total=total)
+roots = {roots!r}
+if roots:
+    G = cb.gen_rmat({scale}, {ef}, symmetrize=True, no_self_loops=True, pattern=True).astype("bool")
+    if kind == "reference":
+        oracle.ref_bfs_threaded(G, roots[:1], threads)     # warm-up
+        _, levels, secs = oracle.ref_bfs_threaded(G, roots, threads)
+    else:
+        import time
+        t0 = time.perf_counter(); levels = sum(oracle.bfs(G, r)[1] for r in roots); secs = time.perf_counter() - t0
+    out["bfs"] = dict(secs=secs, levels=levels, roots=len(roots))
+print(json.dumps(out))
 """
     try:
-        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=budget_s * 4 + 120)
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=budget_s * 8 + 300)
         line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
         d = json.loads(line)
     except Exception as e:  # noqa: BLE001 - a failed baseline must not hide the GPU number
         log(f"cpu baseline failed: {e}")
         return None
-    gflops = 2.0 * d["flops"] / d["secs"] / 1e9 if d["secs"] > 0 else None
+    cpu = host_cpu()
     frac = d["flops"] / d["total"]
-    return {
-        "value": gflops, "unit": "GFLOPS", "cores": d["threads"], "kind": d["kind"],
+    out = {
+        "value": 2.0 * d["flops"] / d["t_prod"] / 1e9 if d["t_prod"] > 0 else None, "unit": "GFLOPS",
+        "cores": d["threads"], "kind": d["kind"], "cpu": cpu,
+        "gflops_with_dcsc_conversion": (2.0 * d["flops"] / d["t_dcsc"] / 1e9 if d["t_dcsc"] > 0 else None),
         "sample": (f"LocalHybridSpGEMM PlusTimes<int64> on {len(d['slices'])} B-column slices of the same "
-                   f"R-MAT s{scale} ef{ef} A*A ({frac:.2%} of its {d['total']} multiply-adds, "
-                   f"{d['secs']:.1f} s), OpenMP threads={d['threads']}"),
+                   f"R-MAT s{scale} ef{ef} A*A ({frac:.2%} of its {d['total']} multiply-adds), one warm-up then "
+                   f"the median of 3 runs per slice (summed: {d['t_prod']:.2f} s alone, {d['t_dcsc']:.2f} s with "
+                   f"SpDCCols(SpTuples)); OpenMP threads={d['threads']} on {cpu['model']}"),
     }
+    if "bfs" in d:
+        b = d["bfs"]
+        ms = 1e3 * b["secs"] / b["roots"]
+        out["bfs"] = {"ms_per_bfs_mean": round(ms, 3), "levels_mean": round(b["levels"] / b["roots"], 2),
+                      "gteps": (round(bfs_edges / b["secs"] / 1e9, 4) if bfs_edges and b["secs"] > 0 else None),
+                      "cores": d["threads"], "kind": d["kind"],
+                      "sample": (f"TopDownBFS level loop, RowSplit({d['threads']}) + generic_gespmv_threaded"
+                                 f"<SelectMaxSRing<bool,int64>>, {b['roots']} roots of the same graph")}
+    return out
 
 
-def run_spgemm(cb, A, dA, mode: str, steps: int, warmup: int) -> dict:
+def run_spgemm(cb, A, dA, mode: str, steps: int, warmup: int, col_nnz=None) -> dict:
     from combblas_amd.checks import product_matches
     for _ in range(warmup):
         C, st = cb.spgemm(dA, dA, "plus_times", mode)
@@ -116,7 +171,7 @@ def run_spgemm(cb, A, dA, mode: str, steps: int, warmup: int) -> dict:
             C.release()
     cb.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    exact = product_matches(C, A, A)  # full size, outside the timed region
+    exact = product_matches(C, A, A, col_nnz=col_nnz)  # full size, outside the timed region
     C.release()
     return {"ms": ms, "stats": stats, "exact": exact}
 
@@ -132,6 +187,7 @@ def bench_bfs(cb, scale: int, ef: int, nroots: int = 64) -> dict:
     roots = cb.bfs_roots(G, nroots)
     out = {"workload": f"BFS (TopDownBFS parents), symmetrised R-MAT s{scale} ef{ef} without self-loops, "
                        f"{len(roots)} roots (seed 4242), one captured graph per level and direction"}
+    total_edges = 0
     for direction in ("optimizing", "top_down"):
         plan.set_direction(direction)
         plan.run(int(roots[0]), copy=False)
@@ -139,6 +195,8 @@ def bench_bfs(cb, scale: int, ef: int, nroots: int = 64) -> dict:
         for r in roots:
             par, st = plan.run(int(r), copy=True)
             edges = int(deg[par >= 0].sum()) // 2  # undirected edges in the reached component
+            if direction == "top_down":
+                total_edges += edges
             ms.append(st["ms"])
             levels.append(st["levels"])
             bu.append(st["bottom_up_levels"])
@@ -152,6 +210,8 @@ def bench_bfs(cb, scale: int, ef: int, nroots: int = 64) -> dict:
         else:
             out["top_down_only"] = res
     plan.release()
+    out["_roots"] = [int(r) for r in roots]
+    out["_edges_total"] = total_edges
     return out
 
 
@@ -314,7 +374,7 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edgefactor", type=int, default=16)
     ap.add_argument("--mode", default="both", choices=["both", "two_phase", "learned"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline budget (0 disables)")
     ap.add_argument("--no-extra", action="store_true", help="skip the BFS, MCL and Galerkin workloads")
     ap.add_argument("--only", choices=["bfs", "mcl", "galerkin", "spmv", "merge"], default=None,
                     help="run only this workload and print its JSON (for profiling)")
@@ -358,6 +418,9 @@ def main() -> int:
     default_cfg = cb.get_config()
     tuned = {}
 
+    # exact entry count of every output column (the two-phase symbolic pass,
+    # outside any timed region): with the checksums it pins the full-size output
+    col_nnz = cb.spgemm_symbolic(dA, dA)
     modes = ["two_phase", "learned"] if args.mode == "both" else [args.mode]
     if "learned" in modes:
         if predictor.MODEL_PATH.exists():
@@ -372,19 +435,22 @@ def main() -> int:
         cb.set_config(**default_cfg)
         tuned[mode] = autotune.apply(name)
         log(f"spgemm {mode}: {args.warmup} warmup + {args.steps} steps, tunables {cb.get_config()}")
-        res[mode] = run_spgemm(cb, A, dA, mode, args.steps, args.warmup)
+        res[mode] = run_spgemm(cb, A, dA, mode, args.steps, args.warmup, col_nnz)
         log(f"spgemm {mode}: {res[mode]['ms']:.2f} ms/step, exact={res[mode]['exact']}")
     baseline = None
-    if "learned" in res:  # the same single pass driven by a non-learned estimate, for comparison
-        ratio = predictor.sampling_ratio(A, dA)
-        cb.set_predictor(predictor.ratio_params(ratio))
-        rb = run_spgemm(cb, A, dA, "learned", 1, 1)
+    if "learned" in res:  # the same single pass sized without the network: the reference's estimator
+        cb.set_predictor(None)
+        cb.set_sampling_predictor(5)
+        rb = run_spgemm(cb, A, dA, "learned", 1, 1, col_nnz)
         sb = rb["stats"][-1]
-        baseline = {"predictor": f"flop_j x sampled global ratio {ratio:.4f} (1% of columns)",
+        baseline = {"predictor": "estimateNNZ_sampling min-hash estimates per column (5 rounds, mtSpGEMM.h:936-1054)",
                     "ms_per_step": round(rb["ms"], 3), "output_bit_exact": rb["exact"],
                     "overflow_units": sb["overflow_cols"], "units": sb["units_light"],
                     "overflow_pct": round(100.0 * sb["overflow_cols"] / max(sb["units_light"], 1), 3),
-                    "fallback": int(sb["learned_fallback"])}
+                    "fallback": int(sb["learned_fallback"]),
+                    "predicted_vs_actual_nnz": (round(sb["predicted_nnz"] / sb["actual_nnz_predicted"], 4)
+                                                if sb["actual_nnz_predicted"] else None)}
+        cb.set_sampling_predictor(0)
         cb.set_predictor(predictor.load())
     # `value` is the faster of the measured exact paths; both are reported
     exact_modes = [m for m in res if res[m]["exact"]] or list(res)
@@ -466,7 +532,15 @@ def main() -> int:
             log(f"{name}: {json.dumps(out[name])}")
             cb.release_cached_memory()
     if args.cpu_seconds > 0:
-        out["cpu_baseline"] = cpu_baseline(args.scale, args.edgefactor, args.cpu_seconds)
+        bfs = out.get("bfs") or {}
+        cb_out = cpu_baseline(args.scale, args.edgefactor, args.cpu_seconds, bfs.get("_roots"),
+                              bfs.get("_edges_total"))
+        if cb_out and "bfs" in cb_out:
+            bfs["cpu_baseline"] = cb_out.pop("bfs")
+        out["cpu_baseline"] = cb_out
+    if isinstance(out.get("bfs"), dict):
+        out["bfs"].pop("_roots", None)
+        out["bfs"].pop("_edges_total", None)
     print(json.dumps(out))
     return 0 if out["output_bit_exact"] else 1
 

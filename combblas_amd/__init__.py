@@ -171,6 +171,11 @@ def lib() -> C.CDLL:
         L.cbx_mlp_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, P(C.c_float)]
         L.cbx_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, P(cbx_mlp_train_opts),
                                     C.c_void_p]
+        L.cbx_spgemm_estimate_sampling.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_void_p, P(C.c_double),
+                                                   C.c_void_p]
+        L.cbx_spgemm_estimate_sampling.restype = C.c_int32
+        L.cbx_spgemm_set_sampling_predictor.argtypes = [C.c_int32]
+        L.cbx_spgemm_set_sampling_predictor.restype = C.c_int32
         for name in ("cbx_spgemm_symbolic", "cbx_spgemm_features", "cbx_mlp_forward", "cbx_mlp_gradient",
                      "cbx_mlp_train"):
             getattr(L, name).restype = C.c_int32
@@ -686,6 +691,20 @@ def spgemm_symbolic(A: DeviceDcsc, B: DeviceDcsc) -> np.ndarray:
     d = DeviceArray(max(B.nzc, 1), np.int64)
     _check(lib().cbx_spgemm_symbolic(C.byref(A.s), C.byref(B.s), d.ptr, None))
     return d.to_host(B.nzc)
+
+
+def spgemm_estimate_sampling(A: DeviceDcsc, B: DeviceDcsc, nrounds: int = 5) -> tuple[np.ndarray, float]:
+    """estimateNNZ_sampling per column on the device: (float32[nzc_B] estimates, their sum)."""
+    d = DeviceArray(max(B.nzc, 1), np.float32)
+    tot = C.c_double(0)
+    _check(lib().cbx_spgemm_estimate_sampling(C.byref(A.s), C.byref(B.s), nrounds, d.ptr, C.byref(tot), None))
+    return d.to_host(B.nzc), float(tot.value)
+
+
+def set_sampling_predictor(nrounds: int) -> None:
+    """Learned mode sized by estimateNNZ_sampling's per-column estimates (nrounds in [2, 8]) instead of
+    the network; 0 goes back to the network."""
+    _check(lib().cbx_spgemm_set_sampling_predictor(int(nrounds)))
 
 
 def spgemm_features(A: DeviceDcsc, B: DeviceDcsc) -> np.ndarray:

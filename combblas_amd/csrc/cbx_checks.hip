@@ -87,6 +87,28 @@ __global__ __launch_bounds__(kCheckNT) void k_check_columns(const int64_t* __res
   block_add_counts<kCheckNT>(bad, desc, &cnt[0], &cnt[2]);
 }
 
+__global__ void k_check_zero(unsigned long long* __restrict__ cnt) {
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
+}
+
+void dcsc_check_launch(const cbx_dcsc* X, unsigned long long* cnt, hipStream_t s) {
+  auto blocks = [](int64_t n) {
+    return (unsigned)std::min<int64_t>(std::max<int64_t>((n + kCheckNT - 1) / kCheckNT, 1), kCheckBlocks);
+  };
+  k_check_zero<<<1, 64, 0, s>>>(cnt);
+  k_check_columns<<<blocks(X->nzc), kCheckNT, 0, s>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->n, cnt);
+  if (X->nnz > 0) k_check_entries<<<blocks(X->nnz), kCheckNT, 0, s>>>(X->ir, X->nnz, X->m, cnt);
+}
+
+cbx_status dcsc_check_result(const unsigned long long* cnt, hipStream_t s) {
+  unsigned long long h[3] = {0, 0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  CBX_HIP(e);
+  return h[0] || h[1] != h[2] ? set_error(CBX_ERR_INVALID, "cbx_dcsc_check: malformed DCSC") : CBX_OK;
+}
+
 }  // namespace cbx
 
 using namespace cbx;
@@ -122,19 +144,10 @@ extern "C" cbx_status cbx_dcsc_check(const cbx_dcsc* X) {
   const hipStream_t s0 = nullptr;
   unsigned long long* d = nullptr;
   CBX_HIP(hipMallocAsync((void**)&d, 3 * sizeof(unsigned long long), s0));
-  CBX_HIP(hipMemsetAsync(d, 0, 3 * sizeof(unsigned long long), s0));
-  auto blocks = [](int64_t n) {
-    return (unsigned)std::min<int64_t>(std::max<int64_t>((n + kCheckNT - 1) / kCheckNT, 1), kCheckBlocks);
-  };
-  k_check_columns<<<blocks(X->nzc), kCheckNT, 0, s0>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->n, d);
-  if (X->nnz > 0) k_check_entries<<<blocks(X->nnz), kCheckNT, 0, s0>>>(X->ir, X->nnz, X->m, d);
-  unsigned long long h[3] = {0, 0, 0};
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s0);
+  dcsc_check_launch(X, d, s0);
+  const cbx_status st = dcsc_check_result(d, s0);
   (void)hipFreeAsync(d, s0);
-  if (e == hipSuccess) e = hipStreamSynchronize(s0);
-  CBX_HIP(e);
-  return h[0] || h[1] != h[2] ? set_error(CBX_ERR_INVALID, "cbx_dcsc_check: malformed DCSC") : CBX_OK;
+  return st;
 }
 
 extern "C" cbx_status cbx_dcsc_structure_to_host(const cbx_dcsc* X, int64_t* cp_host, int64_t* jc_host) {

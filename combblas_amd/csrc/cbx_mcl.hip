@@ -650,8 +650,9 @@ extern "C" cbx_status cbx_mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, doubl
                 "cbx_mcl_expand: mode must be two-phase or learned");
   if ((st = init_device_once())) return st;
   SpgemmConfig cfg = current_config();
-  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor, CBX_ERR_INVALID,
-                "cbx_mcl_expand: learned mode needs a predictor (cbx_spgemm_set_predictor)");
+  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor && cfg.sampling_rounds == 0, CBX_ERR_INVALID,
+                "cbx_mcl_expand: learned mode needs a predictor (cbx_spgemm_set_predictor or "
+                "cbx_spgemm_set_sampling_predictor)");
   cbx_mcl_stats local;
   cbx_mcl_stats* sp = stats ? stats : &local;
   std::memset(sp, 0, sizeof(*sp));

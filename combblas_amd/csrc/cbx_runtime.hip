@@ -276,8 +276,9 @@ extern "C" cbx_status cbx_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semir
                 CBX_ERR_INVALID, "cbx_spgemm: unknown mode %d", (int)mode);
   if ((st = init_device_once())) return st;
   SpgemmConfig cfg = current_config();
-  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor, CBX_ERR_INVALID,
-                "cbx_spgemm: learned mode needs a predictor (cbx_spgemm_set_predictor)");
+  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor && cfg.sampling_rounds == 0, CBX_ERR_INVALID,
+                "cbx_spgemm: learned mode needs a predictor (cbx_spgemm_set_predictor or "
+                "cbx_spgemm_set_sampling_predictor)");
   return spgemm_dispatch(A, B, semiring, mode, cfg, C, stats, (hipStream_t)stream);
 }
 
@@ -305,6 +306,14 @@ extern "C" cbx_status cbx_spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, 
   return spgemm_features(A, B, feats, (hipStream_t)stream);
 }
 
+extern "C" cbx_status cbx_spgemm_estimate_sampling(const cbx_dcsc* A, const cbx_dcsc* B, int32_t nrounds,
+                                                   float* colest, double* total, void* stream) {
+  cbx_status st = check_pair(A, B, "cbx_spgemm_estimate_sampling");
+  if (st) return st;
+  CBX_RETURN_IF(nrounds < 2 || nrounds > 8, CBX_ERR_INVALID, "cbx_spgemm_estimate_sampling: nrounds must be in [2, 8]");
+  return spgemm_estimate_sampling(A, B, nrounds, colest, total, (hipStream_t)stream);
+}
+
 static float* g_predictor = nullptr;
 static std::mutex g_predictor_mu;
 
@@ -327,6 +336,21 @@ extern "C" cbx_status cbx_spgemm_set_predictor(const float* params) {
   return CBX_OK;
 }
 
+static int g_sampling_rounds = 0;
+
+int cbx::sampling_rounds_setting() {
+  std::lock_guard<std::mutex> g(g_predictor_mu);
+  return g_sampling_rounds;
+}
+
+extern "C" cbx_status cbx_spgemm_set_sampling_predictor(int32_t nrounds) {
+  CBX_RETURN_IF(nrounds != 0 && (nrounds < 2 || nrounds > 8), CBX_ERR_INVALID,
+                "cbx_spgemm_set_sampling_predictor: nrounds must be 0 or in [2, 8]");
+  std::lock_guard<std::mutex> g(g_predictor_mu);
+  g_sampling_rounds = nrounds;
+  return CBX_OK;
+}
+
 // ------------------------------------------------------------- tunables
 static std::mutex g_config_mu;
 static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f};
@@ -345,6 +369,7 @@ SpgemmConfig cbx::current_config() {
     c.row_map_share = g_config.row_map_share;
   }
   c.predictor = predictor_params();
+  c.sampling_rounds = sampling_rounds_setting();
   return c;
 }
 

@@ -29,6 +29,10 @@ struct SpgemmConfig {
   // unit size target as a fraction of the value window, and the headroom of
   // the output capacity over the predicted total.
   const float* predictor = nullptr;
+  // Learned mode without the network: with sampling_rounds > 0 the column
+  // sizes come from the min-hash estimator of estimateNNZ_sampling
+  // (sampling_estimate_launch) instead of the predictor.
+  int sampling_rounds = 0;
   float learned_target = 0.75f;
   // Operands the library produced itself (a transpose, an earlier product)
   // are valid by construction and skip the device validation pass.
@@ -41,6 +45,15 @@ struct SpgemmConfig {
 // The process-wide tunables (cbx_spgemm_set_config) with the installed
 // predictor: the starting configuration of every product.
 SpgemmConfig current_config();
+
+// Per-column min-hash estimates of nnz(C(:,j)) (cbx_sampling.hip): `scratch`
+// holds sampling_scratch_floats(A, rounds) floats; rowsA (int32 rows of A)
+// and bpos (B's rows as A stored-column positions) come from the product's
+// setup; est gets nzc(B) floats.  rounds in [2, 8].
+size_t sampling_scratch_floats(const cbx_dcsc* A, int rounds);
+cbx_status sampling_estimate_launch(const cbx_dcsc* A, const cbx_dcsc* B, int rounds, const int32_t* rowsA,
+                                    const int32_t* bpos, float* scratch, float* est, hipStream_t s);
+int sampling_rounds_setting();
 
 // Launches the predictor's inference kernel (cbx_mlp.hip).
 void mlp_infer_launch(const float* params, const float* X, int64_t n, float* out, hipStream_t s);
@@ -61,6 +74,14 @@ cbx_status check_dcsc(const cbx_dcsc* X, const char* name, cbx_location loc);
 cbx_status spgemm_flops(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop_host, hipStream_t s);
 cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, hipStream_t s);
 cbx_status spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, hipStream_t s);
+cbx_status spgemm_estimate_sampling(const cbx_dcsc* A, const cbx_dcsc* B, int rounds, float* est, double* total,
+                                    hipStream_t s);
+
+// Structural check of a device DCSC (the rules of cbx_dcsc_check) with
+// caller-provided counters: the launch clears cnt[0..3) and runs the check on
+// s; the result call waits for s and reads them.
+void dcsc_check_launch(const cbx_dcsc* X, unsigned long long* cnt, hipStream_t s);
+cbx_status dcsc_check_result(const unsigned long long* cnt, hipStream_t s);
 
 cbx_status spgemm_dispatch(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring sr, cbx_spgemm_mode mode,
                            const SpgemmConfig& cfg, cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s);

@@ -1381,18 +1381,20 @@ __global__ void k_col_blocks(const int64_t* __restrict__ cpB, int64_t nzcB, cons
 // lambda = flop_j * w[b] / 4096, g_b = 4096 (1 - e^-lambda) distinct rows
 // (saturating for dense columns, proportional to w for sparse ones); the
 // predicted column size is spread over the blocks in proportion to g_b.  A
-// unit closes when its expected outputs would pass `target`, its expected
-// products would pass `fcap`, or its span would pass rblk blocks.  Capping
-// the products keeps every unit's counting sweep short, so no unit holds its
-// successors' look-back for long.  Pass 1 counts (out == null), pass 2
-// writes at `base`.
+// block expected to hold at least direct_flop products forms a unit of its
+// own (its rows fit the value window: one sweep, no counting sweep).  Other
+// units close when their expected outputs would pass `target`, their
+// expected products would pass `fcap`, or their span would pass rblk blocks.
+// Capping the products keeps every unit's counting sweep short, so no unit
+// holds its successors' look-back for long.  Pass 1 counts (out == null),
+// pass 2 writes at `base`.
 __device__ __forceinline__ float distinct_rows(float flop, float wb) {
   return (float)kBR * (1.f - __expf(-flop * wb * (1.f / (float)kBR)));
 }
 
 __device__ __forceinline__ int cut_units(int col, float pred, float flop, int lo, int hi,
                                          const float* __restrict__ w, float target, float fcap, int rblk,
-                                         UnitL* out) {
+                                         float direct_flop, UnitL* out) {
   float gsum = 0.f;
   for (int b = lo; b <= hi; ++b) gsum += distinct_rows(flop, w[b]);
   const float scale = gsum > 0.f ? pred / gsum : 0.f;
@@ -1400,6 +1402,18 @@ __device__ __forceinline__ int cut_units(int col, float pred, float flop, int lo
   float acc = 0.f, accf = 0.f;
   for (int b = lo; b <= hi; ++b) {
     const float e = scale * distinct_rows(flop, w[b]), ef = flop * w[b];
+    if (ef >= direct_flop) {  // more products than a unit holds in registers: a block of its own, one sweep
+      if (b > start) {
+        if (out) out[n] = UnitL{col, start, b};
+        ++n;
+      }
+      if (out) out[n] = UnitL{col, b, b + 1};
+      ++n;
+      start = b + 1;
+      acc = 0.f;
+      accf = 0.f;
+      continue;
+    }
     if (b > start && (acc + e > target || accf + ef > fcap || b - start >= rblk)) {
       if (out) out[n] = UnitL{col, start, b};
       ++n;
@@ -1410,14 +1424,17 @@ __device__ __forceinline__ int cut_units(int col, float pred, float flop, int lo
     acc += e;
     accf += ef;
   }
-  if (out) out[n] = UnitL{col, start, hi + 1};
-  return n + 1;
+  if (start <= hi) {
+    if (out) out[n] = UnitL{col, start, hi + 1};
+    ++n;
+  }
+  return n;
 }
 
 __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, int64_t small_max,
                                 const float* __restrict__ logpred, const int32_t* __restrict__ blo,
                                 const int32_t* __restrict__ bhi, const float* __restrict__ w, float target,
-                                float heavy_pred, float fcap, int rblk, int64_t m,
+                                float heavy_pred, float fcap, int rblk, float direct_flop, int64_t m,
                                 const int64_t* __restrict__ ubase, int64_t* __restrict__ ucount,
                                 UnitL* __restrict__ units, float* __restrict__ pred_nnz,
                                 uint8_t* __restrict__ heavy_flag) {
@@ -1443,10 +1460,10 @@ __global__ void k_learned_units(const int64_t* __restrict__ flop, int64_t nzcB, 
     return;
   }
   if (!units) {
-    ucount[j] = cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, nullptr);
+    ucount[j] = cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, direct_flop, nullptr);
     pred_nnz[j] = pred;
   } else {
-    cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, units + ubase[j]);
+    cut_units((int)j, pred, (float)flop[j], blo[j], bhi[j], w, target, fcap, rblk, direct_flop, units + ubase[j]);
   }
 }
 
@@ -1525,7 +1542,40 @@ __global__ __launch_bounds__(NT) void k_num_learned(
   int tot = 0, nseg = 0;
   const bool valid = (t_hi - t_lo) <= RBLK;
   if (!valid && threadIdx.x == 0) atomicOr(err, 64);
-  if (single_chunk) {
+  // A unit whose rows fit the value window accumulates at its rows in a
+  // single sweep (its size, known after that sweep, never exceeds the window)
+  const bool direct = valid && (t_hi - t_lo) * kBR <= KMAX;
+  const int ndirect = direct ? (t_hi - t_lo) * kBR : 0;
+  if (direct) {
+    for (int i = threadIdx.x; i < ndirect; i += NT) vals[i] = SR::ident();
+    // (the staging barriers below order the window's initialisation before any accumulation)
+    auto acc_direct = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
+      int32_t r[U];
+      T a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        r[u] = -1;
+        if (idx[u] >= 0) {
+          r[u] = rowsA[idx[u]] - row0;
+          a[u] = valA[idx[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (r[u] >= 0) {
+          atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+          SR::acc(&vals[r[u]], SR::mul(a[u], bvs[u]));
+        }
+    };
+    for (int64_t eb = b0; eb < b1; eb += NT) {
+      int64_t base;
+      int len;
+      T bv;
+      load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+      tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+      visit_staged<T, NT, U>(st, nseg, tot, acc_direct);
+    }
+  } else if (single_chunk) {
     int64_t base;
     int len;
     T bv;
@@ -1631,6 +1681,16 @@ __global__ __launch_bounds__(NT) void k_num_learned(
   const int64_t out = s_excl + hcs[col];
   const bool can_write = out + cnt <= cap;
   if (!can_write && threadIdx.x == 0) atomicOr(err, 512);
+  if (direct) {  // values sit at their rows: one thread per row, stores stay coalesced
+    if (can_write)
+      for (int r = threadIdx.x; r < ndirect; r += NT) {
+        if (!((bits[r >> 5] >> (r & 31)) & 1u)) continue;
+        const int q = rank_of(r);
+        irC[out + q] = (int64_t)row0 + r;
+        valC[out + q] = SR::out(vals[r]);
+      }
+    return;
+  }
   // values, in windows of KMAX ranks (one window unless the unit overflowed)
   for (q0 = 0; q0 < cnt; q0 += KMAX) {
     const int wn = min(KMAX, cnt - q0);
@@ -1718,6 +1778,12 @@ __global__ void k_add_col_totals(int64_t* __restrict__ nnzcol, const unsigned lo
                                  int64_t n) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j < n) nnzcol[j] += (int64_t)col_total[j];
+}
+
+// logpred[j] = log2(1 + est[j]): an estimated column size in the network's output form.
+__global__ void k_log2p1(const float* __restrict__ est, int64_t n, float* __restrict__ logpred) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j < n) logpred[j] = log2f(1.0f + fmaxf(est[j], 0.0f));
 }
 
 __global__ void k_sum_f32(const float* __restrict__ v, int64_t n, double* __restrict__ out) {
@@ -1894,9 +1960,19 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_hcols, nzcB, s, &hcols));
   CBX_TRY(dalloc(b_nh, 1, s, &nh_dev));
   CBX_TRY(dalloc(b_hu, 2, s, &hucnt));
-  // prediction
-  k_features<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, m, feats);
-  mlp_infer_launch(cfg.predictor, feats, nzcB, logpred, s);
+  // prediction: the network on the columns' features, or (without it) the
+  // min-hash estimates of estimateNNZ_sampling
+  if (cfg.sampling_rounds > 0) {
+    DevBuf b_samp, b_est;
+    float *samp, *est;
+    CBX_TRY(dalloc(b_samp, sampling_scratch_floats(A, cfg.sampling_rounds), s, &samp));
+    CBX_TRY(dalloc(b_est, nzcB, s, &est));
+    CBX_TRY(sampling_estimate_launch(A, B, cfg.sampling_rounds, rowsA, bpos, samp, est, s));
+    k_log2p1<<<nblk(nzcB, 256), 256, 0, s>>>(est, nzcB, logpred);
+  } else {
+    k_features<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, m, feats);
+    mlp_infer_launch(cfg.predictor, feats, nzcB, logpred, s);
+  }
   // block prior and per-column block range
   CBX_HIP(hipMemsetAsync(hist, 0, sizeof(unsigned long long) * S, s));
   k_row_hist<<<256, 256, sizeof(unsigned int) * S, s>>>(rowsA, A->nnz, S, hist);
@@ -1906,8 +1982,11 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   const float target = cfg.learned_target * (float)kLearnK;
   const float heavy_pred = (float)cfg.heavy_nnz;
   const float fcap = (float)kLearnFlopCap;
+  // a block expected to hold more products than a unit keeps in registers
+  // between its sweeps becomes a single-block unit, accumulated in one sweep
+  const float direct_flop = (float)(kLearnCR * 4 * kLearnNT);
   k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                  heavy_pred, fcap, kLearnRB, m, nullptr, ucount,
+                                                  heavy_pred, fcap, kLearnRB, direct_flop, m, nullptr, ucount,
                                                   nullptr, pred_nnz, heavy_flag);
   CBX_HIP(hipMemsetAsync(ucount + nzcB, 0, sizeof(int64_t), s));
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ucount, ubase, (int)(nzcB + 1), s));
@@ -1980,7 +2059,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_status, std::max<int64_t>(nunits, 1), s, &status));
   if (nunits)
     k_learned_units<<<nblk(nzcB, 256), 256, 0, s>>>(flop, nzcB, cfg.small_flop_max, logpred, blo, bhi, w, target,
-                                                    heavy_pred, fcap, kLearnRB, m, ubase, nullptr,
+                                                    heavy_pred, fcap, kLearnRB, direct_flop, m, ubase, nullptr,
                                                     units, nullptr, nullptr);
   // output capacity from the predictions (plus the exact hash-tier sizes)
   const int64_t cap = hash_total + (int64_t)(pred_sum * (1.0 + cfg.capacity_margin)) + (1 << 20);
@@ -1996,6 +2075,10 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
     rm.p = nullptr;
     release_cache();
     ast = alloc_out();
+  }
+  if (ast == CBX_ERR_NOMEM) {  // predicted capacity does not fit: the exact two-phase path sizes the output
+    info.over_capacity = true;
+    return CBX_OK;
   }
   CBX_TRY(ast);
   CBX_HIP(hipMemsetAsync(status, 0, sizeof(unsigned long long) * std::max<int64_t>(nunits, 1), s));
@@ -2433,6 +2516,48 @@ cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, h
   if (nnzA) k_rows32<<<nblk(nnzA, 256), 256, 0, s>>>(A->ir, nnzA, rowsA);
   if (nnzB) k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
   k_features<<<wave_grid(nzcB), 256, 0, s>>>(B->cp, nzcB, bpos, A->cp, rowsA, A->m, feats);
+  CBX_HIP(hipGetLastError());
+  CBX_HIP(hipStreamSynchronize(s));
+  return CBX_OK;
+}
+
+cbx_status spgemm_estimate_sampling(const cbx_dcsc* A, const cbx_dcsc* B, int rounds, float* est, double* total,
+                                    hipStream_t s) {
+  const int64_t nzcA = A->nzc, nzcB = B->nzc, nnzA = A->nnz, nnzB = B->nnz;
+  if (total) *total = 0.0;
+  if (nzcB == 0) return CBX_OK;
+  DevBuf b_colmap, b_rows, b_bpos, b_err, b_vcnt, b_samp, b_est, b_sum;
+  int32_t *colmap, *rowsA, *bpos;
+  int* err;
+  unsigned long long* vcnt;
+  float *samp, *e;
+  double* dsum;
+  CBX_TRY(dalloc(b_colmap, A->n, s, &colmap));
+  CBX_TRY(dalloc(b_rows, nnzA, s, &rowsA));
+  CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
+  CBX_TRY(dalloc(b_err, 1, s, &err));
+  CBX_TRY(dalloc(b_vcnt, 6, s, &vcnt));
+  CBX_TRY(dalloc(b_samp, sampling_scratch_floats(A, rounds), s, &samp));
+  CBX_TRY(dalloc(b_sum, 1, s, &dsum));
+  e = est;
+  if (!e) CBX_TRY(dalloc(b_est, nzcB, s, &e));
+  CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  CBX_TRY(launch_validate(A, 4, vcnt, err, s));
+  CBX_TRY(launch_validate(B, 5, vcnt + 3, err, s));
+  int herr = 0;
+  CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  if (herr) return set_error(CBX_ERR_INVALID, "cbx_spgemm_estimate_sampling: operand is not a valid DCSC");
+  k_fill_i32<<<nblk(A->n, 256), 256, 0, s>>>(colmap, A->n, -1);
+  if (nzcA) k_colmap<<<nblk(nzcA, 256), 256, 0, s>>>(A->jc, nzcA, colmap);
+  if (nnzA) k_rows32<<<nblk(nnzA, 256), 256, 0, s>>>(A->ir, nnzA, rowsA);
+  if (nnzB) k_bpos<<<nblk(nnzB, 256), 256, 0, s>>>(B->ir, nnzB, colmap, bpos);
+  CBX_TRY(sampling_estimate_launch(A, B, rounds, rowsA, bpos, samp, e, s));
+  if (total) {
+    CBX_HIP(hipMemsetAsync(dsum, 0, sizeof(double), s));
+    k_sum_f32<<<256, 256, 0, s>>>(e, nzcB, dsum);
+    CBX_HIP(hipMemcpyAsync(total, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
+  }
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipStreamSynchronize(s));
   return CBX_OK;

@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <type_traits>
 
 #include "cbx_device.h"
@@ -186,6 +187,12 @@ __global__ void k_sv_emit(const uint32_t* __restrict__ bits, const int32_t* __re
 
 inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+#define CBX_TRY_SV(x)              \
+  do {                             \
+    cbx_status st_ = (x);          \
+    if (st_ != CBX_OK) return st_; \
+  } while (0)
+
 template <int K, class T>
 T ident_of() {
   if (K == kPlusTimes) return std::is_floating_point<T>::value ? (T)-0.0 : (T)0;
@@ -193,59 +200,92 @@ T ident_of() {
   return std::numeric_limits<T>::lowest();
 }
 
-template <int K, class T>
-cbx_status spmspv_typed(const cbx_dcsc* A, bool boolA, const int32_t* xind, const T* xval, int32_t nx, int32_t* yind,
-                        T* yval, int32_t* ny, hipStream_t s) {
-  const int64_t m = A->m, n = A->n, nw = m / 32 + 1;
-  *ny = 0;
-  if (nx == 0 || A->nnz == 0) return CBX_OK;
+__global__ void k_sv_zero(int32_t* __restrict__ a, int32_t* __restrict__ b) {
+  if (threadIdx.x == 0) {
+    *a = 0;
+    *b = 0;
+  }
+}
+
+// Scratch of one SpMSpV: the column map of A (n), the dense accumulator (m
+// values) and presence bitmap (nw words), per-word counts and their scan
+// (nw + 1 each), a flag and the scan's temporary storage.
+struct SvWork {
   int32_t *colmap = nullptr, *cnt = nullptr, *off = nullptr;
-  T* acc = nullptr;
+  void* acc = nullptr;
   uint32_t* bits = nullptr;
   int* bad = nullptr;
   void* tmp = nullptr;
   size_t tb = 0;
-  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, off, (int)(nw + 1), s));
-  auto cleanup = [&]() {
-    for (void* p : {(void*)colmap, (void*)cnt, (void*)off, (void*)acc, (void*)bits, (void*)bad, tmp})
-      if (p) (void)hipFreeAsync(p, s);
-  };
-  if (hipMallocAsync(&colmap, sizeof(int32_t) * std::max<int64_t>(n, 1), s) != hipSuccess ||
-      hipMallocAsync(&cnt, sizeof(int32_t) * (nw + 1), s) != hipSuccess ||
-      hipMallocAsync(&off, sizeof(int32_t) * (nw + 1), s) != hipSuccess ||
-      hipMallocAsync(&acc, sizeof(T) * std::max<int64_t>(m, 1), s) != hipSuccess ||
-      hipMallocAsync(&bits, sizeof(uint32_t) * nw, s) != hipSuccess ||
-      hipMallocAsync(&bad, sizeof(int), s) != hipSuccess || hipMallocAsync(&tmp, std::max<size_t>(tb, 16), s) != hipSuccess) {
-    cleanup();
-    return set_error(CBX_ERR_NOMEM, "cbx_spmspv: workspace allocation failed");
-  }
+};
+
+inline int64_t sv_words(int64_t m) { return m / 32 + 1; }
+
+template <class T>
+size_t sv_scan_bytes(int64_t m) {
+  size_t tb = 0;
+  const int64_t nw = sv_words(m);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (int32_t*)nullptr, (int32_t*)nullptr, (int)(nw + 1));
+  return std::max<size_t>(tb, 16);
+}
+
+// The product with its scratch in `w` (every array is initialised here, so
+// the scratch may hold anything on entry).
+template <int K, class T>
+cbx_status spmspv_run(const cbx_dcsc* A, bool boolA, const int32_t* xind, const T* xval, int32_t nx, int32_t* yind,
+                      T* yval, int32_t* ny, const SvWork& w, hipStream_t s) {
+  const int64_t m = A->m, n = A->n, nw = sv_words(m);
+  T* acc = (T*)w.acc;
   int hbad = 0;
-  CBX_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
-  k_sv_check_x<<<nb(nx, 256), 256, 0, s>>>(xind, nx, n, bad);
-  CBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  k_sv_zero<<<1, 64, 0, s>>>(w.bad, w.cnt + nw);
+  k_sv_check_x<<<nb(nx, 256), 256, 0, s>>>(xind, nx, n, w.bad);
+  CBX_HIP(hipMemcpyAsync(&hbad, w.bad, sizeof(int), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipStreamSynchronize(s));
-  if (hbad) {
-    cleanup();
-    return set_error(CBX_ERR_INVALID, "cbx_spmspv: x indices must be ascending, unique and below ncols");
-  }
-  k_sv_colmap<<<nb(n, 256), 256, 0, s>>>(A->jc, A->nzc, n, colmap);
-  k_sv_colmap_scatter<<<nb(A->nzc, 256), 256, 0, s>>>(A->jc, A->nzc, colmap);
-  k_sv_init<T><<<nb(std::max(m, nw), 256), 256, 0, s>>>(acc, m, ident_of<K, T>(), bits, nw);
+  if (hbad) return set_error(CBX_ERR_INVALID, "cbx_spmspv: x indices must be ascending, unique and below ncols");
+  k_sv_colmap<<<nb(n, 256), 256, 0, s>>>(A->jc, A->nzc, n, w.colmap);
+  k_sv_colmap_scatter<<<nb(A->nzc, 256), 256, 0, s>>>(A->jc, A->nzc, w.colmap);
+  k_sv_init<T><<<nb(std::max(m, nw), 256), 256, 0, s>>>(acc, m, ident_of<K, T>(), w.bits, nw);
   const unsigned g = std::min<unsigned>(nb(nx, 256), 2048u);
   if (boolA)
-    k_sv_expand<K, T, true, 256, 4><<<g, 256, 0, s>>>(A->cp, A->ir, A->val, colmap, xind, xval, nx, acc, bits);
+    k_sv_expand<K, T, true, 256, 4><<<g, 256, 0, s>>>(A->cp, A->ir, A->val, w.colmap, xind, xval, nx, acc, w.bits);
   else
-    k_sv_expand<K, T, false, 256, 4><<<g, 256, 0, s>>>(A->cp, A->ir, A->val, colmap, xind, xval, nx, acc, bits);
-  k_sv_wordcnt<<<nb(nw, 256), 256, 0, s>>>(bits, nw, cnt);
-  CBX_HIP(hipMemsetAsync(cnt + nw, 0, sizeof(int32_t), s));
-  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, off, (int)(nw + 1), s));
-  k_sv_emit<T><<<nb(nw, 256), 256, 0, s>>>(bits, off, nw, acc, yind, yval);
-  CBX_HIP(hipMemcpyAsync(ny, off + nw, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    k_sv_expand<K, T, false, 256, 4><<<g, 256, 0, s>>>(A->cp, A->ir, A->val, w.colmap, xind, xval, nx, acc, w.bits);
+  k_sv_wordcnt<<<nb(nw, 256), 256, 0, s>>>(w.bits, nw, w.cnt);
+  size_t tb = w.tb;
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(w.tmp, tb, w.cnt, w.off, (int)(nw + 1), s));
+  k_sv_emit<T><<<nb(nw, 256), 256, 0, s>>>(w.bits, w.off, nw, acc, yind, yval);
+  CBX_HIP(hipMemcpyAsync(ny, w.off + nw, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipStreamSynchronize(s));
+  return CBX_OK;
+}
+
+template <int K, class T>
+cbx_status spmspv_typed(const cbx_dcsc* A, bool boolA, const int32_t* xind, const T* xval, int32_t nx, int32_t* yind,
+                        T* yval, int32_t* ny, hipStream_t s) {
+  const int64_t m = A->m, n = A->n, nw = sv_words(m);
+  *ny = 0;
+  if (nx == 0 || A->nnz == 0) return CBX_OK;
+  SvWork w;
+  w.tb = sv_scan_bytes<T>(m);
+  auto cleanup = [&]() {
+    for (void* p : {(void*)w.colmap, (void*)w.cnt, (void*)w.off, w.acc, (void*)w.bits, (void*)w.bad, w.tmp})
+      if (p) (void)hipFreeAsync(p, s);
+  };
+  if (hipMallocAsync(&w.colmap, sizeof(int32_t) * std::max<int64_t>(n, 1), s) != hipSuccess ||
+      hipMallocAsync(&w.cnt, sizeof(int32_t) * (nw + 1), s) != hipSuccess ||
+      hipMallocAsync(&w.off, sizeof(int32_t) * (nw + 1), s) != hipSuccess ||
+      hipMallocAsync(&w.acc, sizeof(T) * std::max<int64_t>(m, 1), s) != hipSuccess ||
+      hipMallocAsync(&w.bits, sizeof(uint32_t) * nw, s) != hipSuccess ||
+      hipMallocAsync(&w.bad, sizeof(int), s) != hipSuccess || hipMallocAsync(&w.tmp, w.tb, s) != hipSuccess) {
+    cleanup();
+    (void)hipGetLastError();
+    return set_error(CBX_ERR_NOMEM, "cbx_spmspv: workspace allocation failed");
+  }
+  const cbx_status st = spmspv_run<K, T>(A, boolA, xind, xval, nx, yind, yval, ny, w, s);
   cleanup();
   CBX_HIP(hipStreamSynchronize(s));
-  return CBX_OK;
+  return st;
 }
 
 }  // namespace

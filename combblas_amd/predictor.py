@@ -125,22 +125,6 @@ def ratio_params(ratio: float) -> np.ndarray:
     return P
 
 
-def sampling_ratio(A: cb.Dcsc, dA, frac: float = 0.01, seed: int = 3) -> float:
-    """Global compression nnz(C)/flop(C) of C = A*A estimated from the exact
-    sizes of a random `frac` of the columns (the idea of the reference's
-    estimateNNZ_sampling, mtSpGEMM.h:936-1054: a global estimate from a sample)."""
-    rng = np.random.default_rng(seed)
-    k = max(1, int(frac * A.nzc))
-    qs = np.sort(rng.choice(A.nzc, k, replace=False))
-    idx = np.concatenate([np.arange(A.cp[q], A.cp[q + 1]) for q in qs])
-    Bs = cb.Dcsc.from_coo(A.m, A.n, A.ir[idx], np.repeat(A.jc[qs], np.diff(A.cp)[qs]), A.val[idx], A.dtype)
-    lens = np.zeros(A.n, np.int64)
-    lens[A.jc] = np.diff(A.cp)
-    flop = int(lens[Bs.ir].sum())
-    nnz = int(cb.spgemm_symbolic(dA, Bs.to_device()).sum())
-    return nnz / max(flop, 1)
-
-
 def load() -> np.ndarray:
     if not MODEL_PATH.exists():
         raise FileNotFoundError(f"{MODEL_PATH} missing: train it with `python -m combblas_amd.predictor train`")

@@ -256,6 +256,22 @@ cbx_status cbx_spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nn
  *   log2(1 + span*(1-exp(-flop_j/span)))   (k ranges over B(:,j)). */
 cbx_status cbx_spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, void* stream);
 
+/* estimateNNZ_sampling (mtSpGEMM.h:936-1054) on the device, kept per column:
+ * colest[j] (device float[nzc_B], may be NULL) = the min-hash estimate of
+ * nnz(C(:,j)) from nrounds exponential samples per row of A (the reference's
+ * sample stream: minstd_rand0 from seed 1, -log(1-u)), and *total (host, may
+ * be NULL) their sum, the reference's return value before its cast to int64.
+ * Matches the reference up to the last bits of -log.  nrounds in [2, 8]
+ * (the reference's default is 5). */
+cbx_status cbx_spgemm_estimate_sampling(const cbx_dcsc* A, const cbx_dcsc* B, int32_t nrounds, float* colest,
+                                        double* total, void* stream);
+
+/* Learned mode without the network: with nrounds in [2, 8] the single pass
+ * sizes its units from the per-column estimates of
+ * cbx_spgemm_estimate_sampling instead of the installed MLP (the method
+ * without learning, for comparison); 0 returns to the MLP. */
+cbx_status cbx_spgemm_set_sampling_predictor(int32_t nrounds);
+
 /* The predictor: a 6 -> 64 -> 64 -> 1 ReLU MLP in FP32 predicting
  * log2(1 + nnz_j) from standardised features.  Parameters (device float
  * array of CBX_MLP_PARAMS): W1[64][6] b1[64] W2[64][64] b2[64] W3[64] b3

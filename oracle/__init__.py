@@ -68,6 +68,11 @@ def _load_oracle() -> C.CDLL:
         L.orc_spmv_dense.restype = C.c_int32
         L.orc_galerkin_direct.argtypes = [C.c_int32, P(cbx_dcsc)]
         L.orc_galerkin_direct.restype = C.c_int32
+        L.orc_estimate_sampling.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32, C.c_void_p, P(C.c_float),
+                                             P(C.c_int64)]
+        L.orc_estimate_sampling.restype = C.c_int32
+        L.orc_exp_samples.argtypes = [C.c_int64, C.c_void_p]
+        L.orc_exp_samples.restype = None
         L.orc_max_threads.restype = C.c_int32
         L.orc_set_threads.argtypes = [C.c_int32]
         _orc = L
@@ -96,6 +101,13 @@ def _load_ref() -> C.CDLL:
             getattr(L, nm).restype = C.c_int32
         L.ref_bfs.argtypes = [P(cbx_dcsc), C.c_int64, C.c_void_p]
         L.ref_bfs.restype = C.c_int32
+        L.ref_time_hybrid_dcsc_i64.argtypes = [P(cbx_dcsc), P(cbx_dcsc), P(C.c_double), P(C.c_double),
+                                               P(C.c_int64)]
+        L.ref_time_hybrid_dcsc_i64.restype = C.c_int32
+        L.ref_bfs_threaded.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, P(C.c_double)]
+        L.ref_bfs_threaded.restype = C.c_int32
+        L.ref_estimate_sampling.argtypes = [P(cbx_dcsc), P(cbx_dcsc), C.c_int32]
+        L.ref_estimate_sampling.restype = C.c_int64
         L.ref_spmv_dense.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32]
         L.ref_spmv_dense.restype = C.c_int32
         L.ref_mcl_prune.argtypes = [P(cbx_dcsc), C.c_double, C.c_int64, C.c_int64, C.c_double, P(cbx_dcsc)]
@@ -164,6 +176,30 @@ def ref_time_hybrid_i64(A: Dcsc, B: Dcsc) -> tuple[float, int]:
     if L.ref_time_hybrid_i64(C.byref(sa), C.byref(sb), C.byref(secs), C.byref(nnz)) != 0:
         raise RuntimeError("ref_time_hybrid_i64 failed")
     return secs.value, int(nnz.value)
+
+
+def ref_time_hybrid_dcsc_i64(A: Dcsc, B: Dcsc) -> tuple[float, float, int]:
+    """(seconds of the genuine LocalHybridSpGEMM<PlusTimes<int64>>, seconds of it plus the
+    SpDCCols(SpTuples) conversion, nnz) on B's columns (oracle/_ref)."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    t1, t2, nnz = C.c_double(0), C.c_double(0), C.c_int64(0)
+    if L.ref_time_hybrid_dcsc_i64(C.byref(sa), C.byref(sb), C.byref(t1), C.byref(t2), C.byref(nnz)) != 0:
+        raise RuntimeError("ref_time_hybrid_dcsc_i64 failed")
+    return t1.value, t2.value, int(nnz.value)
+
+
+def ref_bfs_threaded(A: Dcsc, roots, splits: int) -> tuple[np.ndarray, int, float]:
+    """The reference's multithreaded BFS level loop (RowSplit + generic_gespmv_threaded) over
+    `roots`: (parents of the last root, total levels, seconds of the level loops)."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    r = np.ascontiguousarray(roots, np.int64)
+    par = np.zeros(A.n, np.int64)
+    secs = C.c_double(0)
+    lv = L.ref_bfs_threaded(C.byref(sa), r.ctypes.data, int(r.size), int(splits), par.ctypes.data, C.byref(secs))
+    return par, int(lv), secs.value
 
 
 def column_slice(B: Dcsc, c0: int, c1: int) -> Dcsc:
@@ -293,6 +329,36 @@ def ref_symbolic(A: Dcsc, B: Dcsc) -> tuple[np.ndarray, np.ndarray]:
     nnz = np.zeros(B.nzc, np.int64)
     L.ref_symbolic(C.byref(sa), C.byref(sb), flop.ctypes.data, nnz.ctypes.data)
     return flop, nnz
+
+
+def estimate_sampling(A: Dcsc, B: Dcsc, nrounds: int = 5) -> tuple[np.ndarray, float, int]:
+    """estimateNNZ_sampling (mtSpGEMM.h:936-1054) restated: (per stored column
+    of B the min-hash estimate of nnz(C(:,j)), their float sum, the int64
+    estimate the reference returns)."""
+    L = _load_oracle()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    col = np.zeros(B.nzc, np.float32)
+    tot, est = C.c_float(0), C.c_int64(0)
+    st = L.orc_estimate_sampling(C.byref(sa), C.byref(sb), nrounds, col.ctypes.data, C.byref(tot), C.byref(est))
+    if st != 0:
+        raise RuntimeError(f"orc_estimate_sampling failed with status {st}")
+    return col, float(tot.value), int(est.value)
+
+
+def exp_samples(count: int) -> np.ndarray:
+    """The first `count` exponential(1) draws of estimateNNZ_sampling's stream (float32)."""
+    out = np.zeros(count, np.float32)
+    _load_oracle().orc_exp_samples(count, out.ctypes.data)
+    return out
+
+
+def ref_estimate_sampling(A: Dcsc, B: Dcsc, nrounds: int = 5) -> int:
+    """The genuine estimateNNZ_sampling's int64 estimate (only where oracle/_ref was built)."""
+    L = _load_ref()
+    sa, ka = A._cstruct()
+    sb, kb = B._cstruct()
+    return int(L.ref_estimate_sampling(C.byref(sa), C.byref(sb), nrounds))
 
 
 def mcl_prune(A: Dcsc, thr: float = 1e-4, select: int = 1000, recover: int = 0, pct: float = 0.9,

@@ -22,6 +22,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <random>
 #include <functional>
 #include <cstdint>
 #include <cstdlib>
@@ -524,6 +525,82 @@ cbx_status orc_time_spgemm(const cbx_dcsc* A, const cbx_dcsc* B, int32_t semirin
   *secs = t1 - t0;
   *nnz = C.nnz;
   orc_free(&C);
+  return CBX_OK;
+}
+
+// The first `count` draws of the sample stream estimateNNZ_sampling uses
+// (mtSpGEMM.h:968-979): std::exponential_distribution<float>(1) over a
+// default-seeded std::default_random_engine.
+void orc_exp_samples(int64_t count, float* out) {
+  std::default_random_engine gen;
+  std::exponential_distribution<float> exp_dist(1.0f);
+  for (int64_t i = 0; i < count; ++i) out[i] = exp_dist(gen);
+}
+
+// estimateNNZ_sampling (mtSpGEMM.h:936-1054): Cohen's min-hash size estimate
+// with nrounds exponential samples per row of A.
+//  * samples_init[i], i < m*nrounds: the i-th draw of
+//    std::exponential_distribution<float>(1) from a default-seeded
+//    std::default_random_engine (mtSpGEMM.h:968-979; the loop is an
+//    `omp parallel for` nested in an `omp parallel` region whose threads each
+//    own a default-seeded engine, so with nested parallelism inactive -- the
+//    OpenMP default -- every thread writes the same sequential stream).  The
+//    standard library of this toolchain (libstdc++, GCC 11) draws them, as it
+//    does for the reference: minstd_rand0 (x <- 16807 x mod 2^31-1, x0 = 1),
+//    u = generate_canonical<float,24> = float(x - 1) / 2^31, -log(1 - u).
+//  * samples_mid[c][k] = min over the rows of A(:,c) of samples_init[row][k]
+//    (:991-1005), FLT_MAX for an empty column.
+//  * per stored column i of B: samples_final[k] = min over B(:,i)'s rows r of
+//    samples_mid[r][k] (:1025-1035); colest[i] = float(nrounds-1) /
+//    (samples_final[0] + ... + samples_final[nrounds-1]) summed in k order in
+//    float (:1037-1040).
+//  * the estimate: colest summed in float in column order (the reference's
+//    `reduction(+:nnzest)` is that order on one thread), cast to int64.
+// colest (B.nzc floats) may be null; *total receives the float sum.
+cbx_status orc_estimate_sampling(const cbx_dcsc* A, const cbx_dcsc* B, int32_t nrounds, float* colest,
+                                 float* total, int64_t* estimate) {
+  if (!A || !B || A->n != B->m || nrounds < 1) return CBX_ERR_INVALID;
+  if (estimate) *estimate = 0;
+  if (total) *total = 0.0f;
+  if (A->nnz == 0 || B->nnz == 0) {
+    if (colest) std::fill(colest, colest + B->nzc, 0.0f);
+    return CBX_OK;
+  }
+  const int64_t m = A->m, p = A->n, r = nrounds;
+  std::vector<float> init((size_t)(m * r));
+  {
+    std::default_random_engine gen;
+    std::exponential_distribution<float> exp_dist(1.0f);
+    for (int64_t i = 0; i < m * r; ++i) init[i] = exp_dist(gen);
+  }
+  std::vector<float> mid((size_t)(p * r), std::numeric_limits<float>::max());
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t q = 0; q < A->nzc; ++q) {
+    float* dst = mid.data() + A->jc[q] * r;
+    for (int64_t e = A->cp[q]; e < A->cp[q + 1]; ++e) {
+      const float* src = init.data() + A->ir[e] * r;
+      for (int64_t k = 0; k < r; ++k)
+        if (src[k] < dst[k]) dst[k] = src[k];
+    }
+  }
+  std::vector<float> est((size_t)B->nzc);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < B->nzc; ++i) {
+    std::vector<float> fin((size_t)r, std::numeric_limits<float>::max());
+    for (int64_t e = B->cp[i]; e < B->cp[i + 1]; ++e) {
+      const float* src = mid.data() + B->ir[e] * r;
+      for (int64_t k = 0; k < r; ++k)
+        if (src[k] < fin[k]) fin[k] = src[k];
+    }
+    float c = 0.0f;
+    for (int64_t k = 0; k < r; ++k) c += fin[k];
+    est[i] = static_cast<float>(r - 1) / c;
+  }
+  float sum = 0.0f;
+  for (int64_t i = 0; i < B->nzc; ++i) sum += est[i];
+  if (colest) std::copy(est.begin(), est.end(), colest);
+  if (total) *total = sum;
+  if (estimate) *estimate = static_cast<int64_t>(sum);
   return CBX_OK;
 }
 

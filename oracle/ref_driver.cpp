@@ -249,6 +249,18 @@ int ref_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop, int64_t* n
   return 0;
 }
 
+// estimateNNZ_sampling (mtSpGEMM.h:936-1054), the genuine template: its
+// global estimate of nnz(A*B) with nrounds samples per row.
+int64_t ref_estimate_sampling(const cbx_dcsc* A, const cbx_dcsc* B, int32_t nrounds) {
+  ensure_mpi();
+  auto* MA = make_mat<int64_t, int64_t>(*A);
+  auto* MB = make_mat<int64_t, int64_t>(*B);
+  const int64_t est = estimateNNZ_sampling(*MA, *MB, nrounds);
+  delete MA;
+  delete MB;
+  return est;
+}
+
 // Times combblas::LocalHybridSpGEMM<PlusTimesSRing<int64,int64>,int64>(A, B)
 // alone (operands converted to SpDCCols beforehand; the returned SpTuples is
 // deleted after the clock stops).
@@ -261,6 +273,29 @@ int ref_time_hybrid_i64(const cbx_dcsc* A, const cbx_dcsc* B, double* secs, int6
   const double t1 = omp_get_wtime();
   *secs = t1 - t0;
   *nnz = T->getnnz();
+  delete T;
+  delete MA;
+  delete MB;
+  return 0;
+}
+
+// LocalHybridSpGEMM<PlusTimesSRing<int64,int64>> alone (secs_product) and
+// together with the SpDCCols(SpTuples) conversion the SUMMA drivers apply
+// to its output (SpDCCols.cpp:109-184; secs_dcsc covers both).
+int ref_time_hybrid_dcsc_i64(const cbx_dcsc* A, const cbx_dcsc* B, double* secs_product, double* secs_dcsc,
+                             int64_t* nnz) {
+  ensure_mpi();
+  auto* MA = make_mat<int64_t, int64_t>(*A);
+  auto* MB = make_mat<int64_t, int64_t>(*B);
+  const double t0 = omp_get_wtime();
+  SpTuples<int64_t, int64_t>* T = LocalHybridSpGEMM<PlusTimesSRing<int64_t, int64_t>, int64_t>(*MA, *MB, false, false);
+  const double t1 = omp_get_wtime();
+  auto* D = new SpDCCols<int64_t, int64_t>(*T, false);
+  const double t2 = omp_get_wtime();
+  *secs_product = t1 - t0;
+  *secs_dcsc = t2 - t0;
+  *nnz = D->getnnz();
+  delete D;
   delete T;
   delete MA;
   delete MB;
@@ -328,6 +363,58 @@ int32_t ref_bfs(const cbx_dcsc* A, int64_t root, int64_t* parents) {
   }
   delete MA;
   return it;
+}
+
+// The TopDownBFS level loop with the threaded local SpMSpV of a
+// multithreaded run: the matrix row-split into `splits` pieces
+// (SpParMat::ActivateThreading -> SpDCCols::RowSplit, TopDownBFS.cpp:178)
+// and each level's product through generic_gespmv_threaded with one
+// recipient (Friends.h:181-320; LocalSpMV, ParFriends.h:1965), default SPA
+// as in SpMV(Aeff, fringe, optbuf) (TopDownBFS.cpp:445).  For `nroots`
+// roots: parents of the last root, the summed seconds of the level loops
+// (the row split is done once, outside the clock) and the summed levels.
+int32_t ref_bfs_threaded(const cbx_dcsc* A, const int64_t* roots, int32_t nroots, int32_t splits, int64_t* parents,
+                         double* secs) {
+  ensure_mpi();
+  auto* MA = make_mat<bool, uint8_t>(*A);
+  MA->RowSplit(splits);
+  const int64_t n = A->n;
+  int32_t total_levels = 0;
+  double t = 0.0;
+  std::vector<int32_t> ind, nind;
+  std::vector<int64_t> num, nnum;
+  for (int32_t q = 0; q < nroots; ++q) {
+    const double t0 = omp_get_wtime();
+    std::fill(parents, parents + n, (int64_t)-1);
+    ind.assign(1, (int32_t)roots[q]);
+    while (!ind.empty()) {
+      num.assign(ind.begin(), ind.end());  // setNumToInd
+      int32_t* sind = nullptr;
+      int64_t* snum = nullptr;
+      int* sdis = nullptr;
+      PreAllocatedSPA<int64_t> spa;
+      const int ny = generic_gespmv_threaded<SelectMaxSRing<bool, int64_t>>(*MA, ind.data(), num.data(),
+                                                                             (int32_t)ind.size(), sind, snum, sdis,
+                                                                             1, spa);
+      nind.clear();
+      nnum.clear();
+      for (int i = 0; i < ny; ++i)  // EWiseMult(fringe, parents, exclude=true, -1)
+        if (parents[sind[i]] == -1) {
+          nind.push_back(sind[i]);
+          nnum.push_back(snum[i]);
+        }
+      for (size_t i = 0; i < nind.size(); ++i) parents[nind[i]] = nnum[i];  // parents.Set(fringe)
+      delete[] sind;
+      delete[] snum;
+      delete[] sdis;
+      ind.swap(nind);
+      ++total_levels;
+    }
+    t += omp_get_wtime() - t0;
+  }
+  if (secs) *secs = t;
+  delete MA;
+  return total_levels;
 }
 
 // dcsc_gespmv<SR>(A, x, y) (Friends.h:63-77), the genuine template.

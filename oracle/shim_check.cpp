@@ -2,8 +2,10 @@
 //
 // Integration check of include/cbx_combblas.hpp: genuine CombBLAS objects
 // (SpDCCols built from an R-MAT matrix) go through the GPU drop-ins and
-// through the reference's own LocalHybridSpGEMM / LocalSpGEMMHash on the
-// CPU; the results must be identical (integer semirings, boolean).  Built by
+// through the reference's own LocalHybridSpGEMM / LocalSpGEMMHash,
+// SpMXSpV and generic_gespmv_threaded (on a row-split boolean matrix, the
+// BFS form) on the CPU; the results must be identical (integer semirings,
+// boolean).  Built by
 // oracle/Makefile (target `ref`) into oracle/_ref/shim_check against the
 // reference headers where they lie; exits 77 when no GPU is visible.
 #include <mpi.h>
@@ -70,6 +72,58 @@ bool check(const cbx_dcsc& G, const char* name) {
   return ok;
 }
 
+// A sparse x over every 7th column of A, integer values in [-5, 5].
+template <class VT>
+void make_x(int64_t n, std::vector<int32_t>& xi, std::vector<VT>& xv) {
+  for (int32_t j = 0; j < n; j += 7) {
+    xi.push_back(j);
+    xv.push_back((VT)(j % 11 - 5));
+  }
+}
+
+template <class SR, class NUM, class VT>
+bool check_spmxspv(const cbx_dcsc& G, const char* name) {
+  auto* A = from_cbx<NUM>(G);
+  std::vector<int32_t> xi;
+  std::vector<VT> xv;
+  make_x<VT>(G.n, xi, xv);
+  std::vector<int32_t> ri, gi;
+  std::vector<VT> rv, gv;
+  PreAllocatedSPA<VT> spa;
+  SpMXSpV<SR>(*A->GetDCSC(), (int32_t)G.m, xi.data(), xv.data(), (int32_t)xi.size(), ri, rv, spa);
+  SpMXSpV_HIP<SR>(*A->GetDCSC(), (int32_t)G.m, xi.data(), xv.data(), (int32_t)xi.size(), gi, gv, spa);
+  const bool ok = !ri.empty() && ri == gi && rv == gv;
+  std::printf("%-28s ny=%zu %s\n", name, ri.size(), ok ? "identical" : "DIFFERENT");
+  delete A;
+  return ok;
+}
+
+template <class SR, class VT>
+bool check_gespmv_threaded(const cbx_dcsc& G, int splits, int p_c, const char* name) {
+  auto* A = from_cbx<bool>(G);
+  A->RowSplit(splits);
+  std::vector<int32_t> xi;
+  std::vector<VT> xv;
+  make_x<VT>(G.n, xi, xv);
+  PreAllocatedSPA<VT> spa;
+  int32_t *ri = nullptr, *gi = nullptr;
+  VT *rv = nullptr, *gv = nullptr;
+  int *rd = nullptr, *gd = nullptr;
+  const int rn = generic_gespmv_threaded<SR>(*A, xi.data(), xv.data(), (int32_t)xi.size(), ri, rv, rd, p_c, spa);
+  const int gn = generic_gespmv_threaded_HIP<SR>(*A, xi.data(), xv.data(), (int32_t)xi.size(), gi, gv, gd, p_c, spa);
+  bool ok = rn > 0 && rn == gn && std::equal(rd, rd + p_c, gd);
+  for (int i = 0; ok && i < rn; ++i) ok = ri[i] == gi[i] && rv[i] == gv[i];
+  std::printf("%-28s splits=%d p_c=%d n=%d %s\n", name, splits, p_c, rn, ok ? "identical" : "DIFFERENT");
+  delete[] ri;
+  delete[] rv;
+  delete[] rd;
+  delete[] gi;
+  delete[] gv;
+  delete[] gd;
+  delete A;
+  return ok;
+}
+
 int main() {
   if (cbx_device_count() < 1) {
     std::printf("no GPU visible\n");
@@ -83,6 +137,11 @@ int main() {
   ok &= check<MinPlusSRing<int64_t, int64_t>, int64_t>(G, "MinPlus<int64>");
   ok &= check<SelectMaxSRing<int64_t, int64_t>, int64_t>(G, "SelectMax<int64>");
   ok &= check<PlusTimesSRing<bool, bool>, bool>(G, "PlusTimes<bool> (OR/AND)");
+  ok &= check_spmxspv<SelectMaxSRing<bool, int64_t>, bool, int64_t>(G, "SpMXSpV SelectMax<bool,int64>");
+  ok &= check_spmxspv<PlusTimesSRing<int64_t, int64_t>, int64_t, int64_t>(G, "SpMXSpV PlusTimes<int64>");
+  ok &= check_spmxspv<MinPlusSRing<int64_t, int64_t>, int64_t, int64_t>(G, "SpMXSpV MinPlus<int64>");
+  ok &= check_gespmv_threaded<SelectMaxSRing<bool, int64_t>, int64_t>(G, 32, 4, "gespmv_threaded SelectMax");
+  ok &= check_gespmv_threaded<SelectMaxSRing<bool, int64_t>, int64_t>(G, 7, 3, "gespmv_threaded SelectMax");
   cbx_dcsc_free(&G);
   MPI_Finalize();
   std::printf(ok ? "shim ok\n" : "shim MISMATCH\n");

@@ -74,3 +74,16 @@ def test_oracle_spmspv_matches_reference(semiring):
         y1 = oracle.spmspv(A, x, xv, semiring)
         y2 = oracle.ref_spmspv(A, x, xv, semiring)
         assert np.array_equal(y1[0], y2[0]) and np.array_equal(y1[1], y2[1])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("splits", [1, 4, 13])
+def test_reference_threaded_bfs_matches_oracle(splits):
+    """The CPU baseline's BFS path (RowSplit + generic_gespmv_threaded, the
+    multithreaded TopDownBFS) gives the oracle's parents and level counts
+    whatever the number of row splits."""
+    A = cb.gen_rmat(13, 16, symmetrize=True, no_self_loops=True, pattern=True).astype("bool")
+    for root in cb.bfs_roots(A, 3, seed=splits):
+        got, levels, _ = oracle.ref_bfs_threaded(A, [int(root)], splits)
+        want, it = oracle.bfs(A, int(root))
+        assert levels == it and np.array_equal(got, want)

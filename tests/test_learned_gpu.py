@@ -134,29 +134,35 @@ def test_learned_trained_model_s18_checksums():
 
 @pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
 def test_trained_network_beats_sampling_baseline_s18():
-    """What the trained network gains over the method without it (flop_j times
-    a global compression ratio sampled from 1% of the columns): per-column
-    sizes at least twice as accurate, and the single pass completes without
-    the two-phase fallback that an under-sized capacity forces; both give the
-    exact product."""
+    """What the trained network gains over the method without it, the
+    reference's own estimator (estimateNNZ_sampling's min-hash estimates,
+    5 rounds, per column): per-column sizes of the unit-tier columns at least
+    twice as accurate; both drive the single pass to the exact product (the
+    estimator may over- or under-size the capacity and fall back)."""
     A = cb.gen_rmat(18, 16)
     dA = A.to_device()
     X = cb.spgemm_features(dA, dA)
     nnz = cb.spgemm_symbolic(dA, dA).astype(np.float64)
     flop = np.exp2(X[:, 1].astype(np.float64)) - 1
     big = flop > 4096
-    ratio = predictor.sampling_ratio(A, dA)
     err = lambda pred: float(np.mean(np.abs(pred[big] - nnz[big]) / np.maximum(nnz[big], 1)))
     mlp_pred = np.exp2(cb.mlp_forward(predictor.load(), X).astype(np.float64)) - 1
-    base_pred = np.exp2(cb.mlp_forward(predictor.ratio_params(ratio), X).astype(np.float64)) - 1
-    assert err(mlp_pred) < 0.5 * err(base_pred), (err(mlp_pred), err(base_pred))
-    for P, name in ((predictor.ratio_params(ratio), "baseline"), (predictor.load(), "trained")):
-        cb.set_predictor(P)
+    est, _ = cb.spgemm_estimate_sampling(dA, dA, 5)
+    assert err(mlp_pred) < 0.5 * err(est.astype(np.float64)), (err(mlp_pred), err(est))
+    counts = nnz.astype(np.int64)
+    for name in ("baseline", "trained"):
+        if name == "baseline":
+            cb.set_predictor(None)
+            cb.set_sampling_predictor(5)
+        else:
+            cb.set_sampling_predictor(0)
+            cb.set_predictor(predictor.load())
         C, st = cb.spgemm(dA, dA, "plus_times", "learned")
-        if name == "trained":  # the baseline may under-size the light columns and fall back (still exact)
+        if name == "trained":
             assert st["learned_fallback"] == 0
-        assert product_matches(C, A, A)
+        assert product_matches(C, A, A, col_nnz=counts)
         C.release()
+    cb.set_sampling_predictor(0)
 
 
 @pytest.mark.skipif(not predictor.MODEL_PATH.exists(), reason="no trained predictor in combblas_amd/models")
