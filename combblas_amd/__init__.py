@@ -87,7 +87,7 @@ class cbx_mcl_stats(C.Structure):
 class cbx_spgemm_config(C.Structure):
     _fields_ = [("hash_max_flop", C.c_int64), ("hash_mid_flop", C.c_int64), ("tiny_flop", C.c_int64),
                 ("heavy_nnz", C.c_int64), ("unit_split_nnz", C.c_int64), ("learned_target", C.c_float),
-                ("capacity_margin", C.c_float), ("row_map_share", C.c_float)]
+                ("capacity_margin", C.c_float), ("row_map_share", C.c_float), ("tiny_wg_waves", C.c_int32)]
 
 
 _lib_handle = None

@@ -2,14 +2,17 @@
 
 The reference hard-codes its bin switch and hash-table sizing
 (mtSpGEMM.h:311, :366-367).  Here the flop bound of the LDS hash tier, the
-flop bounds of its three table sizes, the output size that sends a column to
-the large units and the learned pass's unit target are measured per
-workload: every candidate of a small grid runs the workload's product on the
-device (after a warm-up), the fastest exact configuration wins, and the
-result is stored in a versioned table keyed by workload name and device.
+flop bounds of its table sizes (the LDS table sizes), the tiny tier's bound
+and workgroup shape, the output size that sends a column to the large units
+(the other unit workgroup shape) and the learned pass's unit target and
+exact-column bound are measured per workload: every candidate of a small
+grid runs the workload on the device (after a warm-up), the fastest
+configuration with an unchanged output size wins, and the result is stored
+in a versioned table keyed by workload name and device.  Workloads: the
+R-MAT products, the 256 batched Galerkin products and the MCL expansion.
 
-    python -m combblas_amd.autotune rmat_s20_ef16      # on a GPU machine
-    combblas_amd.autotune.apply("rmat_s20_ef16")       # install the stored choice
+    python -m combblas_amd.autotune rmat_s20_ef16 galerkin_256   # on a GPU machine
+    combblas_amd.autotune.apply("galerkin_256")                  # install the stored choice
 """
 from __future__ import annotations
 
@@ -24,12 +27,56 @@ import combblas_amd as cb
 TABLE_PATH = Path(__file__).resolve().parent / "tuning" / "table.json"
 TABLE_VERSION = 1
 
-# name -> (generator, semiring, mode)
-WORKLOADS = {
-    "rmat_s20_ef16": (lambda: cb.gen_rmat(20, 16), "plus_times", "two_phase"),
-    "rmat_s20_ef16_learned": (lambda: cb.gen_rmat(20, 16), "plus_times", "learned"),
-    "rmat_s16_ef16": (lambda: cb.gen_rmat(16, 16), "plus_times", "two_phase"),
-}
+# Galerkin and MCL workloads of the benchmark (the same inputs)
+GALERKIN_GRIDS = dict(count=256, lo=16, hi=96, seed=0xC5)
+MCL_PARAMS = (1e-4, 1000, 0, 0.9)  # hard threshold, select, recover number, recover fraction
+
+
+def _spgemm_workload(scale: int, ef: int, mode: str):
+    def setup():
+        dA = cb.gen_rmat(scale, ef).to_device()
+
+        def run():
+            C, st = cb.spgemm(dA, dA, "plus_times", mode)
+            C.release()
+            return st["nnz"]
+        return run, dA.release
+    return setup
+
+
+def _galerkin_workload():
+    import numpy as np
+    rng = np.random.default_rng(GALERKIN_GRIDS["seed"])
+    grids = [int(g) for g in rng.integers(GALERKIN_GRIDS["lo"], GALERKIN_GRIDS["hi"] + 1, GALERKIN_GRIDS["count"])]
+
+    def setup():
+        As = [cb.gen_stencil27(g).to_device() for g in grids]
+        Ts = [cb.gen_aggregation(g, True).to_device() for g in grids]
+
+        def run():
+            S, st = cb.galerkin_batched(As, Ts)
+            for X in S:
+                X.release()
+            return st["nnz"]
+
+        def done():
+            for X in As + Ts:
+                X.release()
+        return run, done
+    return setup
+
+
+def _mcl_workload(n: int, mode: str):
+    def setup():
+        dA = cb.gen_powerlaw_stochastic(n).to_device()
+
+        def run():
+            P, st = cb.mcl_expand(dA, dA, *MCL_PARAMS, mode=mode)
+            P.release()
+            return st["nnz_product"]
+        return run, dA.release
+    return setup
+
 
 GRID = {
     "hash_max_flop": [1024, 2048, 4096],
@@ -47,6 +94,37 @@ LEARNED_GRID = {
     "learned_target": [0.75, 0.9],
     "heavy_nnz": [8192, 16384, 32768, 65536],
 }
+# Galerkin columns hold 216 (A*T) and 64 (S*(A*T)) products: the tiny tier's
+# bound and workgroup shape decide where they run
+GALERKIN_GRID = {
+    "tiny_flop": [128, 256],
+    "hash_mid_flop": [256, 1024],
+    "tiny_wg_waves": [1, 2, 4],
+}
+MCL_GRID = {
+    "hash_max_flop": [1024, 2048, 4096],
+    "tiny_wg_waves": [2, 4],
+    "unit_split_nnz": [131072, 262144],
+}
+MCL_LEARNED_GRID = {
+    "hash_max_flop": [2048, 4096],
+    "learned_target": [0.75, 1.0],
+    "heavy_nnz": [8192, 65536],
+}
+
+# name -> (setup, description, mode, grid); setup() returns (run, release):
+# run() executes the workload once and returns its output size
+WORKLOADS = {
+    "rmat_s20_ef16": (_spgemm_workload(20, 16, "two_phase"), "R-MAT s20 ef16 A*A", "two_phase", GRID),
+    "rmat_s20_ef16_learned": (_spgemm_workload(20, 16, "learned"), "R-MAT s20 ef16 A*A", "learned", LEARNED_GRID),
+    "rmat_s16_ef16": (_spgemm_workload(16, 16, "two_phase"), "R-MAT s16 ef16 A*A", "two_phase", GRID),
+    "galerkin_256": (_galerkin_workload(), "256 batched Galerkin products S*(A*T), grid edges 16..96, FP64",
+                     "two_phase", GALERKIN_GRID),
+    "mcl_n1m": (_mcl_workload(1_000_000, "two_phase"), "MCL expansion A*A + prune, power-law n=10^6, FP32",
+                "two_phase", MCL_GRID),
+    "mcl_n1m_learned": (_mcl_workload(1_000_000, "learned"), "MCL expansion A*A + prune, power-law n=10^6, FP32",
+                        "learned", MCL_LEARNED_GRID),
+}
 
 
 def device_name() -> str:
@@ -57,47 +135,43 @@ def device_name() -> str:
         return "unknown"
 
 
-def _time_config(dA, semiring: str, mode: str, reps: int) -> tuple[float, int]:
-    C, st = cb.spgemm(dA, dA, semiring, mode)  # warm-up (also fills the allocation cache)
-    nnz = st["nnz"]
-    C.release()
+def _time_config(run, reps: int) -> tuple[float, int]:
+    size = run()  # warm-up (also fills the allocation cache)
     cb.synchronize()
     t = time.perf_counter()
     for _ in range(reps):
-        C, st = cb.spgemm(dA, dA, semiring, mode)
-        C.release()
+        run()
     cb.synchronize()
-    return (time.perf_counter() - t) * 1e3 / reps, nnz
+    return (time.perf_counter() - t) * 1e3 / reps, size
 
 
 def tune(name: str, reps: int = 2, log=print) -> dict:
-    gen, semiring, mode = WORKLOADS[name]
-    A = gen()
-    dA = A.to_device()
+    setup, desc, mode, grid = WORKLOADS[name]
     if mode == "learned":
         from combblas_amd import predictor
         cb.set_predictor(predictor.load())
+    run, release = setup()
     base = cb.get_config()
-    grid = LEARNED_GRID if mode == "learned" else GRID
     keys = list(grid)
     results = []
-    ref_nnz = None
+    ref_size = None
     for combo in itertools.product(*(grid[k] for k in keys)):
-        cfg = dict(zip(keys, combo))
+        cfg = dict(base, **dict(zip(keys, combo)))
         if cfg["hash_mid_flop"] > cfg["hash_max_flop"] or cfg["tiny_flop"] > cfg["hash_mid_flop"]:
             continue
-        cb.set_config(**dict(base, **cfg))
-        ms, nnz = _time_config(dA, semiring, mode, reps)
-        ref_nnz = nnz if ref_nnz is None else ref_nnz
-        results.append({"config": cfg, "ms": round(ms, 3), "same_nnz": nnz == ref_nnz})
-        log(f"{name}: {cfg} -> {ms:.2f} ms")
+        cb.set_config(**cfg)
+        ms, size = _time_config(run, reps)
+        ref_size = size if ref_size is None else ref_size
+        point = dict(zip(keys, combo))
+        results.append({"config": point, "ms": round(ms, 3), "same_nnz": size == ref_size})
+        log(f"{name}: {point} -> {ms:.2f} ms")
     cb.set_config(**base)
+    release()
     best = min((r for r in results if r["same_nnz"]), key=lambda r: r["ms"])
-    entry = {"workload": name, "device": device_name(), "semiring": semiring, "mode": mode,
+    entry = {"workload": name, "description": desc, "device": device_name(), "mode": mode,
              "best": dict(base, **best["config"]), "best_ms": best["ms"],
              "default_ms": next((r["ms"] for r in results if all(base[k] == v for k, v in r["config"].items())), None),
              "sweep": results, "date": time.strftime("%Y-%m-%d")}
-    dA.release()
     return entry
 
 

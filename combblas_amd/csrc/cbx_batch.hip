@@ -476,7 +476,7 @@ extern "C" cbx_status cbx_spgemm_batched(int64_t count, const cbx_dcsc* A, const
                 CBX_ERR_INVALID, "cbx_spgemm_batched: unknown mode %d", (int)mode);
   if ((st = init_device_once())) return st;
   SpgemmConfig cfg = current_config();
-  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor, CBX_ERR_INVALID,
+  CBX_RETURN_IF(mode == CBX_SPGEMM_LEARNED && !cfg.predictor && cfg.sampling_rounds == 0, CBX_ERR_INVALID,
                 "cbx_spgemm_batched: learned mode needs a predictor (cbx_spgemm_set_predictor)");
   cbx_spgemm_stats local;
   cbx_spgemm_stats* sp = stats ? stats : &local;

@@ -353,7 +353,7 @@ extern "C" cbx_status cbx_spgemm_set_sampling_predictor(int32_t nrounds) {
 
 // ------------------------------------------------------------- tunables
 static std::mutex g_config_mu;
-static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f};
+static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f, 4};
 
 SpgemmConfig cbx::current_config() {
   SpgemmConfig c;
@@ -367,6 +367,7 @@ SpgemmConfig cbx::current_config() {
     c.learned_target = g_config.learned_target;
     c.capacity_margin = g_config.capacity_margin;
     c.row_map_share = g_config.row_map_share;
+    c.tiny_wg_waves = g_config.tiny_wg_waves;
   }
   c.predictor = predictor_params();
   c.sampling_rounds = sampling_rounds_setting();
@@ -397,6 +398,8 @@ extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
                 "cbx_spgemm_set_config: capacity_margin must be in [0, 4]");
   CBX_RETURN_IF(!(cfg->row_map_share >= 0.0f && cfg->row_map_share <= 1.0f), CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: row_map_share must be in [0, 1]");
+  CBX_RETURN_IF(cfg->tiny_wg_waves != 1 && cfg->tiny_wg_waves != 2 && cfg->tiny_wg_waves != 4, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: tiny_wg_waves must be 1, 2 or 4");
   std::lock_guard<std::mutex> g(g_config_mu);
   g_config = *cfg;
   return CBX_OK;

@@ -40,6 +40,8 @@ struct SpgemmConfig {
   float capacity_margin = 0.15f;
   // Two-phase mode: share of the free memory the large units' row maps may take (0: none).
   float row_map_share = 0.25f;
+  // Workgroup shape of the tiny tier: columns (one wave each) per workgroup.
+  int tiny_wg_waves = 4;
 };
 
 // The process-wide tunables (cbx_spgemm_set_config) with the installed

@@ -1903,8 +1903,10 @@ constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outp
 constexpr int kLearnNT = 512, kLearnK = 4096, kLearnRB = 32, kLearnCR = 2;
 constexpr int kLearnFlopCap = 131072;  // expected products per learned unit
 constexpr int64_t kLearnChunk = 1 << 21;  // workgroups per learned launch
-inline unsigned tiny_grid(int64_t ncols) {  // 4 columns per workgroup, grid-stride beyond 64K workgroups
-  return (unsigned)std::min<int64_t>((ncols + 3) / 4, 65536);
+// Tiny tier launch: `waves` columns (one wave each) per workgroup, grid-stride
+// beyond 256K columns in flight.
+inline unsigned tiny_grid(int64_t ncols, int waves) {
+  return (unsigned)std::min<int64_t>((ncols + waves - 1) / waves, (int64_t)262144 / waves);
 }                            // learned units: expected products
 
 }  // namespace
@@ -2282,7 +2284,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
 
   // --- learned single pass (unit-tier columns), with the exact hash-tier
   //     symbolic for the small columns
-  bool learned = mode == CBX_SPGEMM_LEARNED && use_units && n_dense > 0 && cfg.predictor != nullptr;
+  bool learned = mode == CBX_SPGEMM_LEARNED && use_units && n_dense > 0 &&
+                 (cfg.predictor != nullptr || cfg.sampling_rounds > 0);
   bool learned_done = false;
   LearnedInfo linfo;
   int64_t nnzC = 0, nnz_main = 0;
@@ -2290,7 +2293,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (learned) {
     if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
     if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
-    if (n_h256) k_tiny<K, T, true><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr, nullptr, nullptr, err);
+    if (n_h256) k_tiny<K, T, true><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr, nullptr, nullptr, err);
     tm.mark(2, s);
     tm.mark(3, s);
     CBX_TRY((learned_numeric<K, T>(A, B, cfg, s, rowsA, bpos, flop, P, S, nnzcol, cpC, tmp, tmp_bytes, C, err, linfo)));
@@ -2367,11 +2370,11 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
       tiny_staged = true;
     (void)hipGetLastError();
     if (tiny_staged)
-      k_tiny<K, T, false><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
+      k_tiny<K, T, false><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
                                                            valA, nnzcol, nullptr, stage_ir, stage_val, err,
                                                            stage_off);
     else
-      k_tiny<K, T, true><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
+      k_tiny<K, T, true><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
                                                           valA, nnzcol, nullptr, nullptr, nullptr, err);
   }
   CBX_HIP(hipGetLastError());
@@ -2450,7 +2453,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     k_tiny_place<T><<<wave_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, stage_off, nnzcol, cpC, stage_ir,
                                                        stage_val, C->ir, (T*)C->val);
   else if (n_h256)
-    k_tiny<K, T, false><<<tiny_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
+    k_tiny<K, T, false><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
                                                          valA, nnzcol, cpC, C->ir, (T*)C->val, err);
 #undef CBX_LAUNCH_HASH
   CBX_HIP(hipGetLastError());

@@ -320,7 +320,7 @@ __global__ void k_bfs_rows32(const int64_t* __restrict__ ir, int64_t nnz, int32_
 
 __global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __restrict__ parents,
                             uint32_t* __restrict__ visited, int64_t nw, int32_t* __restrict__ f0,
-                            int32_t* __restrict__ counts, int64_t root) {
+                            int32_t* __restrict__ counts, unsigned long long* __restrict__ stat, int64_t root) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) {
     cand[i] = -1;
@@ -331,13 +331,20 @@ __global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __
     f0[0] = (int32_t)root;
     counts[0] = 1;
     counts[1] = 0;
+    stat[0] = 0ull;  // levels
+    stat[1] = 0ull;  // vertices reached
   }
 }
 
-__global__ void k_bfs_zero(int32_t* __restrict__ c, unsigned long long* __restrict__ edges) {
+// First kernel of a level: clears the next frontier's count and degree sum,
+// and counts the level if its frontier is not empty (a level graph launched
+// after the search ended changes nothing).
+__global__ void k_bfs_level_begin(const int32_t* __restrict__ cin, int32_t* __restrict__ c,
+                                  unsigned long long* __restrict__ edges, unsigned long long* __restrict__ stat) {
   if (threadIdx.x == 0) {
     *c = 0;
     *edges = 0ull;
+    if (*cin > 0) ++stat[0];
   }
 }
 
@@ -373,8 +380,16 @@ __global__ __launch_bounds__(NT) void k_bfs_expand(const int64_t* __restrict__ b
       for (int u = 0; u < U; ++u) {
         bool push = false;
         if (r[u] >= 0 && !((visited[r[u] >> 5] >> (r[u] & 31)) & 1u)) {
-          const long long old = atomicMax(&cand[r[u]], (long long)ks[u]);
-          push = old == -1;
+          // Offers only grow cand, so a value read without the atomic (possibly
+          // stale, never above the true one) that already holds an offer at
+          // least this large makes the atomic a no-op: skip it.  Hubs receive
+          // thousands of offers per level and only the record-breaking ones
+          // reach the serialising atomic.
+          const long long seen = __hip_atomic_load(&cand[r[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (seen < (long long)ks[u]) {
+            const long long old = atomicMax(&cand[r[u]], (long long)ks[u]);
+            push = old == -1;
+          }
         }
         // one atomic per wave for the appends of this step
         const uint64_t mask = __ballot(push);
@@ -404,9 +419,11 @@ __global__ __launch_bounds__(kBfsFinNT) void k_bfs_finalize(const int32_t* __res
                                                             const int64_t* __restrict__ end,
                                                             int64_t* __restrict__ parents,
                                                             uint32_t* __restrict__ visited,
-                                                            unsigned long long* __restrict__ edges) {
+                                                            unsigned long long* __restrict__ edges,
+                                                            unsigned long long* __restrict__ reached) {
   __shared__ long long ws[kBfsFinNT / kWave];
   const int nf = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && nf) atomicAdd(reached, (unsigned long long)nf);
   long long deg = 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nf; t += gridDim.x * blockDim.x) {
     const int32_t i = f[t];
@@ -524,7 +541,10 @@ struct cbx_bfs_plan_s {
   uint32_t* visited = nullptr;
   unsigned long long* edges = nullptr;  // degree sums of the two frontier buffers
   uint32_t* fbits = nullptr;             // frontier bitmap of a bottom-up level
-  int32_t* host_count = nullptr;  // pinned, 32 B: next frontier size; [2..3] its degree sum; [4..7] root's range
+  unsigned long long* stat = nullptr;  // device: levels, vertices reached
+  // pinned, 64 B: next frontier size; [2..3] its degree sum; [4..7] root's
+  // range; [8..11] the device statistics
+  int32_t* host_count = nullptr;
   int direction = CBX_BFS_DIRECTION_OPTIMIZING;
   // Pattern of A equal to that of its transpose.  Bottom-up levels scan column
   // v of A for frontier rows, which is row v only for a symmetric pattern, so
@@ -533,6 +553,10 @@ struct cbx_bfs_plan_s {
   // [direction (0 top-down, 1 bottom-up)][frontier parity]
   hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  // top-down only: kBfsTdLevels consecutive levels in one graph, launched
+  // until the frontier is empty (levels after the last one do nothing)
+  hipGraph_t td_graph = nullptr;
+  hipGraphExec_t td_exec = nullptr;
 };
 
 static void bfs_plan_free(cbx_bfs_plan_s* p) {
@@ -542,10 +566,12 @@ static void bfs_plan_free(cbx_bfs_plan_s* p) {
       if (p->exec[d][i]) (void)hipGraphExecDestroy(p->exec[d][i]);
       if (p->graph[d][i]) (void)hipGraphDestroy(p->graph[d][i]);
     }
+  if (p->td_exec) (void)hipGraphExecDestroy(p->td_exec);
+  if (p->td_graph) (void)hipGraphDestroy(p->td_graph);
   for (int i = 0; i < 2; ++i)
     if (p->f[i]) (void)hipFree(p->f[i]);
   for (void* q : {(void*)p->beg, (void*)p->end, (void*)p->parents, (void*)p->rows, (void*)p->counts, (void*)p->cand,
-                  (void*)p->visited, (void*)p->edges, (void*)p->fbits})
+                  (void*)p->visited, (void*)p->edges, (void*)p->fbits, (void*)p->stat})
     if (q) (void)hipFree(q);
   if (p->host_count) (void)hipHostFree(p->host_count);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -555,6 +581,7 @@ static void bfs_plan_free(cbx_bfs_plan_s* p) {
 constexpr int kBfsNT = 256;
 constexpr unsigned kBfsGrid = 2048;
 constexpr int64_t kBfsAlpha = 256;
+constexpr int kBfsTdLevels = 8;  // even: the frontier parity is 0 again after one top-down graph
 
 extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** out) {
   CBX_RETURN_IF(!A || !out, CBX_ERR_INVALID, "cbx_bfs_plan_create: null argument");
@@ -610,7 +637,8 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
       hipMalloc(&p->f[0], 4 * std::max<int64_t>(n, 1)) || hipMalloc(&p->f[1], 4 * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->counts, 16) || hipMalloc(&p->cand, 8 * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->visited, 4 * p->nw) || hipMalloc(&p->edges, 2 * sizeof(unsigned long long)) ||
-      hipMalloc(&p->fbits, 4 * p->nw) || hipHostMalloc(&p->host_count, 32, hipHostMallocDefault))
+      hipMalloc(&p->fbits, 4 * p->nw) || hipMalloc(&p->stat, 2 * sizeof(unsigned long long)) ||
+      hipHostMalloc(&p->host_count, 64, hipHostMallocDefault))
     return fail("allocation");
   hipStream_t s = p->stream;
   if (hipMemsetAsync(p->beg, 0, 8 * n, s) || hipMemsetAsync(p->end, 0, 8 * n, s)) return fail("memset");
@@ -627,7 +655,7 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
       int32_t* cout = p->counts + (1 - par);
       unsigned long long* eout = p->edges + (1 - par);
       if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail("capture begin");
-      k_bfs_zero<<<1, 64, 0, s>>>(cout, eout);
+      k_bfs_level_begin<<<1, 64, 0, s>>>(cin, cout, eout, p->stat);
       if (dir == 0) {
         k_bfs_expand<kBfsNT, 4><<<kBfsGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->f[par], cin, p->visited,
                                                            p->cand, p->f[1 - par], cout);
@@ -638,13 +666,31 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
                                                 p->f[1 - par], cout);
       }
       k_bfs_finalize<<<256, kBfsFinNT, 0, s>>>(p->f[1 - par], cout, p->cand, p->beg, p->end, p->parents, p->visited,
-                                               eout);
+                                               eout, p->stat + 1);
       (void)hipMemcpyAsync(p->host_count, cout, sizeof(int32_t), hipMemcpyDeviceToHost, s);
       (void)hipMemcpyAsync(p->host_count + 2, eout, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
       if (hipStreamEndCapture(s, &p->graph[dir][par]) != hipSuccess) return fail("capture end");
       if (hipGraphInstantiate(&p->exec[dir][par], p->graph[dir][par], nullptr, nullptr, 0) != hipSuccess)
         return fail("graph instantiate");
     }
+  // kBfsTdLevels top-down levels in one graph, no host round trip between
+  // them; the host reads the frontier size once per graph
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail("capture begin");
+  for (int l = 0; l < kBfsTdLevels; ++l) {
+    const int par = l & 1;
+    int32_t* cin = p->counts + par;
+    int32_t* cout = p->counts + (1 - par);
+    unsigned long long* eout = p->edges + (1 - par);
+    k_bfs_level_begin<<<1, 64, 0, s>>>(cin, cout, eout, p->stat);
+    k_bfs_expand<kBfsNT, 4><<<kBfsGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->f[par], cin, p->visited, p->cand,
+                                                       p->f[1 - par], cout);
+    k_bfs_finalize<<<256, kBfsFinNT, 0, s>>>(p->f[1 - par], cout, p->cand, p->beg, p->end, p->parents, p->visited,
+                                             eout, p->stat + 1);
+  }
+  (void)hipMemcpyAsync(p->host_count, p->counts, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+  if (hipStreamEndCapture(s, &p->td_graph) != hipSuccess) return fail("capture end");
+  if (hipGraphInstantiate(&p->td_exec, p->td_graph, nullptr, nullptr, 0) != hipSuccess)
+    return fail("graph instantiate");
   *out = p;
   return CBX_OK;
 }
@@ -658,31 +704,43 @@ extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* pare
   CBX_HIP(hipEventCreate(&e1));
   CBX_HIP(hipEventRecord(e0, s));
   k_bfs_reset<<<nb(std::max(p->n, p->nw), 256), 256, 0, s>>>(p->n, p->cand, p->parents, p->visited, p->nw, p->f[0],
-                                                             p->counts, root);
+                                                             p->counts, p->stat, root);
   // direction per level: bottom-up while the frontier's edges exceed 1/256 of
   // the edges still unexplored.  (Beamer's rule compares the two with a factor
   // 14 for equal per-edge costs; here a top-down edge costs a device atomic on
   // the target, some 20-40 times a bottom-up edge, which is a streamed read.)
-  int levels = 0, bottom_up_levels = 0;
-  int64_t reached = 0, mu = p->nnz;
-  int64_t* hdeg = reinterpret_cast<int64_t*>(p->host_count + 4);  // pinned: beg[root], end[root]
-  CBX_HIP(hipMemcpyAsync(hdeg, p->beg + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  CBX_HIP(hipMemcpyAsync(hdeg + 1, p->end + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  CBX_HIP(hipStreamSynchronize(s));
-  int64_t fe = hdeg[1] - hdeg[0];  // edges of the current frontier
-  bool bottom_up = p->direction == CBX_BFS_BOTTOM_UP;
-  for (int par = 0;; par ^= 1) {
-    if (p->direction == CBX_BFS_DIRECTION_OPTIMIZING) bottom_up = p->symmetric && fe > mu / kBfsAlpha;
-    CBX_HIP(hipGraphLaunch(p->exec[bottom_up ? 1 : 0][par], s));
+  int bottom_up_levels = 0;
+  if (p->direction == CBX_BFS_TOP_DOWN || !p->symmetric) {
+    // every level top-down: graphs of kBfsTdLevels levels until the frontier
+    // is empty (a BFS has at most n levels)
+    for (int64_t launched = 0;; launched += kBfsTdLevels) {
+      CBX_HIP(hipGraphLaunch(p->td_exec, s));
+      CBX_HIP(hipStreamSynchronize(s));
+      if (*(volatile int32_t*)p->host_count == 0 || launched > p->n + 1) break;
+    }
+  } else {
+    int levels = 0;
+    int64_t mu = p->nnz;
+    int64_t* hdeg = reinterpret_cast<int64_t*>(p->host_count + 4);  // pinned: beg[root], end[root]
+    CBX_HIP(hipMemcpyAsync(hdeg, p->beg + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CBX_HIP(hipMemcpyAsync(hdeg + 1, p->end + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     CBX_HIP(hipStreamSynchronize(s));
-    ++levels;
-    bottom_up_levels += bottom_up;
-    const int32_t next = *(volatile int32_t*)p->host_count;
-    fe = (int64_t) * (volatile unsigned long long*)(p->host_count + 2);
-    mu -= fe;
-    reached += next;
-    if (next == 0 || levels > p->n + 1) break;
+    int64_t fe = hdeg[1] - hdeg[0];  // edges of the current frontier
+    bool bottom_up = p->direction == CBX_BFS_BOTTOM_UP;
+    for (int par = 0;; par ^= 1) {
+      if (p->direction == CBX_BFS_DIRECTION_OPTIMIZING) bottom_up = fe > mu / kBfsAlpha;
+      CBX_HIP(hipGraphLaunch(p->exec[bottom_up ? 1 : 0][par], s));
+      CBX_HIP(hipStreamSynchronize(s));
+      ++levels;
+      bottom_up_levels += bottom_up;
+      const int32_t next = *(volatile int32_t*)p->host_count;
+      fe = (int64_t) * (volatile unsigned long long*)(p->host_count + 2);
+      mu -= fe;
+      if (next == 0 || levels > p->n + 1) break;
+    }
   }
+  unsigned long long* hstat = reinterpret_cast<unsigned long long*>(p->host_count + 8);
+  CBX_HIP(hipMemcpyAsync(hstat, p->stat, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   CBX_HIP(hipMemcpyAsync(parents_dev, p->parents, sizeof(int64_t) * p->n, hipMemcpyDeviceToDevice, s));
   CBX_HIP(hipEventRecord(e1, s));
   CBX_HIP(hipStreamSynchronize(s));
@@ -691,9 +749,9 @@ extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* pare
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (stats) {
-    stats->levels = levels;
+    stats->levels = (int32_t)hstat[0];
     stats->bottom_up_levels = bottom_up_levels;
-    stats->reached = reached;
+    stats->reached = (int64_t)hstat[1];
     stats->ms = ms;
   }
   return CBX_OK;

@@ -177,7 +177,7 @@ enum {
  * sizes the reference hard-codes: the heap/hash switch cr < 2 at
  * mtSpGEMM.h:311 and minHashTableSize / hashScale at :366-367).  Picked per
  * workload by combblas_amd/autotune.py; defaults {4096, 1024, 256, 65536,
- * 131072, 0.75, 0.15, 0.25}. */
+ * 131072, 0.75, 0.15, 0.25, 4}. */
 typedef struct cbx_spgemm_config {
   int64_t hash_max_flop;   /* columns with flop_j <= this use the LDS hash tier (<= 4096) */
   int64_t hash_mid_flop;   /* ... with flop_j > this use the 8192-slot table, else the 2048-slot one (<= 1024) */
@@ -193,6 +193,7 @@ typedef struct cbx_spgemm_config {
   float row_map_share;     /* two-phase mode: at most this share of the free device memory holds the symbolic
                               pass's row maps of the columns that can form 16384-output units, which those
                               units then read instead of rebuilding them (0 disables; in [0, 1]) */
+  int32_t tiny_wg_waves;   /* workgroup shape of the tiny tier: columns (one wave each) per workgroup, 1, 2 or 4 */
 } cbx_spgemm_config;
 
 cbx_status cbx_spgemm_get_config(cbx_spgemm_config* cfg);
