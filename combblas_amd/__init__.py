@@ -157,6 +157,8 @@ def lib() -> C.CDLL:
         L.cbx_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int32]
         L.cbx_spmspv.argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                  C.c_void_p, P(C.c_int32), C.c_void_p]
+        L.cbx_spmspv_noalloc.argtypes = L.cbx_spmspv.argtypes
+        L.cbx_spmspv_noalloc.restype = C.c_int32
         L.cbx_bfs_plan_create.argtypes = [P(cbx_dcsc), P(C.c_void_p)]
         L.cbx_bfs_run.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, P(cbx_bfs_stats)]
         L.cbx_bfs_plan_destroy.argtypes = [C.c_void_p]
@@ -624,16 +626,20 @@ class DeviceArray:
 _XDT = {np.dtype(np.int64): "int64", np.dtype(np.float64): "float64", np.dtype(np.float32): "float32"}
 
 
-def spmspv(A: DeviceDcsc, xind: np.ndarray, xval: np.ndarray, semiring: str = "select_max") -> tuple[np.ndarray, np.ndarray]:
-    """y = A (x) x for a sparse x (SpMXSpV, SpImpl.cpp:55-213); returns (yind, yval) on the host."""
+def spmspv(A: DeviceDcsc, xind: np.ndarray, xval: np.ndarray, semiring: str = "select_max",
+           noalloc: bool = False) -> tuple[np.ndarray, np.ndarray]:
+    """y = A (x) x for a sparse x (SpMXSpV, SpImpl.cpp:55-213); returns (yind, yval) on the host.
+    noalloc=True calls cbx_spmspv_noalloc (scratch in the library's static
+    per-GPU array instead of allocations; same result)."""
     xind = np.ascontiguousarray(xind, np.int32)
     xval = np.ascontiguousarray(xval)
     dt = _XDT[xval.dtype]
     dx, dv = DeviceArray.from_host(xind), DeviceArray.from_host(xval)
     dyi, dyv = DeviceArray(A.m, np.int32), DeviceArray(A.m, xval.dtype)
     ny = C.c_int32(0)
-    _check(lib().cbx_spmspv(C.byref(A.s), dx.ptr, dv.ptr, xind.size, SEMIRINGS[semiring], DTYPES[dt], dyi.ptr,
-                            dyv.ptr, C.byref(ny), None))
+    fn = lib().cbx_spmspv_noalloc if noalloc else lib().cbx_spmspv
+    _check(fn(C.byref(A.s), dx.ptr, dv.ptr, xind.size, SEMIRINGS[semiring], DTYPES[dt], dyi.ptr, dyv.ptr,
+              C.byref(ny), None))
     return dyi.to_host(ny.value), dyv.to_host(ny.value)
 
 

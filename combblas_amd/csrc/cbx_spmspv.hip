@@ -288,6 +288,57 @@ cbx_status spmspv_typed(const cbx_dcsc* A, bool boolA, const int32_t* xind, cons
   return st;
 }
 
+// Carves the scratch of one SpMSpV (the arrays of SvWork plus the structure
+// check's three counters) out of one caller-given region; with base == null
+// it only adds up the bytes.  Every piece starts on a 256-byte boundary.
+template <class T>
+size_t sv_carve(int64_t m, int64_t n, char* base, SvWork* w, unsigned long long** check_cnt) {
+  const int64_t nw = sv_words(m);
+  size_t used = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* p = base ? base + used : nullptr;
+    used += (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    return p;
+  };
+  SvWork t;
+  t.tb = sv_scan_bytes<T>(m);
+  t.colmap = (int32_t*)take(sizeof(int32_t) * std::max<int64_t>(n, 1));
+  t.cnt = (int32_t*)take(sizeof(int32_t) * (nw + 1));
+  t.off = (int32_t*)take(sizeof(int32_t) * (nw + 1));
+  t.acc = take(sizeof(T) * std::max<int64_t>(m, 1));
+  t.bits = (uint32_t*)take(sizeof(uint32_t) * nw);
+  t.bad = (int*)take(sizeof(int));
+  t.tmp = take(t.tb);
+  unsigned long long* cnt = (unsigned long long*)take(3 * sizeof(unsigned long long));
+  if (w) *w = t;
+  if (check_cnt) *check_cnt = cnt;
+  return used;
+}
+
+// cbx_spmspv with its scratch in `region` (bytes of device memory): the
+// structure check and the product, no allocation.
+template <int K, class T>
+cbx_status spmspv_in_region(const cbx_dcsc* A, bool boolA, const int32_t* xind, const T* xval, int32_t nx,
+                            int32_t* yind, T* yval, int32_t* ny, char* region, size_t bytes, hipStream_t s) {
+  *ny = 0;
+  const size_t need = sv_carve<T>(A->m, A->n, nullptr, nullptr, nullptr);
+  CBX_RETURN_IF(need > bytes, CBX_ERR_LIMIT,
+                "cbx_spmspv_noalloc: a %lld x %lld matrix needs %zu bytes of scratch, more than the %zu available",
+                (long long)A->m, (long long)A->n, need, bytes);
+  SvWork w;
+  unsigned long long* cnt = nullptr;
+  sv_carve<T>(A->m, A->n, region, &w, &cnt);
+  if (A->nzc == 0) {
+    CBX_RETURN_IF(A->nnz != 0, CBX_ERR_INVALID, "cbx_dcsc_check: nnz without columns");
+  } else {
+    dcsc_check_launch(A, cnt, s);  // the kernels index by A's rows and columns
+    const cbx_status st = dcsc_check_result(cnt, s);
+    if (st != CBX_OK) return st;
+  }
+  if (nx == 0 || A->nnz == 0) return CBX_OK;
+  return spmspv_run<K, T>(A, boolA, xind, xval, nx, yind, yval, ny, w, s);
+}
+
 }  // namespace
 
 // ====================================================================== BFS
@@ -1020,6 +1071,78 @@ extern "C" cbx_status cbx_spmspv(const cbx_dcsc* A, const int32_t* xind, const v
   }
   return set_error(CBX_ERR_UNSUPPORTED, "cbx_spmspv: semiring %d with vector dtype %d is not provided", (int)semiring,
                    (int)xdtype);
+}
+
+// ------------------------------------------------ allocation-free SpMSpV
+// The scratch of cbx_spmspv_noalloc: one static array per GPU, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, not device
+// memory, so the host never passes it to a kernel or a copy: each device's
+// address of the array is looked up once (hipGetSymbolAddress, with that
+// device current) and kept in g_scratch_dev.
+constexpr size_t kSpmspvScratchBytes = (size_t)CBX_SPMSPV_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kSpmspvScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_dev[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call made with that device current).
+cbx_status scratch_region(char** out) {
+  int dev = 0;
+  CBX_HIP(hipGetDevice(&dev));
+  CBX_RETURN_IF(dev < 0 || dev >= kMaxScratchDevices, CBX_ERR_LIMIT, "cbx_spmspv_noalloc: device %d out of range",
+                dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(CBX_ERR_HIP, "cbx_spmspv_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                       hipGetErrorString(e));
+    g_scratch_dev[dev] = (char*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return CBX_OK;
+}
+}  // namespace
+
+extern "C" cbx_status cbx_spmspv_noalloc(const cbx_dcsc* A, const int32_t* xind, const void* xval, int32_t nx,
+                                         cbx_semiring semiring, cbx_dtype xdtype, int32_t* yind, void* yval,
+                                         int32_t* ny, void* stream) {
+  CBX_RETURN_IF(!A || !ny || (nx > 0 && (!xind || !xval)) || !yind || !yval, CBX_ERR_INVALID,
+                "cbx_spmspv_noalloc: null argument");
+  CBX_RETURN_IF(A->location != CBX_DEVICE, CBX_ERR_INVALID, "cbx_spmspv_noalloc: expected a device matrix");
+  CBX_RETURN_IF(A->m < 0 || A->n < 0 || A->nnz < 0 || A->nzc < 0, CBX_ERR_INVALID, "cbx_spmspv_noalloc: negative size");
+  CBX_RETURN_IF(A->m >= (1ll << 31) || A->n >= (1ll << 31), CBX_ERR_LIMIT,
+                "cbx_spmspv_noalloc: dimensions must be below 2^31");
+  CBX_RETURN_IF(nx < 0 || nx > A->n, CBX_ERR_INVALID, "cbx_spmspv_noalloc: bad vector length");
+  const bool boolA = A->dtype == CBX_BOOL;
+  CBX_RETURN_IF(!boolA && A->dtype != xdtype, CBX_ERR_UNSUPPORTED,
+                "cbx_spmspv_noalloc: A must be boolean or share the vector's dtype");
+  char* region = nullptr;
+  const cbx_status rs = scratch_region(&region);
+  if (rs != CBX_OK) return rs;
+  hipStream_t s = (hipStream_t)stream;
+  switch (semiring * 8 + xdtype) {
+#define CBX_SV(SRK, SRE, T, DT)                                                                                      \
+  case SRE * 8 + DT:                                                                                                 \
+    return spmspv_in_region<SRK, T>(A, boolA, xind, (const T*)xval, nx, yind, (T*)yval, ny, region,                  \
+                                    kSpmspvScratchBytes, s);
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, int64_t, CBX_I64)
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, double, CBX_F64)
+    CBX_SV(kPlusTimes, CBX_SR_PLUS_TIMES, float, CBX_F32)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, int64_t, CBX_I64)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, double, CBX_F64)
+    CBX_SV(kMinPlus, CBX_SR_MIN_PLUS, float, CBX_F32)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, int64_t, CBX_I64)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, double, CBX_F64)
+    CBX_SV(kSelectMax, CBX_SR_SELECT_MAX, float, CBX_F32)
+#undef CBX_SV
+  }
+  return set_error(CBX_ERR_UNSUPPORTED, "cbx_spmspv_noalloc: semiring %d with vector dtype %d is not provided",
+                   (int)semiring, (int)xdtype);
 }
 
 extern "C" cbx_status cbx_spmv_dense(const cbx_dcsc* A, const void* x, void* y, cbx_semiring semiring,

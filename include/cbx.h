@@ -313,6 +313,23 @@ cbx_status cbx_mlp_train(float* params, const float* X, const float* y, int64_t 
 cbx_status cbx_spmspv(const cbx_dcsc* A, const int32_t* xind, const void* xval, int32_t nx, cbx_semiring semiring,
                       cbx_dtype xdtype, int32_t* yind, void* yval, int32_t* ny, void* stream);
 
+/* cbx_spmspv without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as cbx_spmspv; the structure check of A and
+ * every scratch array (column map, dense accumulator, bitmap, scan storage)
+ * live in one static device array of CBX_SPMSPV_NOALLOC_SCRATCH_BYTES per GPU,
+ * kept until the process ends.  It needs about 4*n + sizeof(value)*m + m/4
+ * bytes plus a few KiB (12.4 MiB for m = n = 2^20 with 8-byte values); a
+ * larger A is refused with CBX_ERR_LIMIT before any device work.  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU (other
+ * host threads, other streams) race and must be serialised by the caller.
+ * Synchronous with respect to the host, like cbx_spmspv. */
+#define CBX_SPMSPV_NOALLOC_SCRATCH_BYTES (16u << 20)
+cbx_status cbx_spmspv_noalloc(const cbx_dcsc* A, const int32_t* xind, const void* xval, int32_t nx,
+                              cbx_semiring semiring, cbx_dtype xdtype, int32_t* yind, void* yval, int32_t* ny,
+                              void* stream);
+
 /* -------------------------------------------------------------------- BFS */
 
 /* y = A*x with a dense x: dcsc_gespmv<SR>(A, x, y) (Friends.h:63-77), y[i] =
