@@ -186,7 +186,8 @@ def bench_bfs(cb, scale: int, ef: int, nroots: int = 64) -> dict:
     plan = cb.BfsPlan(G.to_device())
     roots = cb.bfs_roots(G, nroots)
     out = {"workload": f"BFS (TopDownBFS parents), symmetrised R-MAT s{scale} ef{ef} without self-loops, "
-                       f"{len(roots)} roots (seed 4242), one captured graph per level and direction"}
+                       f"{len(roots)} roots (seed 4242); direction-optimising levels one captured graph each, "
+                       f"top-down-only levels eight to a captured graph, hubs expanded in 1024-edge pieces"}
     total_edges = 0
     for direction in ("optimizing", "top_down"):
         plan.set_direction(direction)

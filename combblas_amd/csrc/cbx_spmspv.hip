@@ -369,9 +369,25 @@ __global__ void k_bfs_rows32(const int64_t* __restrict__ ir, int64_t nnz, int32_
   if (i < nnz) rows[i] = (int32_t)ir[i];
 }
 
+// Frontier vertices with more than kBfsHub neighbours ("hubs") are expanded
+// in pieces of kBfsHub edges by k_bfs_expand_hubs, one workgroup per piece,
+// instead of by the workgroup that holds them in the frontier: R-MAT hubs have
+// 10^5 neighbours and would otherwise keep one workgroup busy long after the
+// others finished.  A piece is (vertex << 32 | piece index).
+constexpr int64_t kBfsHub = 1024;
+
+__device__ __forceinline__ void bfs_add_pieces(int32_t v, int64_t deg, int32_t* __restrict__ pcount,
+                                               long long* __restrict__ pieces) {
+  const int np = (int)((deg + kBfsHub - 1) / kBfsHub);
+  const int b = atomicAdd(pcount, np);
+  for (int k = 0; k < np; ++k) pieces[b + k] = ((long long)v << 32) | k;
+}
+
 __global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __restrict__ parents,
                             uint32_t* __restrict__ visited, int64_t nw, int32_t* __restrict__ f0,
-                            int32_t* __restrict__ counts, unsigned long long* __restrict__ stat, int64_t root) {
+                            int32_t* __restrict__ counts, int32_t* __restrict__ pcount,
+                            long long* __restrict__ pieces0, const int64_t* __restrict__ beg,
+                            const int64_t* __restrict__ end, unsigned long long* __restrict__ stat, int64_t root) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) {
     cand[i] = -1;
@@ -382,8 +398,12 @@ __global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __
     f0[0] = (int32_t)root;
     counts[0] = 1;
     counts[1] = 0;
+    pcount[0] = 0;
+    pcount[1] = 0;
     stat[0] = 0ull;  // levels
     stat[1] = 0ull;  // vertices reached
+    const int64_t deg = end[root] - beg[root];
+    if (deg > kBfsHub) bfs_add_pieces((int32_t)root, deg, pcount, pieces0);
   }
 }
 
@@ -391,11 +411,42 @@ __global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __
 // and counts the level if its frontier is not empty (a level graph launched
 // after the search ended changes nothing).
 __global__ void k_bfs_level_begin(const int32_t* __restrict__ cin, int32_t* __restrict__ c,
-                                  unsigned long long* __restrict__ edges, unsigned long long* __restrict__ stat) {
+                                  unsigned long long* __restrict__ edges, int32_t* __restrict__ pc,
+                                  unsigned long long* __restrict__ stat) {
   if (threadIdx.x == 0) {
     *c = 0;
     *edges = 0ull;
+    *pc = 0;
     if (*cin > 0) ++stat[0];
+  }
+}
+
+// A frontier vertex offers itself (k) to neighbour r: SelectMax keeps the
+// largest offer in cand[r]; the first offer appends r to the next frontier
+// (one atomic per wave for the appends of this step).  All lanes call it.
+__device__ __forceinline__ void bfs_offer(int32_t r, int32_t k, const uint32_t* __restrict__ visited,
+                                          long long* __restrict__ cand, int32_t* __restrict__ fout,
+                                          int32_t* __restrict__ fout_count) {
+  bool push = false;
+  if (r >= 0 && !((visited[r >> 5] >> (r & 31)) & 1u)) {
+    // Offers only grow cand, so a value read without the atomic (possibly
+    // stale, never above the true one) that already holds an offer at least
+    // this large makes the atomic a no-op: skip it.  Hubs receive thousands
+    // of offers per level and only the record-breaking ones reach the
+    // serialising atomic.
+    const long long seen = __hip_atomic_load(&cand[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (seen < (long long)k) {
+      const long long old = atomicMax(&cand[r], (long long)k);
+      push = old == -1;
+    }
+  }
+  const uint64_t mask = __ballot(push);
+  if (mask) {
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    int basei = 0;
+    if (lane_id() == leader) basei = atomicAdd(fout_count, __popcll(mask));
+    basei = __shfl(basei, leader, kWave);
+    if (push) fout[basei + __popcll(mask & ((1ull << lane_id()) - 1ull))] = r;
   }
 }
 
@@ -420,6 +471,7 @@ __global__ __launch_bounds__(NT) void k_bfs_expand(const int64_t* __restrict__ b
       k = fin[t];
       base = beg[k];
       len = (int)(end[k] - base);
+      if (len > kBfsHub) len = 0;  // a hub: k_bfs_expand_hubs takes its pieces
     }
     int nseg;
     const int tot = stage_chunk<int32_t, NT>(st, wsum, base, len, k, nseg);
@@ -428,34 +480,32 @@ __global__ __launch_bounds__(NT) void k_bfs_expand(const int64_t* __restrict__ b
 #pragma unroll
       for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rows[idx[u]] : -1;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        bool push = false;
-        if (r[u] >= 0 && !((visited[r[u] >> 5] >> (r[u] & 31)) & 1u)) {
-          // Offers only grow cand, so a value read without the atomic (possibly
-          // stale, never above the true one) that already holds an offer at
-          // least this large makes the atomic a no-op: skip it.  Hubs receive
-          // thousands of offers per level and only the record-breaking ones
-          // reach the serialising atomic.
-          const long long seen = __hip_atomic_load(&cand[r[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (seen < (long long)ks[u]) {
-            const long long old = atomicMax(&cand[r[u]], (long long)ks[u]);
-            push = old == -1;
-          }
-        }
-        // one atomic per wave for the appends of this step
-        const uint64_t mask = __ballot(push);
-        if (mask) {
-          const int leader = __ffsll((unsigned long long)mask) - 1;
-          int basei = 0;
-          if (lane_id() == leader) basei = atomicAdd(fout_count, __popcll(mask));
-          basei = __shfl(basei, leader, kWave);
-          if (push) {
-            const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
-            fout[basei + __popcll(below)] = r[u];
-          }
-        }
-      }
+      for (int u = 0; u < U; ++u) bfs_offer(r[u], ks[u], visited, cand, fout, fout_count);
     });
+  }
+}
+
+// The hubs' pieces of the same level: one workgroup per piece of kBfsHub
+// consecutive edges (coalesced row reads, kBfsHub / NT edges per thread).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_bfs_expand_hubs(const int64_t* __restrict__ beg,
+                                                        const int64_t* __restrict__ end,
+                                                        const int32_t* __restrict__ rows,
+                                                        const long long* __restrict__ pieces,
+                                                        const int32_t* __restrict__ pcount,
+                                                        const uint32_t* __restrict__ visited,
+                                                        long long* __restrict__ cand, int32_t* __restrict__ fout,
+                                                        int32_t* __restrict__ fout_count) {
+  static_assert(kBfsHub % NT == 0, "whole rounds per piece");
+  const int np = *pcount;
+  for (int p = blockIdx.x; p < np; p += gridDim.x) {
+    const long long pc = pieces[p];
+    const int32_t v = (int32_t)(pc >> 32);
+    const int64_t k = pc & 0xffffffffll;
+    const int64_t e0 = beg[v] + k * kBfsHub, e1 = min(end[v], e0 + kBfsHub);
+#pragma unroll
+    for (int64_t e = e0 + threadIdx.x; e < e0 + kBfsHub; e += NT)  // uniform trip count (ballots inside)
+      bfs_offer(e < e1 ? rows[e] : -1, v, visited, cand, fout, fout_count);
   }
 }
 
@@ -471,7 +521,9 @@ __global__ __launch_bounds__(kBfsFinNT) void k_bfs_finalize(const int32_t* __res
                                                             int64_t* __restrict__ parents,
                                                             uint32_t* __restrict__ visited,
                                                             unsigned long long* __restrict__ edges,
-                                                            unsigned long long* __restrict__ reached) {
+                                                            unsigned long long* __restrict__ reached,
+                                                            int32_t* __restrict__ pcount,
+                                                            long long* __restrict__ pieces) {
   __shared__ long long ws[kBfsFinNT / kWave];
   const int nf = *count;
   if (blockIdx.x == 0 && threadIdx.x == 0 && nf) atomicAdd(reached, (unsigned long long)nf);
@@ -480,7 +532,9 @@ __global__ __launch_bounds__(kBfsFinNT) void k_bfs_finalize(const int32_t* __res
     const int32_t i = f[t];
     parents[i] = cand[i];
     atomicOr(&visited[i >> 5], 1u << (i & 31));
-    deg += end[i] - beg[i];
+    const int64_t d = end[i] - beg[i];
+    deg += d;
+    if (d > kBfsHub) bfs_add_pieces(i, d, pcount, pieces);  // the next level expands hubs in pieces
   }
   deg = wave_sum_ll(deg);
   if (lane_id() == 0) ws[wave_id()] = deg;
@@ -588,6 +642,8 @@ struct cbx_bfs_plan_s {
   hipStream_t stream = nullptr;
   int64_t *beg = nullptr, *end = nullptr, *parents = nullptr;
   int32_t *rows = nullptr, *f[2] = {nullptr, nullptr}, *counts = nullptr;
+  int32_t* pcount = nullptr;                  // hub pieces of the two frontier buffers
+  long long* pieces[2] = {nullptr, nullptr};
   long long* cand = nullptr;
   uint32_t* visited = nullptr;
   unsigned long long* edges = nullptr;  // degree sums of the two frontier buffers
@@ -619,10 +675,12 @@ static void bfs_plan_free(cbx_bfs_plan_s* p) {
     }
   if (p->td_exec) (void)hipGraphExecDestroy(p->td_exec);
   if (p->td_graph) (void)hipGraphDestroy(p->td_graph);
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i) {
     if (p->f[i]) (void)hipFree(p->f[i]);
+    if (p->pieces[i]) (void)hipFree(p->pieces[i]);
+  }
   for (void* q : {(void*)p->beg, (void*)p->end, (void*)p->parents, (void*)p->rows, (void*)p->counts, (void*)p->cand,
-                  (void*)p->visited, (void*)p->edges, (void*)p->fbits, (void*)p->stat})
+                  (void*)p->visited, (void*)p->edges, (void*)p->fbits, (void*)p->stat, (void*)p->pcount})
     if (q) (void)hipFree(q);
   if (p->host_count) (void)hipHostFree(p->host_count);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -631,6 +689,7 @@ static void bfs_plan_free(cbx_bfs_plan_s* p) {
 
 constexpr int kBfsNT = 256;
 constexpr unsigned kBfsGrid = 2048;
+constexpr unsigned kBfsHubGrid = 2048;  // workgroups over the hubs' pieces
 constexpr int64_t kBfsAlpha = 256;
 constexpr int kBfsTdLevels = 8;  // even: the frontier parity is 0 again after one top-down graph
 
@@ -683,12 +742,16 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
   };
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream");
   const int64_t n = p->n;
+  // every hub of a frontier contributes ceil(deg / kBfsHub) <= 2 deg / kBfsHub pieces
+  const int64_t max_pieces = 2 * p->nnz / kBfsHub + 2;
   if (hipMalloc(&p->beg, 8 * std::max<int64_t>(n, 1)) || hipMalloc(&p->end, 8 * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->parents, 8 * std::max<int64_t>(n, 1)) || hipMalloc(&p->rows, 4 * std::max<int64_t>(p->nnz, 1)) ||
       hipMalloc(&p->f[0], 4 * std::max<int64_t>(n, 1)) || hipMalloc(&p->f[1], 4 * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->counts, 16) || hipMalloc(&p->cand, 8 * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->visited, 4 * p->nw) || hipMalloc(&p->edges, 2 * sizeof(unsigned long long)) ||
       hipMalloc(&p->fbits, 4 * p->nw) || hipMalloc(&p->stat, 2 * sizeof(unsigned long long)) ||
+      hipMalloc(&p->pcount, 2 * sizeof(int32_t)) || hipMalloc(&p->pieces[0], 8 * max_pieces) ||
+      hipMalloc(&p->pieces[1], 8 * max_pieces) ||
       hipHostMalloc(&p->host_count, 64, hipHostMallocDefault))
     return fail("allocation");
   hipStream_t s = p->stream;
@@ -706,10 +769,13 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
       int32_t* cout = p->counts + (1 - par);
       unsigned long long* eout = p->edges + (1 - par);
       if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail("capture begin");
-      k_bfs_level_begin<<<1, 64, 0, s>>>(cin, cout, eout, p->stat);
+      k_bfs_level_begin<<<1, 64, 0, s>>>(cin, cout, eout, p->pcount + (1 - par), p->stat);
       if (dir == 0) {
         k_bfs_expand<kBfsNT, 4><<<kBfsGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->f[par], cin, p->visited,
                                                            p->cand, p->f[1 - par], cout);
+        k_bfs_expand_hubs<kBfsNT><<<kBfsHubGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->pieces[par],
+                                                                 p->pcount + par, p->visited, p->cand,
+                                                                 p->f[1 - par], cout);
       } else {
         k_bfs_fbits_clear<<<nb(p->nw, 256), 256, 0, s>>>(p->fbits, p->nw);
         k_bfs_fbits_set<<<256, 256, 0, s>>>(p->f[par], cin, p->fbits);
@@ -717,7 +783,7 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
                                                 p->f[1 - par], cout);
       }
       k_bfs_finalize<<<256, kBfsFinNT, 0, s>>>(p->f[1 - par], cout, p->cand, p->beg, p->end, p->parents, p->visited,
-                                               eout, p->stat + 1);
+                                               eout, p->stat + 1, p->pcount + (1 - par), p->pieces[1 - par]);
       (void)hipMemcpyAsync(p->host_count, cout, sizeof(int32_t), hipMemcpyDeviceToHost, s);
       (void)hipMemcpyAsync(p->host_count + 2, eout, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
       if (hipStreamEndCapture(s, &p->graph[dir][par]) != hipSuccess) return fail("capture end");
@@ -732,11 +798,13 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
     int32_t* cin = p->counts + par;
     int32_t* cout = p->counts + (1 - par);
     unsigned long long* eout = p->edges + (1 - par);
-    k_bfs_level_begin<<<1, 64, 0, s>>>(cin, cout, eout, p->stat);
+    k_bfs_level_begin<<<1, 64, 0, s>>>(cin, cout, eout, p->pcount + (1 - par), p->stat);
     k_bfs_expand<kBfsNT, 4><<<kBfsGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->f[par], cin, p->visited, p->cand,
                                                        p->f[1 - par], cout);
+    k_bfs_expand_hubs<kBfsNT><<<kBfsHubGrid, kBfsNT, 0, s>>>(p->beg, p->end, p->rows, p->pieces[par], p->pcount + par,
+                                                             p->visited, p->cand, p->f[1 - par], cout);
     k_bfs_finalize<<<256, kBfsFinNT, 0, s>>>(p->f[1 - par], cout, p->cand, p->beg, p->end, p->parents, p->visited,
-                                             eout, p->stat + 1);
+                                             eout, p->stat + 1, p->pcount + (1 - par), p->pieces[1 - par]);
   }
   (void)hipMemcpyAsync(p->host_count, p->counts, sizeof(int32_t), hipMemcpyDeviceToHost, s);
   if (hipStreamEndCapture(s, &p->td_graph) != hipSuccess) return fail("capture end");
@@ -755,7 +823,8 @@ extern "C" cbx_status cbx_bfs_run(cbx_bfs_plan_s* p, int64_t root, int64_t* pare
   CBX_HIP(hipEventCreate(&e1));
   CBX_HIP(hipEventRecord(e0, s));
   k_bfs_reset<<<nb(std::max(p->n, p->nw), 256), 256, 0, s>>>(p->n, p->cand, p->parents, p->visited, p->nw, p->f[0],
-                                                             p->counts, p->stat, root);
+                                                             p->counts, p->pcount, p->pieces[0], p->beg, p->end,
+                                                             p->stat, root);
   // direction per level: bottom-up while the frontier's edges exceed 1/256 of
   // the edges still unexplored.  (Beamer's rule compares the two with a factor
   // 14 for equal per-edge costs; here a top-down edge costs a device atomic on
