@@ -226,10 +226,10 @@ def bench_mcl(cb, n: int = 1_000_000, reps: int = 2) -> dict:
     base = cb.get_config()
     per_mode = {}
     for mode in ("two_phase", "learned"):
-        if mode == "learned":
-            if not predictor.MODEL_PATH.exists():
+        if mode == "learned":  # the network trained on this input family (power-law matrices)
+            if not predictor.model_path("powerlaw").exists():
                 continue
-            cb.set_predictor(predictor.load())
+            cb.set_predictor(predictor.load("powerlaw"))
         cb.set_config(**base)
         tuned = autotune.apply("mcl_n1m" + ("_learned" if mode == "learned" else ""))
         P, st = cb.mcl_expand(dA, dA, *params, mode=mode)

@@ -62,6 +62,7 @@ class cbx_spgemm_stats(C.Structure):
         ("units_light", C.c_int64), ("units_heavy", C.c_int64), ("nnz_main", C.c_int64),
         ("ms_setup", C.c_double), ("ms_symbolic", C.c_double), ("ms_alloc", C.c_double),
         ("ms_numeric", C.c_double), ("ms_main_kernel", C.c_double), ("ms_total", C.c_double),
+        ("nnz_before_filter", C.c_int64),
     ]
 
 
@@ -87,7 +88,9 @@ class cbx_mcl_stats(C.Structure):
 class cbx_spgemm_config(C.Structure):
     _fields_ = [("hash_max_flop", C.c_int64), ("hash_mid_flop", C.c_int64), ("tiny_flop", C.c_int64),
                 ("heavy_nnz", C.c_int64), ("unit_split_nnz", C.c_int64), ("learned_target", C.c_float),
-                ("capacity_margin", C.c_float), ("row_map_share", C.c_float), ("tiny_wg_waves", C.c_int32)]
+                ("capacity_margin", C.c_float), ("row_map_share", C.c_float), ("tiny_wg_waves", C.c_int32),
+                ("unit_order", C.c_int32), ("tiny_algo", C.c_int32), ("sym_band", C.c_int32),
+                ("mcl_fuse", C.c_int32), ("unit_lists", C.c_int32)]
 
 
 _lib_handle = None

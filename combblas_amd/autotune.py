@@ -79,10 +79,14 @@ def _mcl_workload(n: int, mode: str):
 
 
 GRID = {
-    "hash_max_flop": [1024, 2048, 4096],
-    "hash_mid_flop": [512, 1024],
-    "tiny_flop": [128, 256],
-    "unit_split_nnz": [65536, 131072, 262144, 524288],
+    "hash_max_flop": [1024, 2048],
+    "hash_mid_flop": [1024],
+    "tiny_flop": [256],
+    "unit_split_nnz": [131072, 262144],
+    "unit_order": [0, 1],
+    "tiny_algo": [0, 1],
+    "sym_band": [0, 16],
+    "unit_lists": [0, 1],
 }
 # the learned single pass adds its own knobs; the hash tiers are swept more
 # coarsely there to keep the product of the grids small
@@ -90,21 +94,25 @@ LEARNED_GRID = {
     "hash_max_flop": [1024, 2048],
     "hash_mid_flop": [1024],
     "tiny_flop": [256],
-    "unit_split_nnz": [131072, 262144],
+    "unit_split_nnz": [131072],
     "learned_target": [0.75, 0.9],
-    "heavy_nnz": [8192, 16384, 32768, 65536],
+    "heavy_nnz": [8192, 16384, 32768],
+    "sym_band": [0, 16],
 }
 # Galerkin columns hold 216 (A*T) and 64 (S*(A*T)) products: the tiny tier's
-# bound and workgroup shape decide where they run
+# bound, form and workgroup shape decide where they run
 GALERKIN_GRID = {
     "tiny_flop": [128, 256],
     "hash_mid_flop": [256, 1024],
     "tiny_wg_waves": [1, 2, 4],
+    "tiny_algo": [0, 1],
 }
 MCL_GRID = {
-    "hash_max_flop": [1024, 2048, 4096],
-    "tiny_wg_waves": [2, 4],
-    "unit_split_nnz": [131072, 262144],
+    "hash_max_flop": [1024, 2048],
+    "unit_split_nnz": [262144],
+    "sym_band": [0, 16],
+    "mcl_fuse": [0, 1],
+    "unit_lists": [0, 1],
 }
 MCL_LEARNED_GRID = {
     "hash_max_flop": [2048, 4096],
@@ -147,9 +155,9 @@ def _time_config(run, reps: int) -> tuple[float, int]:
 
 def tune(name: str, reps: int = 2, log=print) -> dict:
     setup, desc, mode, grid = WORKLOADS[name]
-    if mode == "learned":
+    if mode == "learned":  # the MCL expansion's input family has its own model
         from combblas_amd import predictor
-        cb.set_predictor(predictor.load())
+        cb.set_predictor(predictor.load("powerlaw" if name.startswith("mcl") else "rmat"))
     run, release = setup()
     base = cb.get_config()
     keys = list(grid)
@@ -197,10 +205,10 @@ def main(argv: list[str]) -> int:
     out = Path(argv[argv.index("--out") + 1]) if "--out" in argv else TABLE_PATH
     names = [n for n in names if n in WORKLOADS]
     table = load_table(out)
-    for n in names:
-        table["entries"][n] = tune(n)
     out.parent.mkdir(parents=True, exist_ok=True)
-    out.write_text(json.dumps(table, indent=1) + "\n")
+    for n in names:  # the table is written after every workload, so a cut-off sweep keeps the finished ones
+        table["entries"][n] = tune(n)
+        out.write_text(json.dumps(table, indent=1) + "\n")
     print(json.dumps({n: {k: table["entries"][n][k] for k in ("best", "best_ms", "default_ms")} for n in names},
                      indent=1))
     return 0

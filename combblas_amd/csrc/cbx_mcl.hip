@@ -563,6 +563,17 @@ static cbx_status mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, double thr, i
   }
   phases = std::max<int64_t>(1, std::min<int64_t>(phases, std::max<int64_t>(nzcB, 1)));
   st->phases = phases;
+  // With no recovery possible (recover_num 0), the epilogue's result on the
+  // product thresholded at hard_threshold equals its result on the whole
+  // product: its statistics count entries above the threshold only, and a
+  // selection cut (the select_num-th largest of more than select_num such
+  // entries) lies above it.  The two-phase product then emits only those
+  // entries (fused threshold) instead of all of them.
+  SpgemmConfig pcfg = cfg;
+  if (recover <= 0 && mode == CBX_SPGEMM_TWO_PHASE && cfg.mcl_fuse) {
+    pcfg.filter = true;
+    pcfg.filter_thr = thr;
+  }
   Parts parts;
   Events ev;
   int64_t q0 = 0, acc = 0;
@@ -577,11 +588,11 @@ static cbx_status mcl_expand(const cbx_dcsc* A, const cbx_dcsc* B, double thr, i
     CBX_TRY(column_slice(B, cpB.data(), q0, q1, &Bp.x, s));
     cbx_spgemm_stats sst;
     CBX_HIP(hipEventRecord(ev.a, s));
-    CBX_TRY(spgemm_dispatch(A, &Bp.x, CBX_SR_PLUS_TIMES, mode, cfg, &Cp.x, &sst, s));
+    CBX_TRY(spgemm_dispatch(A, &Bp.x, CBX_SR_PLUS_TIMES, mode, pcfg, &Cp.x, &sst, s));
     CBX_HIP(hipEventRecord(ev.b, s));
     CBX_HIP(hipEventSynchronize(ev.b));
     st->ms_product += ev.span();
-    st->nnz_product += sst.nnz;
+    st->nnz_product += sst.nnz_before_filter;
     st->overflow_cols += sst.overflow_cols;
     st->learned_fallback += sst.learned_fallback;
     cbx_dcsc P{};

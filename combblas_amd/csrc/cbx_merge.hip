@@ -175,14 +175,10 @@ cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipS
   CBX_HIP(rocprim::merge(nullptr, t1, k0, k0, k1, v0, v0, v1, (size_t)N, (size_t)N, rocprim::less<uint64_t>(), s));
   CBX_HIP((assemble_tmp_bytes<K, T>(N, s, t2)));
   tb = std::max(t1, t2);
-  auto cleanup = [&]() {
-    for (void* p : {(void*)k0, (void*)k1, (void*)ukeys, (void*)v0, (void*)v1, (void*)nruns, (void*)counts, tmp})
-      if (p) (void)hipFreeAsync(p, s);
-  };
-  if (hipMallocAsync((void**)&k0, sizeof(uint64_t) * N, s) || hipMallocAsync((void**)&k1, sizeof(uint64_t) * N, s) ||
-      hipMallocAsync((void**)&ukeys, sizeof(uint64_t) * N, s) || hipMallocAsync((void**)&v0, sizeof(T) * N, s) ||
-      hipMallocAsync((void**)&v1, sizeof(T) * N, s) || hipMallocAsync((void**)&nruns, 2 * sizeof(int64_t), s) ||
-      hipMallocAsync((void**)&counts, sizeof(int64_t) * N, s) || hipMallocAsync(&tmp, std::max<size_t>(tb, 16), s)) {
+  ScratchSet ws(s);
+  auto cleanup = [&]() { ws.release(); };
+  if (!ws.take(&k0, N) || !ws.take(&k1, N) || !ws.take(&ukeys, N) || !ws.take(&v0, N) || !ws.take(&v1, N) ||
+      !ws.take(&nruns, 2) || !ws.take(&counts, N) || !ws.take_bytes(&tmp, std::max<size_t>(tb, 16))) {
     (void)hipGetLastError();
     cleanup();
     return set_error(CBX_ERR_NOMEM, "cbx_dcsc_merge: workspace allocation failed");
@@ -274,16 +270,11 @@ cbx_status coo_typed(int64_t m, int64_t n, const int64_t* rows, const int64_t* c
   CBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, k0, k1, v0, v1, N, 0, end_bit, s));
   CBX_HIP((assemble_tmp_bytes<K, T>(N, s, t2)));
   tb = std::max(t1, t2);
-  auto cleanup = [&]() {
-    for (void* p : {(void*)k0, (void*)k1, (void*)ukeys, (void*)v0, (void*)v1, (void*)nruns, (void*)counts,
-                    (void*)bad, tmp})
-      if (p) (void)hipFreeAsync(p, s);
-  };
-  if (hipMallocAsync((void**)&k0, sizeof(uint64_t) * N, s) || hipMallocAsync((void**)&k1, sizeof(uint64_t) * N, s) ||
-      hipMallocAsync((void**)&ukeys, sizeof(uint64_t) * N, s) || hipMallocAsync((void**)&v0, sizeof(T) * N, s) ||
-      hipMallocAsync((void**)&v1, sizeof(T) * N, s) || hipMallocAsync((void**)&nruns, 2 * sizeof(int64_t), s) ||
-      hipMallocAsync((void**)&counts, sizeof(int64_t) * N, s) ||
-      hipMallocAsync((void**)&bad, 2 * sizeof(unsigned long long), s) || hipMallocAsync(&tmp, std::max<size_t>(tb, 16), s)) {
+  ScratchSet ws(s);
+  auto cleanup = [&]() { ws.release(); };
+  if (!ws.take(&k0, N) || !ws.take(&k1, N) || !ws.take(&ukeys, N) || !ws.take(&v0, N) || !ws.take(&v1, N) ||
+      !ws.take(&nruns, 2) || !ws.take(&counts, N) || !ws.take(&bad, 2) ||
+      !ws.take_bytes(&tmp, std::max<size_t>(tb, 16))) {
     (void)hipGetLastError();
     cleanup();
     return set_error(CBX_ERR_NOMEM, "cbx_dcsc_from_coo: workspace allocation failed");

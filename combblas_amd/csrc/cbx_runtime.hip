@@ -151,6 +151,28 @@ using namespace cbx;
 
 extern "C" void cbx_release_cached_memory(void) { release_cache(); }
 
+cbx_status cbx::scratch_alloc(void** p, size_t bytes, hipStream_t s) {
+  bytes = std::max<size_t>(bytes, 256);
+  if (bytes >= kScratchCacheMin) return dev_alloc(p, bytes);
+  const hipError_t e = hipMallocAsync(p, bytes, s);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return set_error(CBX_ERR_NOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  }
+  return CBX_OK;
+}
+
+void cbx::scratch_free(void* p, size_t bytes, hipStream_t s) {
+  if (!p) return;
+  if (std::max<size_t>(bytes, 256) >= kScratchCacheMin) {
+    (void)hipStreamSynchronize(s);  // the stream's kernels may still read it
+    dev_free(p);
+  } else {
+    (void)hipFreeAsync(p, s);
+  }
+}
+
 extern "C" int32_t cbx_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -353,7 +375,7 @@ extern "C" cbx_status cbx_spgemm_set_sampling_predictor(int32_t nrounds) {
 
 // ------------------------------------------------------------- tunables
 static std::mutex g_config_mu;
-static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f, 4};
+static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f, 4, 1, 0, 0, 1, 1};
 
 SpgemmConfig cbx::current_config() {
   SpgemmConfig c;
@@ -368,6 +390,11 @@ SpgemmConfig cbx::current_config() {
     c.capacity_margin = g_config.capacity_margin;
     c.row_map_share = g_config.row_map_share;
     c.tiny_wg_waves = g_config.tiny_wg_waves;
+    c.unit_order = g_config.unit_order;
+    c.tiny_algo = g_config.tiny_algo;
+    c.sym_band = g_config.sym_band;
+    c.mcl_fuse = g_config.mcl_fuse != 0;
+    c.unit_lists = g_config.unit_lists;
   }
   c.predictor = predictor_params();
   c.sampling_rounds = sampling_rounds_setting();
@@ -400,6 +427,16 @@ extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
                 "cbx_spgemm_set_config: row_map_share must be in [0, 1]");
   CBX_RETURN_IF(cfg->tiny_wg_waves != 1 && cfg->tiny_wg_waves != 2 && cfg->tiny_wg_waves != 4, CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: tiny_wg_waves must be 1, 2 or 4");
+  CBX_RETURN_IF(cfg->unit_order != 0 && cfg->unit_order != 1, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: unit_order must be 0 or 1");
+  CBX_RETURN_IF(cfg->tiny_algo != 0 && cfg->tiny_algo != 1, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: tiny_algo must be 0 or 1");
+  CBX_RETURN_IF(cfg->sym_band != 0 && cfg->sym_band != 16 && cfg->sym_band != 64, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: sym_band must be 0, 16 or 64");
+  CBX_RETURN_IF(cfg->mcl_fuse != 0 && cfg->mcl_fuse != 1, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: mcl_fuse must be 0 or 1");
+  CBX_RETURN_IF(cfg->unit_lists != 0 && cfg->unit_lists != 1, CBX_ERR_INVALID,
+                "cbx_spgemm_set_config: unit_lists must be 0 or 1");
   std::lock_guard<std::mutex> g(g_config_mu);
   g_config = *cfg;
   return CBX_OK;

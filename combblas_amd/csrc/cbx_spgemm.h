@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+#include <vector>
+
 #include "cbx_internal.h"
 
 namespace cbx {
@@ -42,6 +45,24 @@ struct SpgemmConfig {
   float row_map_share = 0.25f;
   // Workgroup shape of the tiny tier: columns (one wave each) per workgroup.
   int tiny_wg_waves = 4;
+  // Exactly sized (two-phase) units in the symbolic pass's order (0) or
+  // sorted by first row block (1).
+  int unit_order = 1;
+  // Tiny tier: register bitonic sort (0) or per-wave LDS hash table (1).
+  int tiny_algo = 0;
+  // Symbolic pass of the unit tier: per column (0) or in bands of 16 / 64
+  // row blocks dispatched band by band.
+  int sym_band = 0;
+  // Fused threshold (set by the MCL expansion, two-phase mode only): the
+  // product keeps only entries with !(v < filter_thr).
+  bool filter = false;
+  double filter_thr = 0.0;
+  // cbx_mcl_expand may fuse its hard threshold into two-phase products.
+  bool mcl_fuse = true;
+  // Two-phase, per-column symbolic form: the symbolic pass writes the sorted
+  // rows of the small-unit columns, whose units are then cut by output count
+  // over any number of row blocks (k_num_list).
+  int unit_lists = 1;
 };
 
 // The process-wide tunables (cbx_spgemm_set_config) with the installed
@@ -66,6 +87,43 @@ cbx_status dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
 void release_cache();
 size_t cache_idle_bytes();  // bytes held by the cache and free for reuse
+
+// Scratch of one call on stream s.  Blocks of kScratchCacheMin bytes or more
+// come from the caching allocator above (plain hipMalloc on a miss, which is
+// fast); smaller ones from the stream-ordered pool.  Growing the pool by
+// gigabytes has taken up to 1.5 s per allocation, so large scratch stays out
+// of it.  scratch_free waits for s before it caches a large block.
+constexpr size_t kScratchCacheMin = (size_t)64 << 20;
+cbx_status scratch_alloc(void** p, size_t bytes, hipStream_t s);
+void scratch_free(void* p, size_t bytes, hipStream_t s);
+
+// The scratch blocks of one call, released together (at the latest when the
+// set goes out of scope).
+struct ScratchSet {
+  hipStream_t s;
+  std::vector<std::pair<void*, size_t>> blocks;
+  explicit ScratchSet(hipStream_t st) : s(st) {}
+  ScratchSet(const ScratchSet&) = delete;
+  ScratchSet& operator=(const ScratchSet&) = delete;
+  bool take_bytes(void** p, size_t bytes) {
+    bytes = bytes < 256 ? 256 : bytes;
+    if (scratch_alloc(p, bytes, s) != CBX_OK) return false;
+    blocks.emplace_back(*p, bytes);
+    return true;
+  }
+  template <class X>
+  bool take(X** p, size_t count) {
+    void* q = nullptr;
+    const bool ok = take_bytes(&q, count * sizeof(X));
+    *p = (X*)q;
+    return ok;
+  }
+  void release() {
+    for (auto& b : blocks) scratch_free(b.first, b.second, s);
+    blocks.clear();
+  }
+  ~ScratchSet() { release(); }
+};
 
 // Shared front-door helpers (cbx_runtime.hip).
 size_t dtype_size(cbx_dtype t);

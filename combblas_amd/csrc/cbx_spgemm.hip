@@ -112,6 +112,42 @@ struct Unit {
   int32_t slot;    // the column's row-map slot (its workgroup in the symbolic launch)
 };
 
+// Fused threshold of the MCL expansion (MCLPruneRecoverySelect's hard
+// threshold, ParFriends.h:185-345, when no column can need recovery): with
+// `on`, the numeric kernels emit only the entries with !(v < thr) of every
+// output range they own, compacted at the range's start, and append the
+// range (start, kept count, column) to a run list; the host code then gathers
+// the runs into the product.  The epilogue on that product keeps exactly what
+// it keeps on the unfiltered one (its statistics count only entries > thr and
+// its cuts are never below thr), and the product's writes shrink to the kept
+// entries.
+template <class T>
+struct Filter {
+  int on = 0;
+  T thr{};
+  int64_t* run_src = nullptr;
+  int32_t* run_kept = nullptr;
+  int32_t* run_col = nullptr;
+  int* run_count = nullptr;
+  int run_cap = 0;
+  // entries the staged tiny tier left out (its kept counts size the columns,
+  // so the product's unfiltered size is the scan's total plus these)
+  unsigned long long* dropped = nullptr;
+};
+
+template <class T>
+__device__ __forceinline__ void filter_add_run(const Filter<T>& f, int64_t src, int kept, int col, int* err) {
+  if (kept <= 0) return;
+  const int k = atomicAdd(f.run_count, 1);
+  if (k < f.run_cap) {
+    f.run_src[k] = src;
+    f.run_kept[k] = kept;
+    f.run_col[k] = col;
+  } else {
+    atomicOr(err, 1024);
+  }
+}
+
 // Segment of B entry e restricted to P tiles [t_lo, t_hi) (or the whole A
 // column when `full`).
 template <class T, bool VAL>
@@ -371,7 +407,8 @@ __device__ __forceinline__ void tiny_finish(int32_t (&key)[kTinyMax / kWave],
                                             typename Sem<K, T>::Acc (&val)[kTinyMax / kWave], int lane, int col,
                                             int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpC,
                                             int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err,
-                                            const int64_t* __restrict__ stage_off, int64_t slot) {
+                                            const int64_t* __restrict__ stage_off, int64_t slot,
+                                            const Filter<T>& flt) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int32_t kNone = 0x7fffffff;
@@ -450,6 +487,7 @@ __device__ __forceinline__ void tiny_finish(int32_t (&key)[kTinyMax / kWave],
     Acc carry = SR::ident();
     bool open = false;  // a run continues from the previous register
     int before = 0;     // heads in earlier registers
+    int kept = 0;       // entries written (filter on)
 #pragma unroll
     for (int u = 0; u < NR; ++u) {
       // segmented inclusive scan in element order
@@ -469,18 +507,104 @@ __device__ __forceinline__ void tiny_finish(int32_t (&key)[kTinyMax / kWave],
       int32_t next = __shfl_down(key[u], 1, kWave);
       const int32_t nfirst = u + 1 < NR ? __shfl(key[u + 1 < NR ? u + 1 : u], 0, kWave) : kNone;
       if (lane == kWave - 1) next = nfirst;
-      if (key[u] != kNone && next != key[u]) {
-        const int rank = before + __popcll(hm[u] & ((2ull << lane) - 1ull)) - 1;
-        if (rank < have) {
-          irC[out + rank] = key[u];
-          valC[out + rank] = SR::out(x);
+      const bool end = key[u] != kNone && next != key[u];
+      if (!flt.on) {
+        if (end) {
+          const int rank = before + __popcll(hm[u] & ((2ull << lane) - 1ull)) - 1;
+          if (rank < have) {
+            irC[out + rank] = key[u];
+            valC[out + rank] = SR::out(x);
+          }
         }
+      } else {  // the fused threshold: kept entries, in row order, at out
+        const T v = SR::out(x);
+        const bool keep = end && !(v < flt.thr);
+        const uint64_t km = __ballot(keep);
+        if (keep) {
+          const int rank = kept + __popcll(km & ((1ull << lane) - 1ull));
+          if (rank < have) {
+            irC[out + rank] = key[u];
+            valC[out + rank] = v;
+          }
+        }
+        kept += __popcll(km);
       }
       carry = __shfl(x, kWave - 1, kWave);
       open = __shfl((int)(key[u] != kNone), kWave - 1, kWave) != 0;
       before += __popcll(hm[u]);
     }
+    if (flt.on) {  // staged: the kept count sizes the column; else its range joins the runs
+      if (stage_off) {
+        if (lane == 0) {
+          nnzcol[col] = kept;
+          if (count > kept) atomicAdd(flt.dropped, (unsigned long long)(count - kept));
+        }
+      } else if (lane == 0) {
+        filter_add_run(flt, out, kept, col, err);
+      }
+    }
   }
+}
+
+// The products of one tiny column (at most kTinyMax of them) gathered into
+// the wave's registers: element e = u * 64 + lane sits in key[u] (its row)
+// and val[u] (its product); elements past the column's products hold kNone.
+// Returns the number of products (wave-uniform).  All lanes must call it.
+template <int K, class T, bool SYM>
+__device__ __forceinline__ int tiny_gather(int64_t b0, int64_t b1, const int32_t* __restrict__ bpos,
+                                           const T* __restrict__ valB, const int64_t* __restrict__ cpA,
+                                           const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
+                                           int32_t (&key)[kTinyMax / kWave],
+                                           typename Sem<K, T>::Acc (&val)[kTinyMax / kWave]) {
+  using SR = Sem<K, T>;
+  constexpr int kE = kTinyMax / kWave;
+  constexpr int32_t kNone = 0x7fffffff;
+  const int lane = lane_id();
+#pragma unroll
+  for (int u = 0; u < kE; ++u) {
+    key[u] = kNone;
+    val[u] = SR::ident();
+  }
+  int filled = 0;  // wave-uniform
+  for (int64_t eb = b0; eb < b1; eb += kWave) {
+    int64_t base;
+    int len;
+    T bv;
+    load_seg<T, !SYM>(eb + lane, b1, bpos, cpA, nullptr, 0, 0, 0, true, valB, base, len, bv);
+    const int inc = wave_incl_scan(len);
+    const int off = filled + inc - len;  // first element of this lane's segment
+    const int total = __shfl(inc, kWave - 1, kWave);
+    // lanes that hold B entries in this chunk, and the search's first step:
+    // the largest power of two below that count (all wave-uniform)
+    const int nb = (int)(b1 - eb < kWave ? b1 - eb : kWave);
+    int top = 0;
+    if (nb > 1) top = 1 << (31 - __clz(nb - 1));
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      // registers whose 64 elements miss this chunk's range stay as they are
+      if ((u + 1) * kWave <= filled || u * kWave >= filled + total) continue;
+      const int e = u * kWave + lane;
+      // last lane s < nb with off_s <= e (segments of length 0 share
+      // offsets; the search lands on the one that holds e when e is in this
+      // chunk)
+      int lo = 0;
+      for (int step = top; step > 0; step >>= 1) {
+        const int o = __shfl(off, lo + step, kWave);
+        if (lo + step < nb && o <= e) lo += step;
+      }
+      const int o = __shfl(off, lo, kWave);
+      const int l = __shfl(len, lo, kWave);
+      const int64_t bs = shfl_x(base, lo);
+      const T bval = shfl_x(bv, lo);
+      if (e >= filled && e < filled + total && e - o < l) {
+        const int64_t idx = bs + (e - o);
+        key[u] = rowsA[idx];
+        if constexpr (!SYM) val[u] = SR::mul(valA[idx], bval);
+      }
+    }
+    filled += total;
+  }
+  return filled;
 }
 
 template <int K, class T, bool SYM>
@@ -490,7 +614,7 @@ __global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order,
                                               const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
                                               int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpC,
                                               int64_t* __restrict__ irC, T* __restrict__ valC, int* __restrict__ err,
-                                              const int64_t* __restrict__ stage_off = nullptr) {
+                                              const int64_t* __restrict__ stage_off = nullptr, Filter<T> flt = {}) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int kE = kTinyMax / kWave;  // 4 elements per lane
@@ -502,56 +626,198 @@ __global__ __launch_bounds__(256) void k_tiny(const int32_t* __restrict__ order,
     const int64_t b0 = cpB[col], b1 = cpB[col + 1];
     int32_t key[kE];
     Acc val[kE];
-#pragma unroll
-    for (int u = 0; u < kE; ++u) {
-      key[u] = kNone;
-      val[u] = SR::ident();
-    }
-    int filled = 0;  // wave-uniform
-    for (int64_t eb = b0; eb < b1; eb += kWave) {
-      int64_t base;
-      int len;
-      T bv;
-      load_seg<T, !SYM>(eb + lane, b1, bpos, cpA, nullptr, 0, 0, 0, true, valB, base, len, bv);
-      const int inc = wave_incl_scan(len);
-      const int off = filled + inc - len;  // first element of this lane's segment
-      const int total = __shfl(inc, kWave - 1, kWave);
-      // lanes that hold B entries in this chunk, and the search's first step:
-      // the largest power of two below that count (all wave-uniform)
-      const int nb = (int)(b1 - eb < kWave ? b1 - eb : kWave);
-      int top = 0;
-      if (nb > 1) top = 1 << (31 - __clz(nb - 1));
-#pragma unroll
-      for (int u = 0; u < kE; ++u) {
-        // registers whose 64 elements miss this chunk's range stay as they are
-        if ((u + 1) * kWave <= filled || u * kWave >= filled + total) continue;
-        const int e = u * kWave + lane;
-        // last lane s < nb with off_s <= e (segments of length 0 share
-        // offsets; the search lands on the one that holds e when e is in this
-        // chunk)
-        int lo = 0;
-        for (int step = top; step > 0; step >>= 1) {
-          const int o = __shfl(off, lo + step, kWave);
-          if (lo + step < nb && o <= e) lo += step;
-        }
-        const int o = __shfl(off, lo, kWave);
-        const int l = __shfl(len, lo, kWave);
-        const int64_t bs = shfl_x(base, lo);
-        const T bval = shfl_x(bv, lo);
-        if (e >= filled && e < filled + total && e - o < l) {
-          const int64_t idx = bs + (e - o);
-          key[u] = rowsA[idx];
-          if constexpr (!SYM) val[u] = SR::mul(valA[idx], bval);
-        }
-      }
-      filled += total;
-    }
+    const int filled = tiny_gather<K, T, SYM>(b0, b1, bpos, valB, cpA, rowsA, valA, key, val);
     if (filled > kTinyMax && lane == 0) atomicOr(err, 4);
     // sort only as many registers as the column fills (wave-uniform)
-    if (filled <= kWave) tiny_finish<K, T, SYM, 1>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i);
+    if (filled <= kWave) tiny_finish<K, T, SYM, 1>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i, flt);
     else if (filled <= 2 * kWave)
-      tiny_finish<K, T, SYM, 2>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i);
-    else tiny_finish<K, T, SYM, 4>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i);
+      tiny_finish<K, T, SYM, 2>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i, flt);
+    else tiny_finish<K, T, SYM, 4>(key, val, lane, col, nnzcol, cpC, irC, valC, err, stage_off, i, flt);
+  }
+}
+
+// LDS form of the tiny tier: one wave per column, no sort.  The column's
+// products (gathered into registers as above) are inserted into a per-wave
+// linear-probing table of kTinyHash slots keyed by row (at most half full),
+// accumulating each product into its row's slot with the semiring's LDS
+// atomic; the occupied slots are compacted, and each distinct row's rank (its
+// position in the sorted column) is the number of distinct rows below it,
+// counted against the compacted rows read back four at a time (every lane
+// reads the same address).  Cross-lane traffic per column: a few dozen LDS
+// instructions against the several hundred lane shuffles of the register
+// bitonic sort.
+constexpr int kTinyHash = 2 * kTinyMax;  // slots per wave
+constexpr int kTinyWaves = 4;            // waves per workgroup at most (launch bound 256)
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int K, class T, bool SYM>
+__global__ __launch_bounds__(256) void k_tiny_h(const int32_t* __restrict__ order, int64_t first, int64_t ncols,
+                                                const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+                                                const T* __restrict__ valB, const int64_t* __restrict__ cpA,
+                                                const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
+                                                int64_t* __restrict__ nnzcol, const int64_t* __restrict__ cpC,
+                                                int64_t* __restrict__ irC, T* __restrict__ valC,
+                                                int* __restrict__ err, const int64_t* __restrict__ stage_off = nullptr,
+                                                Filter<T> flt = {}) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int kE = kTinyMax / kWave;
+  constexpr int kSPL = kTinyHash / kWave;  // table slots owned by a lane in the compaction
+  constexpr int32_t kNone = 0x7fffffff, kEmpty = -1;
+  constexpr int kLgHash = 9;
+  static_assert((1 << kLgHash) == kTinyHash, "table size");
+  __shared__ int32_t s_key[kTinyWaves][kTinyHash];
+  __shared__ Acc s_val[SYM ? 1 : kTinyWaves][SYM ? 1 : kTinyHash];
+  __shared__ __align__(16) int32_t s_row[kTinyWaves][kTinyMax + 4];  // compacted rows, kNone-padded
+  __shared__ uint16_t s_slot[SYM ? 1 : kTinyWaves][SYM ? 1 : kTinyMax];
+  const int lane = lane_id(), w = wave_id();
+  int32_t* hk = s_key[w];
+  int32_t* crow = s_row[w];
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + w; i < ncols; i += waves) {
+    const int col = order[first + i];
+    const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+    int32_t key[kE];
+    Acc val[kE];
+    const int filled = tiny_gather<K, T, SYM>(b0, b1, bpos, valB, cpA, rowsA, valA, key, val);
+    if (filled > kTinyMax && lane == 0) atomicOr(err, 4);
+    const int nr = (min(filled, kTinyMax) + kWave - 1) / kWave;  // registers in use (wave-uniform)
+#pragma unroll
+    for (int q = 0; q < kSPL; ++q) {
+      hk[q * kWave + lane] = kEmpty;
+      if constexpr (!SYM) s_val[w][q * kWave + lane] = SR::ident();
+    }
+    wave_lds_sync();
+    // insert: every product finds its row's slot (linear probing)
+    int slot[kE];
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      slot[u] = -1;
+      if (u < nr && key[u] != kNone) {
+        uint32_t h = hash_slot(key[u], kLgHash);
+        while (true) {
+          const int32_t old = atomicCAS(&hk[h], kEmpty, key[u]);
+          if (old == kEmpty || old == key[u]) break;
+          h = (h + 1) & (kTinyHash - 1);
+        }
+        slot[u] = (int)h;
+      }
+    }
+    if constexpr (!SYM) {
+#pragma unroll
+      for (int u = 0; u < kE; ++u)
+        if (slot[u] >= 0) SR::acc(&s_val[w][slot[u]], val[u]);
+    }
+    wave_lds_sync();
+    // compaction of the occupied slots (lane owns slots lane*kSPL ..)
+    int32_t kk[kSPL];
+    int occ = 0;
+#pragma unroll
+    for (int q = 0; q < kSPL; ++q) {
+      kk[q] = hk[lane * kSPL + q];
+      occ += kk[q] != kEmpty;
+    }
+    const int incl = wave_incl_scan(occ);
+    const int D = __shfl(incl, kWave - 1, kWave);  // distinct rows of the column
+    if constexpr (SYM) {
+      if (lane == 0) nnzcol[col] = D;
+    } else {
+      int pos = incl - occ;
+#pragma unroll
+      for (int q = 0; q < kSPL; ++q)
+        if (kk[q] != kEmpty) {
+          crow[pos] = kk[q];
+          s_slot[w][pos] = (uint16_t)(lane * kSPL + q);
+          ++pos;
+        }
+      if (lane < 4) crow[D + lane] = kNone;  // padding for the four-wide reads
+      wave_lds_sync();
+      int64_t have, out;
+      if (stage_off) {  // staged (no symbolic pass): its slot of flop_j entries; the count goes to nnzcol
+        out = stage_off[i];
+        have = stage_off[i + 1] - out;
+        if (lane == 0) nnzcol[col] = D;
+        if (lane == 0 && D > have) atomicOr(err, 8);
+      } else {
+        have = nnzcol[col];
+        if (lane == 0 && D != have) atomicOr(err, 8);
+        out = cpC[col];
+      }
+      // rank of every distinct row: the distinct rows below it
+      const int nc = (D + kWave - 1) / kWave;  // compacted entries per lane (wave-uniform)
+      int32_t mine[kE];
+      int rank[kE];
+#pragma unroll
+      for (int r = 0; r < kE; ++r) {
+        mine[r] = r < nc && r * kWave + lane < D ? crow[r * kWave + lane] : kNone;
+        rank[r] = 0;
+      }
+      for (int j = 0; j < D; j += 4) {
+        const int4 q4 = *reinterpret_cast<const int4*>(crow + j);
+#pragma unroll
+        for (int r = 0; r < kE; ++r)
+          if (r < nc) rank[r] += (q4.x < mine[r]) + (q4.y < mine[r]) + (q4.z < mine[r]) + (q4.w < mine[r]);
+      }
+      if (!flt.on) {
+#pragma unroll
+        for (int r = 0; r < kE; ++r) {
+          const int e = r * kWave + lane;
+          if (r < nc && e < D && rank[r] < have) {
+            irC[out + rank[r]] = mine[r];
+            valC[out + rank[r]] = SR::out(s_val[w][s_slot[w][e]]);
+          }
+        }
+      } else {  // the fused threshold: a kept row's place is the kept rows below it
+        T v[kE];
+        bool keep[kE];
+#pragma unroll
+        for (int r = 0; r < kE; ++r) {
+          const int e = r * kWave + lane;
+          keep[r] = false;
+          if (r < nc && e < D) {
+            v[r] = SR::out(s_val[w][s_slot[w][e]]);
+            keep[r] = !(v[r] < flt.thr);
+          }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < kE; ++r)  // kept rows replace the compacted rows (others: kNone)
+          if (r < nc && r * kWave + lane < D) crow[r * kWave + lane] = keep[r] ? mine[r] : kNone;
+        wave_lds_sync();
+        int kr[kE];
+#pragma unroll
+        for (int r = 0; r < kE; ++r) kr[r] = 0;
+        for (int j = 0; j < D; j += 4) {
+          const int4 q4 = *reinterpret_cast<const int4*>(crow + j);
+#pragma unroll
+          for (int r = 0; r < kE; ++r)
+            if (r < nc) kr[r] += (q4.x < mine[r]) + (q4.y < mine[r]) + (q4.z < mine[r]) + (q4.w < mine[r]);
+        }
+        int kept = 0;
+#pragma unroll
+        for (int r = 0; r < kE; ++r) {
+          kept += __popcll(__ballot(keep[r]));
+          if (keep[r] && kr[r] < have) {
+            irC[out + kr[r]] = mine[r];
+            valC[out + kr[r]] = v[r];
+          }
+        }
+        if (lane == 0) {
+          if (stage_off) {
+            nnzcol[col] = kept;
+            if (D > kept) atomicAdd(flt.dropped, (unsigned long long)(D - kept));
+          } else {
+            filter_add_run(flt, out, kept, col, err);
+          }
+        }
+      }
+    }
+    wave_lds_sync();  // the next column reuses the wave's table
   }
 }
 
@@ -561,7 +827,8 @@ template <class T>
 __global__ void k_tiny_place(const int32_t* __restrict__ order, int64_t first, int64_t ncols,
                              const int64_t* __restrict__ stage_off, const int64_t* __restrict__ nnzcol,
                              const int64_t* __restrict__ cpC, const int64_t* __restrict__ sir,
-                             const T* __restrict__ sval, int64_t* __restrict__ irC, T* __restrict__ valC) {
+                             const T* __restrict__ sval, int64_t* __restrict__ irC, T* __restrict__ valC,
+                             Filter<T> flt = {}, int* __restrict__ err = nullptr) {
   const int64_t i = global_wave();
   if (i >= ncols) return;
   const int col = order[first + i];
@@ -570,6 +837,7 @@ __global__ void k_tiny_place(const int32_t* __restrict__ order, int64_t first, i
     irC[dst + e] = sir[src + e];
     valC[dst + e] = sval[src + e];
   }
+  if (flt.on && lane_id() == 0) filter_add_run(flt, dst, (int)cnt, col, err);  // staged entries are the kept ones
 }
 
 // -------------------------------------------------------- symbolic: dense
@@ -634,7 +902,7 @@ __global__ __launch_bounds__(NT) void k_num_hash(const int32_t* __restrict__ ord
                                                  const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                                                  const T* __restrict__ valA, const int64_t* __restrict__ cpC,
                                                  int64_t* __restrict__ irC, T* __restrict__ valC,
-                                                 int* __restrict__ err) {
+                                                 int* __restrict__ err, Filter<T> flt = {}) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   __shared__ int32_t keys[HMAX];
@@ -729,10 +997,27 @@ __global__ __launch_bounds__(NT) void k_num_hash(const int32_t* __restrict__ ord
     }
   }
   const int64_t out = cpC[col];
-  for (int i = threadIdx.x; i < cnt; i += NT) {
-    irC[out + i] = keys[i];
-    valC[out + i] = SR::out(acc[i]);
+  if (!flt.on) {
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+      irC[out + i] = keys[i];
+      valC[out + i] = SR::out(acc[i]);
+    }
+    return;
   }
+  int kept = 0;  // the fused threshold: the kept entries, in order, at out
+  for (int c0 = 0; c0 < cnt; c0 += NT) {
+    const int i = c0 + threadIdx.x;
+    const T v = i < cnt ? SR::out(acc[i]) : T{};
+    const bool keep = i < cnt && !(v < flt.thr);
+    long long tot;
+    const int ex = (int)block_excl_scan_ll<NT>(keep ? 1 : 0, wsum, tot);
+    if (keep) {
+      irC[out + kept + ex] = keys[i];
+      valC[out + kept + ex] = v;
+    }
+    kept += (int)tot;
+  }
+  if (threadIdx.x == 0) filter_add_run(flt, out, kept, col, err);
 }
 
 // --------------------------------------------------------- numeric: dense
@@ -811,6 +1096,14 @@ __global__ __launch_bounds__(NT) void k_num_dense(const int32_t* __restrict__ or
 // built IBLK blocks at a time (each sub-pass gathers only the products in its
 // rows, through the tile table P), so the LDS bitmap stays small enough for
 // several workgroups per CU.  Writes nnz_j and the column's units.
+//
+// With `lists`, the column's sorted output rows are also written, as 12-bit
+// offsets within their 4096-row blocks, into the column's region
+// lists[list_off[slot] .. list_off[slot + 1]) (slot = the workgroup), and its
+// per-block counts into bcnt16[slot * S ..].  The small units of a pass whose
+// rows all fit the region are then cut by output count alone, over any
+// number of blocks: those wider than rblk_l blocks go to units_s, whose
+// numeric pass takes ranks from the list instead of a bitmap.
 template <int NT, int U, int IBLK>
 __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ order, int64_t first,
                                                   const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
@@ -820,24 +1113,39 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
                                                   int64_t* __restrict__ nnzcol, Unit* __restrict__ units_l,
                                                   int* __restrict__ n_l, int cap_l, Unit* __restrict__ units_h,
                                                   int* __restrict__ n_h, int cap_h, uint32_t* __restrict__ rmap,
-                                                  int64_t rm_cols, int* __restrict__ err) {
+                                                  int64_t rm_cols, int* __restrict__ err,
+                                                  uint16_t* __restrict__ lists = nullptr,
+                                                  const int64_t* __restrict__ list_off = nullptr,
+                                                  uint16_t* __restrict__ bcnt16 = nullptr,
+                                                  Unit* __restrict__ units_s = nullptr, int* __restrict__ n_s = nullptr,
+                                                  int cap_s = 0) {
   constexpr int BPP = 256;             // blocks per cutting pass (2^20 rows)
   constexpr int BMW = IBLK * kBRW;     // bitmap words of one sub-pass
   static_assert(BPP % IBLK == 0, "sub-passes tile the cutting pass");
+  static_assert(IBLK <= kWave, "a wave scans the block counts of a sub-pass");
+  static_assert(sizeof(SegStage<uint8_t, NT>) >= BPP * sizeof(Unit), "list units are staged in the gather stage");
   __shared__ uint32_t bits[BMW];
   __shared__ int bpre[BPP];           // block counts, then their inclusive prefix
   __shared__ Unit su[BPP];
   __shared__ SegStage<uint8_t, NT> st;
   __shared__ long long wsum[NT / kWave];
-  __shared__ int s_nu, s_slot;
+  __shared__ int s_nu, s_slot, s_ns, s_sslot;
+  Unit* su2 = reinterpret_cast<Unit*>(&st);  // list units of a pass (the stage is idle while cutting)
   const int col = order[first + blockIdx.x];
   const int64_t b0 = cpB[col], b1 = cpB[col + 1];
   const int npass = (S + BPP - 1) / BPP;
   const bool whole = S <= IBLK;  // one sub-pass covers every row: no tile look-ups
+  int64_t list_base = 0, region = 0;
+  if (lists) {
+    list_base = list_off[blockIdx.x];
+    region = list_off[blockIdx.x + 1] - list_base;
+  }
   int64_t colcnt = 0;
   int unit_off = 0;  // meaningful in thread 0 only
   for (int sp = 0; sp < npass; ++sp) {
     const int t_lo = sp * BPP, t_hi = min(S, t_lo + BPP), nblk = t_hi - t_lo;
+    bool listok = lists != nullptr;  // every row of this pass written so far (uniform)
+    int64_t run = 0;                 // this pass's rows before the current sub-pass (uniform)
     for (int i_lo = t_lo; i_lo < t_hi; i_lo += IBLK) {
       const int i_hi = min(t_hi, i_lo + IBLK), inb = i_hi - i_lo;
       const int32_t row0 = i_lo * kBR;
@@ -874,9 +1182,42 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
         const uint32_t* w = bits + b * kBRW;
         int c = __popc(w[lane_id()]) + __popc(w[lane_id() + kWave]);
         c = (int)wave_sum_ll(c);
-        if (lane_id() == 0) bpre[i_lo - t_lo + b] = c;
+        if (lane_id() == 0) {
+          bpre[i_lo - t_lo + b] = c;
+          if (lists) bcnt16[(int64_t)blockIdx.x * S + i_lo + b] = (uint16_t)c;
+        }
       }
       __syncthreads();  // the counts are read, and the next sub-pass clears the bitmap
+      if (lists) {
+        // the sub-pass's rows in order, one wave per block as for the counts
+        // (two words, 64 rows, per lane): every wave scans the sub-pass's
+        // block counts (IBLK <= 64, one per lane) for the block offsets and
+        // the total, and its lanes place their rows by a wave scan, so the
+        // stores of a wave cover one contiguous range
+        const int bc = lane_id() < inb ? bpre[i_lo - t_lo + lane_id()] : 0;
+        const int bincl = wave_incl_scan(bc);
+        const int sub_total = __shfl(bincl, kWave - 1, kWave);
+        const int64_t have = colcnt + run;
+        listok = listok && have + sub_total <= region;
+        if (listok) {
+          for (int b = wave_id(); b < inb; b += NT / kWave) {
+            const int boff = __shfl(bincl - bc, b, kWave);
+            const uint32_t* w = bits + b * kBRW;
+            uint64_t x = (uint64_t)w[2 * lane_id()] | ((uint64_t)w[2 * lane_id() + 1] << 32);
+            const int c = __popcll(x);
+            const int ex = wave_incl_scan(c) - c;
+            uint16_t* dst = lists + list_base + have + boff + ex;
+            int k = 0;
+            while (x) {
+              const int bit = __ffsll((unsigned long long)x) - 1;
+              x &= x - 1;
+              dst[k++] = (uint16_t)(lane_id() * 64 + bit);
+            }
+          }
+        }
+        run += sub_total;
+        __syncthreads();  // the bitmap is read before the next sub-pass clears it
+      }
     }
     if (wave_id() == 0) {  // inclusive prefix over blocks
       int carry = 0;
@@ -891,13 +1232,15 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
     __syncthreads();
     const int pass_total = bpre[nblk - 1];
     if (threadIdx.x == 0) {
-      // greedy cut into units (a single block never exceeds kBR <= kmax)
+      // greedy cut into units (a single block never exceeds kBR <= kmax);
+      // a small-unit pass with its list written is cut by count alone
       const bool heavy = pass_total > heavy_thr;
+      const bool by_list = !heavy && listok;
       const int kmax = heavy ? kmax_h : kmax_l, rblk = heavy ? rblk_h : rblk_l;
-      int start = 0, before = 0, nu = 0;
+      int start = 0, before = 0, nu = 0, ns = 0;
       while (start < nblk) {
         const int lim = before + kmax;
-        const int endmax = min(nblk, start + rblk);
+        const int endmax = by_list ? nblk : min(nblk, start + rblk);
         int lo = start, hi = endmax - 1;  // last b in [start, endmax) with bpre[b] <= lim
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
@@ -910,13 +1253,147 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
         if (lo - start + 1 > dl && bpre[start + dl - 1] - before >= kmax / 2) lo = start + dl - 1;
         const int c = bpre[lo] - before;
         if (c > 0) {
+          Unit& u = (by_list && lo + 1 - start > rblk) ? su2[ns++] : su[nu++];
+          u.col = col;
+          u.blk_lo = t_lo + start;
+          u.blk_hi = t_lo + lo + 1;
+          u.cnt = c;
+          u.off = unit_off;
+          u.slot = (int32_t)blockIdx.x;
+          unit_off += c;
+        }
+        before = bpre[lo];
+        start = lo + 1;
+      }
+      s_nu = nu;
+      s_ns = ns;
+      int* cnt = heavy ? n_h : n_l;
+      s_slot = nu ? atomicAdd(cnt, nu) : 0;
+      if (s_slot + nu > (heavy ? cap_h : cap_l)) atomicOr(err, 32);
+      s_sslot = ns ? atomicAdd(n_s, ns) : 0;
+      if (ns && s_sslot + ns > cap_s) atomicOr(err, 32);
+    }
+    __syncthreads();
+    {
+      const bool heavy = pass_total > heavy_thr;
+      Unit* dst = heavy ? units_h : units_l;
+      const int cap = heavy ? cap_h : cap_l;
+      for (int i = threadIdx.x; i < s_nu; i += NT)
+        if (s_slot + i < cap) dst[s_slot + i] = su[i];
+      for (int i = threadIdx.x; i < s_ns; i += NT)
+        if (s_sslot + i < cap_s) units_s[s_sslot + i] = su2[i];
+    }
+    colcnt += pass_total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nnzcol[col] = colcnt;
+}
+
+// Banded form of the same symbolic pass, in two kernels.  k_sym_band runs one
+// workgroup per (column, band of IBLK row blocks) with the band as the slow
+// grid dimension, so the workgroups in flight at any moment gather only the
+// A entries of one band of rows (IBLK * 4096 rows: about nnz(A) * IBLK / S
+// entries, which the L2 caches can hold), where the per-column pass sweeps
+// every band of a few columns.  It builds the band's presence bitmap, keeps
+// the row-map slice, and writes the band's per-block counts.  k_sym_cut then
+// cuts every column into units from its block counts, as k_sym_units does.
+template <int NT, int U, int IBLK>
+__global__ __launch_bounds__(NT) void k_sym_band(const int32_t* __restrict__ order, int64_t first,
+                                                 const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
+                                                 const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+                                                 const int32_t* __restrict__ P, int S, int32_t* __restrict__ bcnt,
+                                                 uint32_t* __restrict__ rmap, int64_t rm_cols) {
+  constexpr int BMW = IBLK * kBRW;  // bitmap words of one band
+  __shared__ uint32_t bits[BMW];
+  __shared__ SegStage<uint8_t, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  const int64_t slot = blockIdx.x;  // the column's position in the launch
+  const int col = order[first + slot];
+  const int i_lo = (int)blockIdx.y * IBLK, i_hi = min(S, i_lo + IBLK), inb = i_hi - i_lo;
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  const int32_t row0 = i_lo * kBR;
+  const bool whole = S <= IBLK;  // one band covers every row: no tile look-ups
+  for (int i = threadIdx.x; i < inb * kBRW; i += NT) bits[i] = 0u;
+  // (the staging barrier below orders the clear before any insertion)
+  for (int64_t eb = b0; eb < b1; eb += NT) {
+    int64_t base;
+    int len;
+    uint8_t dummy;
+    load_seg<uint8_t, false>(eb + threadIdx.x, b1, bpos, cpA, P, S, i_lo, i_hi, whole, nullptr, base, len, dummy);
+    int nseg;
+    const int tot = stage_chunk<uint8_t, NT>(st, wsum, base, len, 0, nseg);
+    visit_staged<uint8_t, NT, U>(st, nseg, tot, [&](const int64_t(&idx)[U], const uint8_t(&)[U]) {
+      int32_t r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) r[u] = idx[u] >= 0 ? rowsA[idx[u]] - row0 : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (r[u] >= 0) atomicOr(&bits[r[u] >> 5], 1u << (r[u] & 31));
+    });
+  }
+  __syncthreads();
+  if (rmap && slot < rm_cols) {
+    uint32_t* dst = rmap + slot * S * kBRW + (int64_t)i_lo * kBRW;
+    for (int i = threadIdx.x; i < inb * kBRW; i += NT) dst[i] = bits[i];
+  }
+  for (int b = wave_id(); b < inb; b += NT / kWave) {  // one wave per block, two words per lane
+    const uint32_t* wv = bits + b * kBRW;
+    int c = __popc(wv[lane_id()]) + __popc(wv[lane_id() + kWave]);
+    c = (int)wave_sum_ll(c);
+    if (lane_id() == 0) bcnt[slot * S + i_lo + b] = c;
+  }
+}
+
+// Unit cutting of the banded symbolic pass: one workgroup of 256 threads per
+// column, over passes of 256 blocks, with the greedy rule of k_sym_units.
+__global__ __launch_bounds__(256) void k_sym_cut(const int32_t* __restrict__ order, int64_t first,
+                                                 const int32_t* __restrict__ bcnt, int S, int kmax_l, int rblk_l,
+                                                 int kmax_h, int rblk_h, int64_t heavy_thr,
+                                                 int64_t* __restrict__ nnzcol, Unit* __restrict__ units_l,
+                                                 int* __restrict__ n_l, int cap_l, Unit* __restrict__ units_h,
+                                                 int* __restrict__ n_h, int cap_h, int* __restrict__ err) {
+  constexpr int BPP = 256;
+  __shared__ int bpre[BPP];
+  __shared__ Unit su[BPP];
+  __shared__ long long wsum[256 / kWave];
+  __shared__ int s_nu, s_slot;
+  const int64_t slot = blockIdx.x;
+  const int col = order[first + slot];
+  const int npass = (S + BPP - 1) / BPP;
+  int64_t colcnt = 0;
+  int unit_off = 0;  // meaningful in thread 0 only
+  for (int sp = 0; sp < npass; ++sp) {
+    const int t_lo = sp * BPP, nblk = min(S, t_lo + BPP) - t_lo;
+    const int c = (int)threadIdx.x < nblk ? bcnt[slot * S + t_lo + threadIdx.x] : 0;
+    long long total;
+    const long long ex = block_excl_scan_ll<256>(c, wsum, total);
+    bpre[threadIdx.x] = (int)(ex + c);  // inclusive prefix over the pass's blocks
+    __syncthreads();
+    const int pass_total = (int)total;
+    if (threadIdx.x == 0) {
+      const bool heavy = pass_total > heavy_thr;
+      const int kmax = heavy ? kmax_h : kmax_l, rblk = heavy ? rblk_h : rblk_l;
+      int start = 0, before = 0, nu = 0;
+      while (start < nblk) {
+        const int lim = before + kmax;
+        const int endmax = min(nblk, start + rblk);
+        int lo = start, hi = endmax - 1;  // last b in [start, endmax) with bpre[b] <= lim
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (bpre[mid] <= lim) lo = mid;
+          else hi = mid - 1;
+        }
+        const int dl = kmax / kBR;
+        if (lo - start + 1 > dl && bpre[start + dl - 1] - before >= kmax / 2) lo = start + dl - 1;
+        const int cu = bpre[lo] - before;
+        if (cu > 0) {
           su[nu].col = col;
           su[nu].blk_lo = t_lo + start;
           su[nu].blk_hi = t_lo + lo + 1;
-          su[nu].cnt = c;
+          su[nu].cnt = cu;
           su[nu].off = unit_off;
-          su[nu].slot = (int32_t)blockIdx.x;
-          unit_off += c;
+          su[nu].slot = (int32_t)slot;
+          unit_off += cu;
           ++nu;
         }
         before = bpre[lo];
@@ -932,13 +1409,115 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
       const bool heavy = pass_total > heavy_thr;
       Unit* dst = heavy ? units_h : units_l;
       const int cap = heavy ? cap_h : cap_l;
-      for (int i = threadIdx.x; i < s_nu; i += NT)
+      for (int i = threadIdx.x; i < s_nu; i += 256)
         if (s_slot + i < cap) dst[s_slot + i] = su[i];
     }
     colcnt += pass_total;
     __syncthreads();
   }
   if (threadIdx.x == 0) nnzcol[col] = colcnt;
+}
+
+// Emission of a unit's cnt outputs at `out`, rows ascending.  direct: the
+// value of window row r sits at vals[r] (rows [0, ninit), presence in bits);
+// otherwise at vals[rank], with the rank of a row given by its bitmap word's
+// prefix count pre[] and the rows staged in rank order in the value array's
+// LDS after the values have left (every Acc is at least 4 bytes), so both
+// leave in full, coalesced lines.  With the filter on, only entries with
+// !(v < thr) are written, compacted at `out`, and the unit's range is
+// appended to the run list; `scratch` (LDS no longer needed by the caller,
+// cnt bits) holds the kept flags between the value and row passes.
+template <int K, class T, int NT>
+__device__ __forceinline__ void unit_emit(bool direct, int ninit, int cnt, int nw64, const uint32_t* bits,
+                                          const uint16_t* pre, typename Sem<K, T>::Acc* vals, int32_t row0,
+                                          int64_t out, int64_t* __restrict__ irC, T* __restrict__ valC,
+                                          const Filter<T>& flt, int col, uint32_t* scratch, long long* wsum,
+                                          int* err) {
+  using SR = Sem<K, T>;
+  auto rank_of = [&](int32_t r) {
+    const int w = r >> 6, b = r & 63;
+    const uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+    return (int)pre[w] + __popcll(word & ((1ull << b) - 1ull));
+  };
+  auto present = [&](int r) { return ((bits[r >> 5] >> (r & 31)) & 1u) != 0; };
+  if (direct) {
+    if (!flt.on) {  // one thread per row, stores stay coalesced
+      for (int r = threadIdx.x; r < ninit; r += NT) {
+        if (!present(r)) continue;
+        const int q = rank_of(r);
+        if (q < cnt) {
+          irC[out + q] = (int64_t)row0 + r;
+          valC[out + q] = SR::out(vals[r]);
+        }
+      }
+      return;
+    }
+    int kept = 0;
+    for (int c0 = 0; c0 < ninit; c0 += NT) {
+      const int r = c0 + threadIdx.x;
+      T v{};
+      bool keep = false;
+      if (r < ninit && present(r)) {
+        v = SR::out(vals[r]);
+        keep = !(v < flt.thr);
+      }
+      long long tot;
+      const int ex = (int)block_excl_scan_ll<NT>(keep ? 1 : 0, wsum, tot);
+      if (keep && kept + ex < cnt) {
+        irC[out + kept + ex] = (int64_t)row0 + r;
+        valC[out + kept + ex] = v;
+      }
+      kept += (int)tot;
+    }
+    if (threadIdx.x == 0) filter_add_run(flt, out, min(kept, cnt), col, err);
+    return;
+  }
+  int kept = 0;
+  if (!flt.on) {
+    for (int i = threadIdx.x; i < cnt; i += NT) valC[out + i] = SR::out(vals[i]);
+  } else {
+    for (int i = threadIdx.x; i <= cnt / 32; i += NT) scratch[i] = 0u;
+    __syncthreads();
+    for (int c0 = 0; c0 < cnt; c0 += NT) {
+      const int i = c0 + threadIdx.x;
+      const T v = i < cnt ? SR::out(vals[i]) : T{};
+      const bool keep = i < cnt && !(v < flt.thr);
+      long long tot;
+      const int ex = (int)block_excl_scan_ll<NT>(keep ? 1 : 0, wsum, tot);
+      if (keep) {
+        valC[out + kept + ex] = v;
+        atomicOr(&scratch[i >> 5], 1u << (i & 31));
+      }
+      kept += (int)tot;
+    }
+  }
+  __syncthreads();
+  uint32_t* rstage = reinterpret_cast<uint32_t*>(vals);
+  for (int w = threadIdx.x; w < nw64; w += NT) {
+    uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
+    int q = pre[w];
+    while (word) {
+      const int b = __ffsll((unsigned long long)word) - 1;
+      word &= word - 1;
+      if (q < cnt) rstage[q] = (uint32_t)(w * 64 + b);
+      ++q;
+    }
+  }
+  __syncthreads();
+  if (!flt.on) {
+    for (int i = threadIdx.x; i < cnt; i += NT) irC[out + i] = (int64_t)row0 + rstage[i];
+    return;
+  }
+  int kept2 = 0;
+  for (int c0 = 0; c0 < cnt; c0 += NT) {
+    const int i = c0 + threadIdx.x;
+    const bool keep = i < cnt && ((scratch[i >> 5] >> (i & 31)) & 1u);
+    long long tot;
+    const int ex = (int)block_excl_scan_ll<NT>(keep ? 1 : 0, wsum, tot);
+    if (keep) irC[out + kept2 + ex] = (int64_t)row0 + rstage[i];
+    kept2 += (int)tot;
+  }
+  if (threadIdx.x == 0) filter_add_run(flt, out, kept, col, err);
 }
 
 // Numeric pass of the unit tier: one workgroup per unit.  Sweep 1 sets the
@@ -952,7 +1531,7 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
                                                   const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                                                   const T* __restrict__ valA, const int32_t* __restrict__ P, int S,
                                                   const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
-                                                  T* __restrict__ valC, int* __restrict__ err) {
+                                                  T* __restrict__ valC, int* __restrict__ err, Filter<T> flt = {}) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int NW32 = RBLK * kBRW;   // bitmap words
@@ -1101,15 +1680,9 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
     if (threadIdx.x == 0 && total != cnt) atomicOr(err, 128);
   }
   __syncthreads();
-  if (direct) {  // values already sit at their rows: one thread per row, stores stay coalesced
-    for (int r = threadIdx.x; r < ninit; r += NT) {
-      if (!((bits[r >> 5] >> (r & 31)) & 1u)) continue;
-      const int q = rank_of(r);
-      if (q < cnt) {
-        irC[out + q] = (int64_t)row0 + r;
-        valC[out + q] = SR::out(vals[r]);
-      }
-    }
+  if (direct) {  // values already sit at their rows
+    unit_emit<K, T, NT>(true, ninit, cnt, nw64, bits, pre, vals, row0, out, irC, valC, flt, col,
+                        reinterpret_cast<uint32_t*>(&st), wsum, err);
     return;
   }
   // sweep 2: accumulate values at their ranks
@@ -1135,24 +1708,8 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
     }
   }
   __syncthreads();
-  // emit: values in rank order (coalesced); then the rows, staged in rank
-  // order in the value array's LDS (every Acc is at least 4 bytes) so that
-  // they too leave in full, coalesced lines
-  for (int i = threadIdx.x; i < cnt; i += NT) valC[out + i] = SR::out(vals[i]);
-  __syncthreads();
-  uint32_t* rstage = reinterpret_cast<uint32_t*>(vals);
-  for (int w = threadIdx.x; w < nw64; w += NT) {
-    uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
-    int q = pre[w];
-    while (word) {
-      const int b = __ffsll((unsigned long long)word) - 1;
-      word &= word - 1;
-      if (q < cnt) rstage[q] = (uint32_t)(w * 64 + b);
-      ++q;
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < cnt; i += NT) irC[out + i] = (int64_t)row0 + rstage[i];
+  unit_emit<K, T, NT>(false, ninit, cnt, nw64, bits, pre, vals, row0, out, irC, valC, flt, col,
+                      reinterpret_cast<uint32_t*>(&st), wsum, err);
 }
 
 // Numeric pass of the unit tier with the symbolic pass's row map: the unit's
@@ -1167,7 +1724,8 @@ __global__ __launch_bounds__(NT) void k_num_units_rm(const Unit* __restrict__ un
                                                      const int32_t* __restrict__ rowsA, const T* __restrict__ valA,
                                                      const int32_t* __restrict__ P, int S, int64_t rm_cols,
                                                      const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
-                                                     T* __restrict__ valC, int* __restrict__ err) {
+                                                     T* __restrict__ valC, int* __restrict__ err,
+                                                     Filter<T> flt = {}) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int NW32 = RBLK * kBRW;
@@ -1250,34 +1808,134 @@ __global__ __launch_bounds__(NT) void k_num_units_rm(const Unit* __restrict__ un
     visit_staged<T, NT, U>(st, nseg, tot, accumulate);
   }
   __syncthreads();
-  if (direct) {  // values sit at their rows: one thread per row, stores stay coalesced
-    for (int r = threadIdx.x; r < ninit; r += NT) {
-      if (!((bits[r >> 5] >> (r & 31)) & 1u)) continue;
-      const int q = rank_of(r);
-      if (q < cnt) {
-        irC[out + q] = (int64_t)row0 + r;
-        valC[out + q] = SR::out(vals[r]);
+  unit_emit<K, T, NT>(direct, ninit, cnt, nw64, bits, pre, vals, row0, out, irC, valC, flt, col,
+                      reinterpret_cast<uint32_t*>(&st), wsum, err);
+}
+
+// Numeric pass of the list units: at most KMAX outputs of one column over
+// any number of the row blocks of one 256-block pass.  Their ranks come from
+// the column's sorted row list that the symbolic pass wrote (12-bit offsets
+// grouped by block, with per-block counts), so one sweep gathers every
+// product once: a product's rank is its block's first list position plus a
+// binary search among that block's offsets.  Units this sparse would need a
+// bitmap of up to 256 blocks (128 KB), or many narrow units that each
+// re-read the B column and gather one- or two-entry A segments.
+template <int K, class T, int NT, int U, int KMAX>
+__global__ __launch_bounds__(NT) void k_num_list(const Unit* __restrict__ units, const uint16_t* __restrict__ lists,
+                                                 const int64_t* __restrict__ list_off,
+                                                 const uint16_t* __restrict__ bcnt16, const int64_t* __restrict__ cpB,
+                                                 const int32_t* __restrict__ bpos, const T* __restrict__ valB,
+                                                 const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
+                                                 const T* __restrict__ valA, const int32_t* __restrict__ P, int S,
+                                                 const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
+                                                 T* __restrict__ valC, int* __restrict__ err, Filter<T> flt = {}) {
+  using SR = Sem<K, T>;
+  using Acc = typename SR::Acc;
+  constexpr int BPP = 256;  // blocks of one symbolic cutting pass
+  static_assert(NT > BPP, "one block count per thread, plus the total");
+  __shared__ Acc vals[KMAX];
+  __shared__ uint16_t lst[KMAX];
+  __shared__ int dir[BPP + 1];  // list position of each block's first row; dir[nb] = cnt
+  __shared__ SegStage<T, NT> st;
+  __shared__ long long wsum[NT / kWave];
+  const Unit un = units[blockIdx.x];
+  const int col = un.col;
+  const int t_lo = un.blk_lo, t_hi = un.blk_hi, nb = t_hi - t_lo;
+  const int32_t row0 = t_lo * kBR;
+  const int cnt = un.cnt;
+  const int64_t out = cpC[col] + un.off;
+  const int64_t b0 = cpB[col], b1 = cpB[col + 1];
+  if (cnt > KMAX || nb <= 0 || nb > BPP) {  // units are cut to fit; guard anyway
+    if (threadIdx.x == 0) atomicOr(err, 64);
+    return;
+  }
+  const uint16_t* src = lists + list_off[un.slot] + un.off;
+  for (int i = threadIdx.x; i < cnt; i += NT) {
+    lst[i] = src[i];
+    vals[i] = SR::ident();
+  }
+  {
+    const int c = threadIdx.x < nb ? (int)bcnt16[(int64_t)un.slot * S + t_lo + threadIdx.x] : 0;
+    long long tot;
+    const int ex = (int)block_excl_scan_ll<NT>(c, wsum, tot);  // barriers: lst and vals are ready after it
+    if (tot != cnt) {  // the counts and the unit disagree (uniform): positions would leave the list
+      if (threadIdx.x == 0) atomicOr(err, 128);
+      return;
+    }
+    if (threadIdx.x <= nb) dir[threadIdx.x] = ex;
+  }
+  __syncthreads();
+  auto accumulate = [&](const int64_t(&idx)[U], const T(&bvs)[U]) {
+    int32_t r[U];
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      r[u] = -1;
+      if (idx[u] >= 0) {
+        r[u] = rowsA[idx[u]] - row0;
+        a[u] = valA[idx[u]];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r[u] < 0) continue;
+      const int b = r[u] >> 12;  // kBR = 4096 rows per block
+      const uint16_t o = (uint16_t)(r[u] & (kBR - 1));
+      const int end = dir[b + 1];
+      int lo = dir[b], hi = end;  // first position in the block with offset >= o
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (lst[mid] < o) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < end && lst[lo] == o) SR::acc(&vals[lo], SR::mul(a[u], bvs[u]));
+      else atomicOr(err, 4);  // symbolic and numeric agree; guard anyway
+    }
+  };
+  for (int64_t eb = b0; eb < b1; eb += NT) {
+    int64_t base;
+    int len;
+    T bv;
+    load_seg<T, true>(eb + threadIdx.x, b1, bpos, cpA, P, S, t_lo, t_hi, false, valB, base, len, bv);
+    int nseg;
+    const int tot = stage_chunk<T, NT>(st, wsum, base, len, bv, nseg);
+    visit_staged<T, NT, U>(st, nseg, tot, accumulate);
+  }
+  __syncthreads();
+  auto row_of = [&](int i) {  // the block holding list position i: largest b with dir[b] <= i
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (dir[mid] <= i) lo = mid;
+      else hi = mid;
+    }
+    return (int64_t)row0 + (int64_t)lo * kBR + lst[i];
+  };
+  if (!flt.on) {
+    for (int i = threadIdx.x; i < cnt; i += NT) {
+      irC[out + i] = row_of(i);
+      valC[out + i] = SR::out(vals[i]);
     }
     return;
   }
-  // values in rank order (coalesced); then the rows, staged in rank order in
-  // the value array's LDS (every Acc is at least 4 bytes)
-  for (int i = threadIdx.x; i < cnt; i += NT) valC[out + i] = SR::out(vals[i]);
-  __syncthreads();
-  uint32_t* rstage = reinterpret_cast<uint32_t*>(vals);
-  for (int w = threadIdx.x; w < nw64; w += NT) {
-    uint64_t word = (uint64_t)bits[2 * w] | ((uint64_t)bits[2 * w + 1] << 32);
-    int q = pre[w];
-    while (word) {
-      const int b = __ffsll((unsigned long long)word) - 1;
-      word &= word - 1;
-      if (q < cnt) rstage[q] = (uint32_t)(w * 64 + b);
-      ++q;
+  int kept = 0;
+  for (int c0 = 0; c0 < cnt; c0 += NT) {
+    const int i = c0 + threadIdx.x;
+    T v{};
+    bool keep = false;
+    if (i < cnt) {
+      v = SR::out(vals[i]);
+      keep = !(v < flt.thr);
     }
+    long long tot;
+    const int ex = (int)block_excl_scan_ll<NT>(keep ? 1 : 0, wsum, tot);
+    if (keep) {
+      irC[out + kept + ex] = row_of(i);
+      valC[out + kept + ex] = v;
+    }
+    kept += (int)tot;
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < cnt; i += NT) irC[out + i] = (int64_t)row0 + rstage[i];
+  if (threadIdx.x == 0) filter_add_run(flt, out, kept, col, err);
 }
 
 // Predictor features per stored column of B (one wave per column); see
@@ -1800,10 +2458,9 @@ namespace {
 
 struct DevBuf {
   void* p = nullptr;
+  size_t bytes = 0;
   hipStream_t s = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFreeAsync(p, s);
-  }
+  ~DevBuf() { scratch_free(p, bytes, s); }
 };
 
 // Large per-call arrays whose size repeats from product to product (the unit
@@ -1849,11 +2506,11 @@ cbx_status calloc_cached(CachedBuf& b, size_t count, hipStream_t s, X** out) {
 template <class X>
 cbx_status dalloc(DevBuf& b, size_t count, hipStream_t s, X** out) {
   b.s = s;
-  size_t bytes = std::max<size_t>(count * sizeof(X), 256);
-  hipError_t e = hipMallocAsync(&b.p, bytes, s);
-  if (e != hipSuccess) {
+  b.bytes = std::max<size_t>(count * sizeof(X), 256);
+  const cbx_status st = scratch_alloc(&b.p, b.bytes, s);
+  if (st != CBX_OK) {
     b.p = nullptr;
-    return set_error(CBX_ERR_NOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return st;
   }
   *out = (X*)b.p;
   return CBX_OK;
@@ -1866,6 +2523,47 @@ cbx_status dalloc(DevBuf& b, size_t count, hipStream_t s, X** out) {
   } while (0)
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Unit order for the numeric pass: by first row block, so that the units in
+// flight at any moment gather the A entries of a narrow band of rows (a few
+// hundred KB of A), which the XCDs' L2 caches can hold, instead of every row
+// range of a few heavy columns.  Units are independent (their output offsets
+// are known), so the order changes nothing but the cache footprint.
+__global__ void k_unit_keys(const Unit* __restrict__ u, int n, int32_t* __restrict__ key, int32_t* __restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    key[i] = u[i].blk_lo;
+    idx[i] = i;
+  }
+}
+
+__global__ void k_unit_permute(const Unit* __restrict__ src, const int32_t* __restrict__ idx, int n,
+                               Unit* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// A copy of units[0, n) sorted by blk_lo (stable), kept in `keep`.
+cbx_status sort_units_by_block(const Unit* units, int n, int S, hipStream_t s, DevBuf& keep, Unit** out) {
+  DevBuf b_k, b_k2, b_i, b_i2, b_t;
+  int32_t *k1, *k2, *i1, *i2;
+  uint8_t* t;
+  CBX_TRY(dalloc(b_k, n, s, &k1));
+  CBX_TRY(dalloc(b_k2, n, s, &k2));
+  CBX_TRY(dalloc(b_i, n, s, &i1));
+  CBX_TRY(dalloc(b_i2, n, s, &i2));
+  CBX_TRY(dalloc(keep, n, s, out));
+  int bits = 1;
+  while ((1 << bits) <= S) ++bits;
+  size_t tb = 0;
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k2, i1, i2, n, 0, bits, s));
+  CBX_TRY(dalloc(b_t, std::max<size_t>(tb, 16), s, &t));
+  k_unit_keys<<<nblk(n, 256), 256, 0, s>>>(units, n, k1, i1);
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, k1, k2, i1, i2, n, 0, bits, s));
+  k_unit_permute<<<nblk(n, 256), 256, 0, s>>>(units, i2, n, *out);
+  CBX_HIP(hipGetLastError());
+  return CBX_OK;
+}
 
 struct Timer {
   hipEvent_t ev[6];
@@ -1895,6 +2593,7 @@ constexpr int kSymNT = 512;
 constexpr int kSymRBW = 32768;                // bitmap words per symbolic pass (2^20 rows)
 // unit tier
 constexpr int kSymUnitNT = 512, kSymUnitIBLK = 64;  // 32 KB bitmap: three workgroups per CU
+constexpr int kSymBandNT = 256;                     // banded symbolic pass
 // light units span at most 16 blocks: ~53 KB of LDS, three workgroups per CU
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 16;    // <= 4096 outputs, <= 64K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
@@ -1907,7 +2606,72 @@ constexpr int64_t kLearnChunk = 1 << 21;  // workgroups per learned launch
 // beyond 256K columns in flight.
 inline unsigned tiny_grid(int64_t ncols, int waves) {
   return (unsigned)std::min<int64_t>((ncols + waves - 1) / waves, (int64_t)262144 / waves);
-}                            // learned units: expected products
+}
+
+// The tiny tier in the configured form (register sort or LDS table) and
+// workgroup shape.
+template <int K, class T, bool SYM>
+void launch_tiny(const SpgemmConfig& cfg, const int32_t* order, int64_t first, int64_t ncols, const int64_t* cpB,
+                 const int32_t* bpos, const T* valB, const int64_t* cpA, const int32_t* rowsA, const T* valA,
+                 int64_t* nnzcol, const int64_t* cpC, int64_t* irC, T* valC, int* err, const int64_t* stage_off,
+                 hipStream_t s, const Filter<T>& flt = {}) {
+  const dim3 g(tiny_grid(ncols, cfg.tiny_wg_waves)), b(kWave * cfg.tiny_wg_waves);
+  if (cfg.tiny_algo == 1)
+    k_tiny_h<K, T, SYM><<<g, b, 0, s>>>(order, first, ncols, cpB, bpos, valB, cpA, rowsA, valA, nnzcol, cpC, irC,
+                                        valC, err, stage_off, flt);
+  else
+    k_tiny<K, T, SYM><<<g, b, 0, s>>>(order, first, ncols, cpB, bpos, valB, cpA, rowsA, valA, nnzcol, cpC, irC, valC,
+                                      err, stage_off, flt);
+}
+
+// Row lists of the small units (k_sym_units); lists == nullptr: none.
+struct UnitLists {
+  uint16_t* lists = nullptr;
+  int64_t* off = nullptr;  // per symbolic slot, ncols + 1 entries
+  uint16_t* bcnt = nullptr;
+  Unit* units = nullptr;
+  int* n = nullptr;
+  int cap = 0;
+};
+
+// Region of symbolic slot i in the row lists: the column's outputs in the
+// small-unit passes are at most min(flop_j, unit_split_nnz, m).
+__global__ void k_list_regions(const int64_t* __restrict__ flop_s, int64_t n, int64_t cap,
+                               int64_t* __restrict__ sz) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) sz[i] = flop_s[i] < cap ? flop_s[i] : cap;
+  else if (i == n) sz[i] = 0;
+}
+
+// The unit tier's symbolic pass: per column (sym_band 0, optionally with the
+// small units' row lists) or banded (the band height in row blocks, 16 or 64;
+// bcnt_buf receives the block counts).
+cbx_status launch_sym_units(const SpgemmConfig& cfg, const int32_t* order, int64_t ncols, const cbx_dcsc* A,
+                            const cbx_dcsc* B, const int32_t* bpos, const int32_t* rowsA, const int32_t* P, int S,
+                            int64_t* nnzcol, Unit* units_l, int* n_l, int cap_l, Unit* units_h, int* n_h, int cap_h,
+                            uint32_t* rmap, int64_t rm_cols, int* err, DevBuf& bcnt_buf, hipStream_t s,
+                            const UnitLists& ul = {}) {
+  if (cfg.sym_band == 0) {
+    k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)ncols, kSymUnitNT, 0, s>>>(
+        order, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
+        units_l, n_l, cap_l, units_h, n_h, cap_h, rmap, rm_cols, err, ul.lists, ul.off, ul.bcnt, ul.units, ul.n,
+        ul.cap);
+    return CBX_OK;
+  }
+  int32_t* bcnt;
+  CBX_TRY(dalloc(bcnt_buf, (size_t)ncols * S, s, &bcnt));
+  const int band = cfg.sym_band;
+  const dim3 g((unsigned)ncols, (unsigned)((S + band - 1) / band));  // band-major dispatch order
+  if (band == 16)
+    k_sym_band<kSymBandNT, 4, 16><<<g, kSymBandNT, 0, s>>>(order, 0, B->cp, bpos, A->cp, rowsA, P, S, bcnt, rmap,
+                                                          rm_cols);
+  else
+    k_sym_band<kSymBandNT, 4, 64><<<g, kSymBandNT, 0, s>>>(order, 0, B->cp, bpos, A->cp, rowsA, P, S, bcnt, rmap,
+                                                          rm_cols);
+  k_sym_cut<<<(unsigned)ncols, 256, 0, s>>>(order, 0, bcnt, S, kLightK, kLightRB, kHeavyK, kHeavyRB,
+                                            cfg.unit_split_nnz, nnzcol, units_l, n_l, cap_l, units_h, n_h, cap_h, err);
+  return CBX_OK;
+}
 
 }  // namespace
 
@@ -1932,6 +2696,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   DevBuf b_feat, b_lp, b_hist, b_w, b_blo, b_bhi, b_uc, b_ub, b_pn, b_units, b_status, b_tick, b_ct, b_ovf, b_hcs, b_sum,
       b_hflag, b_hcols, b_nh, b_hu;
   CachedBuf b_ul, b_uh;
+  DevBuf b_bcnt;
   uint8_t* heavy_flag;
   int32_t* hcols;
   int64_t* nh_dev;
@@ -2039,9 +2804,8 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
         rm_cols = nheavy;
       (void)hipGetLastError();
     }
-    k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)nheavy, kSymUnitNT, 0, s>>>(
-        hcols, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
-        units_l, hucnt, (int)cap_l, units_h, hucnt + 1, (int)cap_h, rm.p, rm_cols, err);
+    CBX_TRY(launch_sym_units(cfg, hcols, nheavy, A, B, bpos, rowsA, P, S, nnzcol, units_l, hucnt, (int)cap_l, units_h,
+                             hucnt + 1, (int)cap_h, rm.p, rm_cols, err, b_bcnt, s));
     CBX_HIP(hipGetLastError());
   }
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
@@ -2124,6 +2888,11 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32), CBX_ERR_LIMIT,
                 "cbx_spgemm: too many units for a single launch");
   info.heavy_units = hu[0] + hu[1];
+  DevBuf b_ul_sorted, b_uh_sorted;
+  if (cfg.unit_order == 1) {
+    if (hu[0] > 1) CBX_TRY(sort_units_by_block(units_l, hu[0], S, s, b_ul_sorted, &units_l));
+    if (hu[1] > 1) CBX_TRY(sort_units_by_block(units_h, hu[1], S, s, b_uh_sorted, &units_h));
+  }
   if (hu[1] && rm.p)
     k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
         units_h, rm.p, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, rm_cols, cpC, C->ir,
@@ -2146,10 +2915,102 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   return CBX_OK;
 }
 
+// Gather of the fused threshold's runs: run k (in start order) moves its kept
+// entries from its start in the unfiltered layout to dst[k].
+template <class T>
+__global__ __launch_bounds__(256) void k_run_copy(const int32_t* __restrict__ idx, const int64_t* __restrict__ src,
+                                                  const int32_t* __restrict__ kept, const int64_t* __restrict__ dst,
+                                                  const int64_t* __restrict__ ir, const T* __restrict__ val,
+                                                  int64_t* __restrict__ ir2, T* __restrict__ val2) {
+  const int i = idx[blockIdx.x];
+  const int64_t a = src[i], d = dst[blockIdx.x];
+  const int n = kept[i];
+  for (int e = threadIdx.x; e < n; e += 256) {
+    ir2[d + e] = ir[a + e];
+    val2[d + e] = val[a + e];
+  }
+}
+
+__global__ void k_run_sizes(const int32_t* __restrict__ idx, const int32_t* __restrict__ kept,
+                            const int32_t* __restrict__ col, int n, int64_t* __restrict__ kept_sorted,
+                            unsigned long long* __restrict__ colkept) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = idx[k];
+  kept_sorted[k] = kept[i];
+  atomicAdd(&colkept[col[i]], (unsigned long long)kept[i]);
+}
+
+__global__ void k_iota32(int32_t* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+// The product after a fused threshold: the runs the numeric kernels appended
+// (sorted by start, which is column then row order in the unfiltered layout)
+// gathered into new arrays; nnzcol and cpC become the kept counts and their
+// scan.  C keeps its other fields.
+template <class T>
+static cbx_status filter_compact(const Filter<T>& flt, int64_t nzcB, int64_t* nnzcol, int64_t* cpC, void* tmp,
+                                 size_t tmp_bytes, cbx_dcsc* C, hipStream_t s) {
+  int nrun = 0;
+  CBX_HIP(hipMemcpyAsync(&nrun, flt.run_count, sizeof(int), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  CBX_RETURN_IF(nrun > flt.run_cap, CBX_ERR_HIP, "cbx_spgemm: run list overflow (%d > %d)", nrun, flt.run_cap);
+  DevBuf b_k2, b_i, b_i2, b_ks, b_dst, b_t;
+  int64_t *k2, *ks, *dst;
+  int32_t *i1, *i2;
+  uint8_t* t;
+  const int nr = std::max(nrun, 1);
+  CBX_TRY(dalloc(b_k2, nr, s, &k2));
+  CBX_TRY(dalloc(b_i, nr, s, &i1));
+  CBX_TRY(dalloc(b_i2, nr, s, &i2));
+  CBX_TRY(dalloc(b_ks, nr + 1, s, &ks));
+  CBX_TRY(dalloc(b_dst, nr + 1, s, &dst));
+  size_t tb = 0, tb2 = 0;
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, flt.run_src, k2, i1, i2, nr, 0, 64, s));
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, ks, dst, nr + 1, s));
+  tb = std::max(tb, tb2);
+  CBX_TRY(dalloc(b_t, std::max<size_t>(tb, 16), s, &t));
+  CBX_HIP(hipMemsetAsync(nnzcol, 0, sizeof(int64_t) * (nzcB + 1), s));
+  int64_t total = 0;
+  if (nrun) {
+    k_iota32<<<nblk(nrun, 256), 256, 0, s>>>(i1, nrun);
+    size_t tb1 = tb;
+    CBX_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb1, flt.run_src, k2, i1, i2, nrun, 0, 64, s));
+    k_run_sizes<<<nblk(nrun, 256), 256, 0, s>>>(i2, flt.run_kept, flt.run_col, nrun, ks,
+                                                (unsigned long long*)nnzcol);
+    CBX_HIP(hipMemsetAsync(ks + nrun, 0, sizeof(int64_t), s));
+    size_t tb3 = tb;
+    CBX_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb3, ks, dst, nrun + 1, s));
+    CBX_HIP(hipMemcpyAsync(&total, dst + nrun, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    CBX_HIP(hipStreamSynchronize(s));
+  }
+  int64_t* ir2 = nullptr;
+  void* val2 = nullptr;
+  CBX_TRY(dev_alloc((void**)&ir2, sizeof(int64_t) * std::max<int64_t>(total, 1)));
+  if (dev_alloc(&val2, sizeof(T) * std::max<int64_t>(total, 1)) != CBX_OK) {
+    dev_free(ir2);
+    return set_error(CBX_ERR_NOMEM, "cbx_spgemm: allocation of the thresholded product failed");
+  }
+  if (nrun) k_run_copy<T><<<(unsigned)nrun, 256, 0, s>>>(i2, flt.run_src, flt.run_kept, dst, C->ir, (const T*)C->val,
+                                                        ir2, (T*)val2);
+  size_t tb4 = tmp_bytes;
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb4, nnzcol, cpC, (int)(nzcB + 1), s));
+  CBX_HIP(hipGetLastError());
+  CBX_HIP(hipStreamSynchronize(s));
+  dev_free(C->ir);
+  dev_free(C->val);
+  C->ir = ir2;
+  C->val = val2;
+  C->nnz = total;
+  return CBX_OK;
+}
+
 template <int K, class T>
-static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_mode mode, const SpgemmConfig& cfg,
-                               cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s,
-                               int64_t* symbolic_out = nullptr) {
+static cbx_status spgemm_typed_once(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_mode mode,
+                                    const SpgemmConfig& cfg, cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s,
+                                    int64_t* symbolic_out, bool* lists_used) {
   const int64_t m = A->m, n = B->n;
   const int64_t nzcA = A->nzc, nzcB = B->nzc, nnzA = A->nnz, nnzB = B->nnz;
   Timer tm(stats != nullptr);
@@ -2178,14 +3039,23 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   // when that table stays modest, otherwise the streaming dense tier runs.
   const int S4 = (int)((m + kBR - 1) / kBR);
   const bool use_units = m <= (1ll << 24) && (double)nzcA * (S4 + 1) * 4.0 <= 8.0e9;
+  // the fused threshold (two-phase, unit or hash tiers; the streaming tier has no filtered emission)
+  Filter<T> flt;
+  flt.on = cfg.filter && mode == CBX_SPGEMM_TWO_PHASE && use_units && symbolic_out == nullptr;
+  flt.thr = (T)cfg.filter_thr;
+  DevBuf b_drop;
+  if (flt.on) {
+    CBX_TRY(dalloc(b_drop, 1, s, &flt.dropped));
+    CBX_HIP(hipMemsetAsync(flt.dropped, 0, sizeof(unsigned long long), s));
+  }
   const int TR = kDenseTR;
   const int S16 = (int)((m + TR - 1) / TR);
   const int S = use_units ? S4 : S16;
   const int G = kSymRBW * 32 / TR;  // tiles per symbolic pass of the streaming tier
 
   DevBuf b_colmap, b_rows, b_bpos, b_flop, b_order, b_flop_s, b_order_s, b_P, b_nnz, b_cp, b_tmp, b_thr, b_bounds,
-      b_flags, b_nsel, b_err, b_vcnt, b_ucnt, b_soff, b_sir, b_sval;
-  CachedBuf b_ul, b_uh;
+      b_flags, b_nsel, b_err, b_vcnt, b_ucnt, b_soff, b_sir, b_sval, b_bcnt;
+  CachedBuf b_ul, b_uh, b_us;
   int32_t *colmap, *rowsA, *bpos, *order, *order_s, *P;
   int *err, *ucnt;
   unsigned long long* vcnt;
@@ -2195,6 +3065,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   int64_t *flop, *flop_s, *nnzcol, *cpC, *thr, *bounds, *nsel;
   uint8_t* flags;
   Unit *units_l = nullptr, *units_h = nullptr;
+  UnitLists ul;
+  DevBuf b_lsz, b_loff, b_lists, b_lbc;
   CBX_TRY(dalloc(b_colmap, A->n, s, &colmap));
   CBX_TRY(dalloc(b_rows, nnzA, s, &rowsA));
   CBX_TRY(dalloc(b_bpos, nnzB, s, &bpos));
@@ -2210,9 +3082,9 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_TRY(dalloc(b_nsel, 2, s, &nsel));
   CBX_TRY(dalloc(b_err, 1, s, &err));
   CBX_TRY(dalloc(b_vcnt, 6, s, &vcnt));
-  CBX_TRY(dalloc(b_ucnt, 2, s, &ucnt));
+  CBX_TRY(dalloc(b_ucnt, 3, s, &ucnt));
   CBX_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
-  CBX_HIP(hipMemsetAsync(ucnt, 0, 2 * sizeof(int), s));
+  CBX_HIP(hipMemsetAsync(ucnt, 0, 3 * sizeof(int), s));
 
   const T* valA = (const T*)A->val;
   const T* valB = (const T*)B->val;
@@ -2289,11 +3161,13 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   bool learned_done = false;
   LearnedInfo linfo;
   int64_t nnzC = 0, nnz_main = 0;
-  int hu[2] = {0, 0}, herr = 0;
+  int hu[3] = {0, 0, 0}, herr = 0;  // small (bitmap), large, and small list units
   if (learned) {
     if (n_h8) k_sym_hash<8192, kHashNT><<<(unsigned)n_h8, kHashNT, 0, s>>>(order_s, hb[0], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
     if (n_h2) k_sym_hash<2048, kHashNT><<<(unsigned)n_h2, kHashNT, 0, s>>>(order_s, hb[1], flop, B->cp, bpos, A->cp, rowsA, nnzcol, err);
-    if (n_h256) k_tiny<K, T, true><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr, nullptr, nullptr, err);
+    if (n_h256)
+      launch_tiny<K, T, true>(cfg, order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr,
+                              nullptr, nullptr, err, nullptr, s);
     tm.mark(2, s);
     tm.mark(3, s);
     CBX_TRY((learned_numeric<K, T>(A, B, cfg, s, rowsA, bpos, flop, P, S, nnzcol, cpC, tmp, tmp_bytes, C, err, linfo)));
@@ -2314,6 +3188,8 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
       tm.mark(4, s);
     }
   }
+  int64_t nnz_unfiltered = 0;
+  DevBuf b_rsrc, b_rkept, b_rcol, b_rcnt;  // the fused threshold's run list
   if (!learned_done) {
   // --- symbolic
   int64_t cap_l = 0, cap_h = 0;
@@ -2341,9 +3217,40 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     cap_h = std::max<int64_t>(1, std::min<int64_t>(n_heavy_max * (int64_t)S, 1ll << 30));
     CBX_TRY(calloc_cached(b_ul, cap_l, s, &units_l));
     CBX_TRY(calloc_cached(b_uh, cap_h, s, &units_h));
-    k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)n_dense, kSymUnitNT, 0, s>>>(
-        order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
-        units_l, ucnt, (int)cap_l, units_h, ucnt + 1, (int)cap_h, rmap, rm_cols, err);
+    // the small units' row lists (per-column symbolic form only): one region
+    // of min(flop_j, unit_split_nnz, m) 2-byte entries per column, plus the
+    // per-block counts, when they fit a fifth of the free memory
+    if (cfg.unit_lists && cfg.sym_band == 0 && symbolic_out == nullptr) {
+      int64_t* lsz;
+      CBX_TRY(dalloc(b_lsz, n_dense + 1, s, &lsz));
+      CBX_TRY(dalloc(b_loff, n_dense + 1, s, &ul.off));
+      k_list_regions<<<nblk(n_dense + 1, 256), 256, 0, s>>>(flop_s, n_dense, std::min<int64_t>(cfg.unit_split_nnz, m),
+                                                            lsz);
+      size_t need = 0;
+      CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, lsz, ul.off, (int)(n_dense + 1), s));
+      DevBuf b_t3;
+      void* t3 = tmp;
+      if (need > tmp_bytes) CBX_TRY(dalloc(b_t3, need, s, (uint8_t**)&t3));
+      CBX_HIP(hipcub::DeviceScan::ExclusiveSum(t3, need, lsz, ul.off, (int)(n_dense + 1), s));
+      int64_t list_total = 0;
+      CBX_HIP(hipMemcpyAsync(&list_total, ul.off + n_dense, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      CBX_HIP(hipStreamSynchronize(s));
+      size_t freeb = 0, totb = 0;
+      const double bytes = 2.0 * (double)list_total + 2.0 * (double)n_dense * S + (double)sizeof(Unit) * cap_l;
+      if (hipMemGetInfo(&freeb, &totb) == hipSuccess && bytes <= 0.2 * (double)(freeb + cache_idle_bytes()) &&
+          dalloc(b_lists, std::max<int64_t>(list_total, 1), s, &ul.lists) == CBX_OK &&
+          dalloc(b_lbc, n_dense * (int64_t)S, s, &ul.bcnt) == CBX_OK &&
+          calloc_cached(b_us, cap_l, s, &ul.units) == CBX_OK) {
+        ul.n = ucnt + 2;
+        ul.cap = (int)cap_l;
+        if (lists_used) *lists_used = true;
+      } else {
+        ul.lists = nullptr;
+      }
+      (void)hipGetLastError();
+    }
+    CBX_TRY(launch_sym_units(cfg, order_s, n_dense, A, B, bpos, rowsA, P, S, nnzcol, units_l, ucnt, (int)cap_l, units_h,
+                             ucnt + 1, (int)cap_h, rmap, rm_cols, err, b_bcnt, s, ul));
   } else if (n_dense) {
     k_sym_dense<kSymNT, kSymRBW><<<(unsigned)n_dense, kSymNT, 0, s>>>(order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, G,
                                                                        TR, nnzcol);
@@ -2363,19 +3270,19 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     CBX_HIP(hipMemcpyAsync(&slots, stage_off + n_h256, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     CBX_HIP(hipStreamSynchronize(s));
     size_t freeb = 0, totb = 0;
-    if (hipMemGetInfo(&freeb, &totb) == hipSuccess &&
+    // a symbolic-only call (symbolic_out) stages nothing: it counts rows
+    // without reading values, so it runs for operands of any value type
+    if (symbolic_out == nullptr && hipMemGetInfo(&freeb, &totb) == hipSuccess &&
         (double)slots * (sizeof(int64_t) + sizeof(T)) <= 0.25 * (double)(freeb + cache_idle_bytes()) &&
         dalloc(b_sir, std::max<int64_t>(slots, 1), s, &stage_ir) == CBX_OK &&
         dalloc(b_sval, std::max<int64_t>(slots, 1), s, &stage_val) == CBX_OK)
       tiny_staged = true;
     (void)hipGetLastError();
     if (tiny_staged)
-      k_tiny<K, T, false><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
-                                                           valA, nnzcol, nullptr, stage_ir, stage_val, err,
-                                                           stage_off);
+      launch_tiny<K, T, false>(cfg, order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr,
+                               stage_ir, stage_val, err, stage_off, s, flt);
     else
-      k_tiny<K, T, true><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
-                                                          valA, nnzcol, nullptr, nullptr, nullptr, err);
+      launch_tiny<K, T, true>(cfg, order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, nullptr, nullptr, nullptr, err, nullptr, s);
   }
   CBX_HIP(hipGetLastError());
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, cpC, (int)(nzcB + 1), s));
@@ -2392,6 +3299,21 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   CBX_HIP(hipStreamSynchronize(s));
   if (herr) return set_error(CBX_ERR_HIP, "cbx_spgemm: symbolic pass failed (flag %d)", herr);
   if (symbolic_out) return CBX_OK;
+  nnz_unfiltered = nnzC;
+  if (flt.on) {
+    unsigned long long dropped = 0;
+    CBX_HIP(hipMemcpy(&dropped, flt.dropped, sizeof(dropped), hipMemcpyDeviceToHost));
+    nnz_unfiltered += (int64_t)dropped;
+    // one run per unit and per hash- or tiny-tier column at most
+    const int64_t cap = (int64_t)hu[0] + hu[1] + hu[2] + nzcB + 16;
+    CBX_RETURN_IF(cap >= (1ll << 31), CBX_ERR_LIMIT, "cbx_spgemm: too many output ranges for the fused threshold");
+    CBX_TRY(dalloc(b_rsrc, cap, s, &flt.run_src));
+    CBX_TRY(dalloc(b_rkept, cap, s, &flt.run_kept));
+    CBX_TRY(dalloc(b_rcol, cap, s, &flt.run_col));
+    CBX_TRY(dalloc(b_rcnt, 1, s, &flt.run_count));
+    CBX_HIP(hipMemsetAsync(flt.run_count, 0, sizeof(int), s));
+    flt.run_cap = (int)cap;
+  }
 
   // --- output allocation (cached: repeated products reuse the arrays)
   auto alloc_out = [&]() -> cbx_status {
@@ -2414,8 +3336,15 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   tm.mark(3, s);
 
   // --- numeric
-  CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32), CBX_ERR_LIMIT,
-                "cbx_spgemm: too many units for a single launch");
+  CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32) ||
+                    (int64_t)hu[2] * kLightNT >= (1ll << 32),
+                CBX_ERR_LIMIT, "cbx_spgemm: too many units for a single launch");
+  DevBuf b_ul_sorted, b_uh_sorted, b_us_sorted;
+  if (use_units && cfg.unit_order == 1) {
+    if (hu[0] > 1) CBX_TRY(sort_units_by_block(units_l, hu[0], S, s, b_ul_sorted, &units_l));
+    if (hu[1] > 1) CBX_TRY(sort_units_by_block(units_h, hu[1], S, s, b_uh_sorted, &units_h));
+    if (hu[2] > 1) CBX_TRY(sort_units_by_block(ul.units, hu[2], S, s, b_us_sorted, &ul.units));
+  }
   if (use_units) {
     // large units: one sweep over the row map when it was kept (their
     // products exceed what the two-sweep kernel holds in registers); small
@@ -2423,13 +3352,17 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     // registers
     if (hu[1] && rmap)
       k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
-          units_h, rmap, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, rm_cols, cpC, C->ir, (T*)C->val, err);
+          units_h, rmap, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, rm_cols, cpC, C->ir, (T*)C->val, err, flt);
     else if (hu[1])
       k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
-          units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err);
+          units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err, flt);
     if (hu[0])
       k_num_units<K, T, kLightNT, 4, kLightK, kLightRB, 2><<<(unsigned)hu[0], kLightNT, 0, s>>>(
-          units_l, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err);
+          units_l, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err, flt);
+    if (hu[2])
+      k_num_list<K, T, kLightNT, 4, kLightK><<<(unsigned)hu[2], kLightNT, 0, s>>>(
+          ul.units, ul.lists, ul.off, ul.bcnt, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val,
+          err, flt);
   } else if (n_dense) {
     k_num_dense<K, T, kDenseNT><<<(unsigned)n_dense, kDenseNT, 0, s>>>(order_s, 0, B->cp, bpos, valB, A->cp, rowsA,
                                                                      valA, P, S, cpC, C->ir, (T*)C->val, err);
@@ -2441,23 +3374,30 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (NCOL) {                                                                                                     \
     if (sort)                                                                                                     \
       k_num_hash<K, T, HM, HNT, true><<<(unsigned)(NCOL), HNT, 0, s>>>(                                           \
-          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);           \
+          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err, flt);      \
     else                                                                                                          \
       k_num_hash<K, T, HM, HNT, false><<<(unsigned)(NCOL), HNT, 0, s>>>(                                          \
-          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err);           \
+          order_s, FIRST, nnzcol, B->cp, bpos, valB, A->cp, rowsA, valA, cpC, C->ir, (T*)C->val, err, flt);      \
   }
   // the 8192-slot table takes ~100 KB of LDS, so one workgroup per CU: give it 16 waves
   CBX_LAUNCH_HASH(8192, 1024, n_h8, hb[0]);
   CBX_LAUNCH_HASH(2048, kHashNT, n_h2, hb[1]);
   if (n_h256 && tiny_staged)
     k_tiny_place<T><<<wave_grid(n_h256), 256, 0, s>>>(order_s, hb[2], n_h256, stage_off, nnzcol, cpC, stage_ir,
-                                                       stage_val, C->ir, (T*)C->val);
+                                                       stage_val, C->ir, (T*)C->val, flt, err);
   else if (n_h256)
-    k_tiny<K, T, false><<<tiny_grid(n_h256, cfg.tiny_wg_waves), kWave * cfg.tiny_wg_waves, 0, s>>>(order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA,
-                                                         valA, nnzcol, cpC, C->ir, (T*)C->val, err);
+    launch_tiny<K, T, false>(cfg, order_s, hb[2], n_h256, B->cp, bpos, valB, A->cp, rowsA, valA, nnzcol, cpC, C->ir,
+                             (T*)C->val, err, nullptr, s, flt);
 #undef CBX_LAUNCH_HASH
   CBX_HIP(hipGetLastError());
 
+  if (flt.on) {  // the fused threshold's runs into the final arrays
+    CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    CBX_HIP(hipStreamSynchronize(s));
+    if (herr) return set_error(CBX_ERR_HIP, "cbx_spgemm: numeric pass inconsistency (flag %d)", herr);
+    CBX_TRY(filter_compact<T>(flt, nzcB, nnzcol, cpC, tmp, tmp_bytes, C, s));
+    nnzC = C->nnz;
+  }
   // --- DCSC of C: keep the non-empty columns
   k_nonempty_flags<<<nblk(nzcB, 256), 256, 0, s>>>(nnzcol, nzcB, flags);
   CBX_HIP(hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, B->jc, flags, C->jc, nsel, (int)nzcB, s));
@@ -2476,6 +3416,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
   if (stats) {
     stats->flops = total_flop;
     stats->nnz = nnzC;
+    stats->nnz_before_filter = learned_done ? nnzC : nnz_unfiltered;
     stats->nzc = nzcC;
     stats->ms_setup = tm.ms(0, 1);
     stats->ms_symbolic = tm.ms(1, 2);
@@ -2483,7 +3424,7 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     stats->ms_numeric = tm.ms(3, 5);
     stats->ms_main_kernel = learned_done ? linfo.kernel_ms : tm.ms(3, 4);
     stats->ms_total = tm.ms(0, 5);
-    stats->units_light = learned_done ? linfo.units : hu[0];
+    stats->units_light = learned_done ? linfo.units : hu[0] + hu[2];
     stats->units_heavy = learned_done ? linfo.heavy_units : hu[1];
     stats->nnz_main = nnz_main;
     stats->overflow_cols = linfo.overflow_units;
@@ -2492,6 +3433,24 @@ static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_
     stats->learned_fallback = learned && !learned_done;
   }
   return CBX_OK;
+}
+
+// A product whose small units took row lists and then found no room for its
+// output runs again without the lists (its list units cannot run without
+// them, and the lists are released when the first attempt returns).
+template <int K, class T>
+static cbx_status spgemm_typed(const cbx_dcsc* A, const cbx_dcsc* B, cbx_spgemm_mode mode, const SpgemmConfig& cfg,
+                               cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s,
+                               int64_t* symbolic_out = nullptr) {
+  bool lists_used = false;
+  const cbx_status st = spgemm_typed_once<K, T>(A, B, mode, cfg, C, stats, s, symbolic_out, &lists_used);
+  if (st != CBX_ERR_NOMEM || !lists_used) return st;
+  (void)hipStreamSynchronize(s);
+  cbx_dcsc_free(C);
+  release_cache();
+  SpgemmConfig plain = cfg;
+  plain.unit_lists = 0;
+  return spgemm_typed_once<K, T>(A, B, mode, plain, C, stats, s, symbolic_out, nullptr);
 }
 
 cbx_status spgemm_features(const cbx_dcsc* A, const cbx_dcsc* B, float* feats, hipStream_t s) {
@@ -2585,6 +3544,10 @@ cbx_status spgemm_flops(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* flop_host
   return CBX_OK;
 }
 
+// The symbolic pass alone for operands of any value type: with symbolic_out
+// set, spgemm_typed runs only structure kernels (the unit, hash and tiny
+// tiers' counting forms, no staged tiny numeric), so no value is read and the
+// instantiation's value type is immaterial.
 cbx_status spgemm_symbolic(const cbx_dcsc* A, const cbx_dcsc* B, int64_t* nnz, hipStream_t s) {
   cbx_dcsc C;
   SpgemmConfig cfg = current_config();

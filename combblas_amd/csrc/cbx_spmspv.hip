@@ -1077,16 +1077,10 @@ static cbx_status spmv_dense_typed(const cbx_dcsc* A, bool boolA, const T* x, T*
   void* tmp = nullptr;
   size_t tb = 0;
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist, offs, (int)nh, s));
-  auto cleanup = [&]() {
-    for (void* p : {(void*)hist, (void*)offs, (void*)cstart, (void*)brow, (void*)bval, tmp})
-      if (p) (void)hipFreeAsync(p, s);
-  };
-  if (hipMallocAsync(&hist, sizeof(int64_t) * nh, s) != hipSuccess ||
-      hipMallocAsync(&offs, sizeof(int64_t) * nh, s) != hipSuccess ||
-      hipMallocAsync(&cstart, sizeof(int64_t) * (nblk + 1), s) != hipSuccess ||
-      hipMallocAsync(&brow, sizeof(uint16_t) * nnz, s) != hipSuccess ||
-      hipMallocAsync(&bval, sizeof(T) * nnz, s) != hipSuccess ||
-      hipMallocAsync(&tmp, std::max<size_t>(tb, 16), s) != hipSuccess) {
+  ScratchSet ws(s);
+  auto cleanup = [&]() { ws.release(); };
+  if (!ws.take(&hist, nh) || !ws.take(&offs, nh) || !ws.take(&cstart, nblk + 1) || !ws.take(&brow, nnz) ||
+      !ws.take(&bval, nnz) || !ws.take_bytes(&tmp, std::max<size_t>(tb, 16))) {
     cleanup();
     return set_error(CBX_ERR_NOMEM, "cbx_spmv_dense: workspace allocation failed");
   }

@@ -177,7 +177,7 @@ enum {
  * sizes the reference hard-codes: the heap/hash switch cr < 2 at
  * mtSpGEMM.h:311 and minHashTableSize / hashScale at :366-367).  Picked per
  * workload by combblas_amd/autotune.py; defaults {4096, 1024, 256, 65536,
- * 131072, 0.75, 0.15, 0.25, 4}. */
+ * 131072, 0.75, 0.15, 0.25, 4, 1, 0, 0, 1, 1}. */
 typedef struct cbx_spgemm_config {
   int64_t hash_max_flop;   /* columns with flop_j <= this use the LDS hash tier (<= 4096) */
   int64_t hash_mid_flop;   /* ... with flop_j > this use the 8192-slot table, else the 2048-slot one (<= 1024) */
@@ -194,6 +194,22 @@ typedef struct cbx_spgemm_config {
                               pass's row maps of the columns that can form 16384-output units, which those
                               units then read instead of rebuilding them (0 disables; in [0, 1]) */
   int32_t tiny_wg_waves;   /* workgroup shape of the tiny tier: columns (one wave each) per workgroup, 1, 2 or 4 */
+  int32_t unit_order;      /* order of the exactly sized row-range units in the numeric pass: 0 as the symbolic
+                              pass cut them (column by column), 1 by first row block (the units in flight then
+                              gather from a narrow band of A's rows, which the L2 caches hold) */
+  int32_t tiny_algo;       /* tiny tier: 0 sorts a column's products in registers (bitonic network across lanes),
+                              1 merges them in a per-wave LDS hash table and ranks the distinct rows */
+  int32_t sym_band;        /* symbolic pass of the row-range units: 0 one workgroup per column over all its rows;
+                              16 or 64 one workgroup per (column, band of that many 4096-row blocks), dispatched
+                              band by band so the gathers in flight stay in one band of A's rows */
+  int32_t mcl_fuse;        /* cbx_mcl_expand with two-phase sizing and recover_num 0: 1 fuses the hard threshold
+                              into the product's emission (only entries that can survive the epilogue are
+                              written; same result), 0 writes the whole product first */
+  int32_t unit_lists;      /* two-phase mode with sym_band 0: 1 lets the symbolic pass write the sorted rows of the
+                              columns that form 4096-output units (2 bytes per output, when they fit a fifth of
+                              the free memory), so those columns are cut by output count alone, over any number
+                              of row blocks, and each such unit gathers its products once; 0 cuts them at 16
+                              blocks and counts each unit's rows in a first sweep */
 } cbx_spgemm_config;
 
 cbx_status cbx_spgemm_get_config(cbx_spgemm_config* cfg);
@@ -222,6 +238,7 @@ typedef struct cbx_spgemm_stats {
   double ms_numeric;      /* numeric pass including exact re-runs and DCSC assembly */
   double ms_main_kernel;  /* the row-range-unit numeric kernels alone */
   double ms_total;
+  int64_t nnz_before_filter; /* entries before a fused threshold (cbx_mcl_expand's products); = nnz otherwise */
 } cbx_spgemm_stats;
 
 /* Installs the nnz predictor used by CBX_SPGEMM_LEARNED (host array of

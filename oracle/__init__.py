@@ -96,7 +96,8 @@ def _load_ref() -> C.CDLL:
         L.ref_symbolic.restype = C.c_int32
         L.ref_time_hybrid_i64.argtypes = [P(cbx_dcsc), P(cbx_dcsc), P(C.c_double), P(C.c_int64)]
         L.ref_time_hybrid_i64.restype = C.c_int32
-        for nm in ("ref_spmspv_selectmax_bool", "ref_spmspv_plustimes_i64"):
+        for nm in ("ref_spmspv_selectmax_bool", "ref_spmspv_plustimes_i64", "ref_spmspv_minplus_i64",
+                   "ref_spmspv_plustimes_f64"):
             getattr(L, nm).argtypes = [P(cbx_dcsc), C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
             getattr(L, nm).restype = C.c_int32
         L.ref_bfs.argtypes = [P(cbx_dcsc), C.c_int64, C.c_void_p]
@@ -480,14 +481,21 @@ def bfs(A: Dcsc, root: int) -> tuple[np.ndarray, int]:
 
 
 def ref_spmspv(A: Dcsc, xind: np.ndarray, xval: np.ndarray, semiring: str) -> tuple[np.ndarray, np.ndarray]:
-    """Genuine generic_gespmv: SelectMax on a boolean A, or PlusTimes<int64>."""
+    """Genuine generic_gespmv: SelectMax on a boolean A, or on an A of the
+    vector's type PlusTimes<int64>, MinPlus<int64> or PlusTimes<double>."""
     L = _load_ref()
     sa, ka = A._cstruct()
     xind = np.ascontiguousarray(xind, np.int32)
-    xval = np.ascontiguousarray(xval, np.int64)
+    fdt = A.dtype == "float64"
+    xval = np.ascontiguousarray(xval, np.float64 if fdt else np.int64)
     yi = np.zeros(A.m, np.int32)
-    yv = np.zeros(A.m, np.int64)
-    fn = L.ref_spmspv_selectmax_bool if semiring == "select_max" else L.ref_spmspv_plustimes_i64
+    yv = np.zeros(A.m, xval.dtype)
+    if semiring == "select_max":
+        fn = L.ref_spmspv_selectmax_bool
+    elif semiring == "min_plus":
+        fn = L.ref_spmspv_minplus_i64
+    else:
+        fn = L.ref_spmspv_plustimes_f64 if fdt else L.ref_spmspv_plustimes_i64
     ny = fn(C.byref(sa), xind.ctypes.data, xval.ctypes.data, xind.size, yi.ctypes.data, yv.ctypes.data)
     return yi[:ny], yv[:ny]
 

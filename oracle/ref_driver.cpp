@@ -157,6 +157,22 @@ int mcl_prune(const cbx_dcsc* A, double thr, int64_t select, int64_t recover, do
 
 }  // namespace
 
+// generic_gespmv<SR> on an A of the vector's value type (heap form).
+template <class SR, class NT>
+static int32_t ref_spmspv_typed(const cbx_dcsc* A, const int32_t* xind, const NT* xval, int32_t nx, int32_t* yind,
+                                NT* yval) {
+  ensure_mpi();
+  auto* MA = make_mat<NT, NT>(*A);
+  std::vector<int32_t> indy;
+  std::vector<NT> numy;
+  PreAllocatedSPA<NT> spa;
+  generic_gespmv<SR>(*MA, xind, xval, nx, indy, numy, spa);
+  std::copy(indy.begin(), indy.end(), yind);
+  std::copy(numy.begin(), numy.end(), yval);
+  delete MA;
+  return (int32_t)indy.size();
+}
+
 extern "C" {
 
 int32_t ref_mcl_prune(const cbx_dcsc* A, double thr, int64_t select, int64_t recover, double pct, cbx_dcsc* C) {
@@ -331,6 +347,18 @@ int32_t ref_spmspv_plustimes_i64(const cbx_dcsc* A, const int32_t* xind, const i
   std::copy(numy.begin(), numy.end(), yval);
   delete MA;
   return (int32_t)indy.size();
+}
+
+// generic_gespmv<MinPlusSRing<int64,int64>> (inf_plus multiply, min add) and
+// generic_gespmv<PlusTimesSRing<double,double>> on A of the same value type.
+int32_t ref_spmspv_minplus_i64(const cbx_dcsc* A, const int32_t* xind, const int64_t* xval, int32_t nx,
+                               int32_t* yind, int64_t* yval) {
+  return ref_spmspv_typed<MinPlusSRing<int64_t, int64_t>, int64_t>(A, xind, xval, nx, yind, yval);
+}
+
+int32_t ref_spmspv_plustimes_f64(const cbx_dcsc* A, const int32_t* xind, const double* xval, int32_t nx,
+                                 int32_t* yind, double* yval) {
+  return ref_spmspv_typed<PlusTimesSRing<double, double>, double>(A, xind, xval, nx, yind, yval);
 }
 
 // The TopDownBFS level loop (TopDownBFS.cpp:442-448) on one process with the

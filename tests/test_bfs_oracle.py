@@ -61,19 +61,33 @@ def test_oracle_bfs_matches_reference(scale):
 
 
 @needs_ref
-@pytest.mark.parametrize("semiring", ["select_max", "plus_times"])
-def test_oracle_spmspv_matches_reference(semiring):
+@pytest.mark.parametrize("semiring,dtype", [("select_max", "bool"), ("plus_times", "int64"),
+                                            ("min_plus", "int64"), ("plus_times", "float64")])
+def test_oracle_spmspv_matches_reference(semiring, dtype):
+    """The oracle's SpMXSpV against the genuine generic_gespmv, bit for bit
+    (the FP64 sums too: the oracle keeps the reference's heap order); min-plus
+    with operands at numeric_limits<int64>::max() exercises inf_plus."""
     rng = np.random.default_rng(3)
     G = graph(11, 8)
-    A = G if semiring == "select_max" else G.astype("int64")
-    if semiring == "plus_times":
+    A = G.astype(dtype)
+    if dtype == "int64":
         A.val = rng.integers(-5, 9, A.nnz).astype(np.int64)
+        if semiring == "min_plus":
+            A.val[rng.random(A.nnz) < 0.1] = np.iinfo(np.int64).max
+    elif dtype == "float64":
+        A.val = rng.uniform(-2.0, 2.0, A.nnz)
     for frac in (0.001, 0.05, 0.5):
         x = np.sort(rng.choice(A.n, max(1, int(frac * A.n)), replace=False)).astype(np.int32)
-        xv = rng.integers(-100, 100, x.size).astype(np.int64)
+        if dtype == "float64":
+            xv = rng.uniform(-1.0, 1.0, x.size)
+        else:
+            xv = rng.integers(-100, 100, x.size).astype(np.int64)
+            if semiring == "min_plus":
+                xv[rng.random(x.size) < 0.1] = np.iinfo(np.int64).max
         y1 = oracle.spmspv(A, x, xv, semiring)
         y2 = oracle.ref_spmspv(A, x, xv, semiring)
-        assert np.array_equal(y1[0], y2[0]) and np.array_equal(y1[1], y2[1])
+        assert np.array_equal(y1[0], y2[0])
+        assert np.array_equal(y1[1].view(np.uint8), y2[1].view(np.uint8))
 
 
 @pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference tree)")

@@ -77,6 +77,76 @@ def test_mcl_expand_exact(phases, params):
     assert st["phases"] == phases and st["nnz_product"] == C.nnz and st["nnz"] == want.nnz
 
 
+@pytest.mark.parametrize("tiers", ["default", "units"])
+@pytest.mark.parametrize("tiny_algo", [0, 1])
+def test_mcl_fused_threshold_matches_unfused(tiers, tiny_algo):
+    """With recover_num 0 the two-phase expansion fuses the hard threshold into
+    the product's emission (mcl_fuse 1): the kernels write only entries with
+    !(v < threshold), compacted per output range, and the ranges are gathered.
+    On dyadic values (every sum exact in FP32, so the LDS atomics' order does
+    not show) the result equals the unfused expansion's bit for bit and the
+    oracle's, for a sparse power-law matrix and a denser one (mean 50 entries
+    per column, at most 500), with the unit tier forced for most columns or not, in both
+    tiny-tier forms, in one and in three phases."""
+    A = dyadic(6000, 13)
+    B = cb.gen_powerlaw_stochastic(5000, seed=6)
+    # k/64, k in 1..4: products are multiples of 2^-12, at most 2^-8, and a
+    # column's at most 500 * 500 of them sum below 2^10: exact in 22 bits
+    B.val = (np.random.default_rng(6).integers(1, 5, B.nnz) / 64.0).astype(np.float32)
+    params = (2.0 ** -14, 40, 0, 0.9)
+    base = cb.get_config()
+    extra = {"tiny_flop": 128, "hash_mid_flop": 256, "hash_max_flop": 256} if tiers == "units" else {}
+    try:
+        for X in (A, B):
+            dX = X.to_device()
+            out = {}
+            for fuse in (0, 1):
+                cb.set_config(**dict(base, mcl_fuse=fuse, tiny_algo=tiny_algo, **extra))
+                for phases in (1, 3):
+                    got, st = cb.mcl_expand(dX, dX, *params, phases=phases)
+                    out[fuse, phases] = (got.to_host(), st)
+            for key in out:
+                bit_exact(out[key][0], out[0, 1][0])
+                assert out[key][1]["nnz_product"] == out[0, 1][1]["nnz_product"]
+                assert out[key][1]["nnz"] == out[0, 1][1]["nnz"]
+            C, _ = oracle.spgemm(X, X)
+            bit_exact(out[1, 1][0], oracle.mcl_prune(C, *params))
+            assert out[1, 3][1]["nnz_product"] == C.nnz
+    finally:
+        cb.set_config(**base)
+
+
+def test_mcl_fused_threshold_on_row_list_units():
+    """The fused threshold through the row-list units' emission: a uniform
+    random 300,007-column matrix (12 entries per column, dyadic values) with
+    the unit tier forced for every column, so that each column's ~140
+    outputs over 74 row blocks form one list unit.  Lists on and off, fused
+    and not: bit for bit the same, and the oracle's."""
+    rng = np.random.default_rng(8)
+    n = 300_007
+    key = np.unique(rng.integers(0, n, 12 * n) * n + rng.integers(0, n, 12 * n))
+    vals = (rng.integers(1, 17, key.size) / 1024.0).astype(np.float32)  # <= 900 terms of <= 2^-12: exact
+    A = cb.Dcsc.from_coo(n, n, key % n, key // n, vals, "float32")
+    params = (2.0 ** -14, 40, 0, 0.9)
+    base = cb.get_config()
+    dA = A.to_device()
+    out = {}
+    try:
+        for lists in (0, 1):
+            for fuse in (0, 1):
+                cb.set_config(**dict(base, tiny_flop=64, hash_mid_flop=64, hash_max_flop=64, unit_lists=lists,
+                                     mcl_fuse=fuse))
+                got, st = cb.mcl_expand(dA, dA, *params, phases=2)
+                out[lists, fuse] = (got.to_host(), st)
+    finally:
+        cb.set_config(**base)
+    C, _ = oracle.spgemm(A, A)
+    want = oracle.mcl_prune(C, *params)
+    for key_, (got, st) in out.items():
+        bit_exact(got, want)
+        assert st["nnz_product"] == C.nnz, key_
+
+
 def test_mcl_expand_learned_exact():
     from tests.test_learned_gpu import constant_predictor
     A = dyadic(6000, 12)
@@ -89,6 +159,28 @@ def test_mcl_expand_learned_exact():
     finally:
         cb.set_predictor(None)
     bit_exact(got.to_host(), want)
+
+
+def test_mcl_expand_learned_powerlaw_network():
+    """The learned expansion sized by the committed power-law network (the
+    one the MCL benchmark installs) equals the oracle's pruned product, in
+    one and in two phases."""
+    from combblas_amd import predictor
+    if not predictor.model_path("powerlaw").exists():
+        pytest.skip("power-law network not trained (python -m combblas_amd.predictor train powerlaw)")
+    A = dyadic(6000, 14)
+    C, _ = oracle.spgemm(A, A)
+    params = (2.0 ** -14, 40, 0, 0.9)
+    want = oracle.mcl_prune(C, *params)
+    cb.set_predictor(predictor.load("powerlaw"))
+    try:
+        dA = A.to_device()
+        for phases in (1, 2):
+            got, st = cb.mcl_expand(dA, dA, *params, phases=phases, mode="learned")
+            bit_exact(got.to_host(), want)
+            assert st["nnz_product"] == C.nnz
+    finally:
+        cb.set_predictor(None)
 
 
 def _keys(X: cb.Dcsc) -> np.ndarray:

@@ -62,3 +62,22 @@ def test_features_and_symbolic_labels():
     assert np.allclose(np.exp2(X[:, 1]) - 1, flop, rtol=1e-5)
     b = np.diff(A.cp)
     assert np.allclose(np.exp2(X[:, 0]) - 1, b, rtol=1e-5)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "bool", "float64"])
+def test_symbolic_labels_any_value_type(dtype):
+    """The symbolic pass alone reads no values, so it serves operands of every
+    value type (the power-law model's labels come from float32 matrices; the
+    symbolic-only call used to run the tiny tier's staged numeric form, which
+    read a 1- or 4-byte value array as int64, past its end): per-column counts
+    equal to the oracle's on the pattern, with tiny, hash and unit-tier
+    columns present."""
+    import oracle
+    A = cb.gen_rmat(13, 8).astype(dtype)
+    dA = A.to_device()
+    nnz = cb.spgemm_symbolic(dA, dA)
+    P = A.astype("int64")
+    P.val = np.ones(P.nnz, np.int64)
+    flop, want = oracle.symbolic(P, P)
+    assert np.array_equal(nnz, want)
+    assert (flop <= 256).any() and (flop > 4096).any()

@@ -98,6 +98,36 @@ def test_semirings_rmat(semiring, dtype):
         bit_exact(got, want)
 
 
+@pytest.mark.parametrize("semiring,dtype", SR_DTYPES)
+@pytest.mark.parametrize("waves", [1, 4])
+def test_tiny_tier_lds_table(semiring, dtype, waves):
+    """The tiny tier's LDS-table form (tiny_algo 1: per-wave hash table,
+    ranks by counting) in two workgroup shapes, on products whose columns
+    hold up to 256 products: the staged two-phase numeric pass and, through
+    the learned mode's exact small columns, the symbolic form, against the
+    oracle (integer semirings bit-exact, FP plus-times within the bound)."""
+    rng = np.random.default_rng(23)
+    A = signed(cb.gen_rmat(12, 4).astype(dtype), rng)
+    B = signed(cb.gen_rmat(12, 4, seed=9).astype(dtype), rng)
+    want = None if semiring == "plus_times" and dtype.startswith("float") else oracle.spgemm(A, B, semiring)[0]
+    base = cb.get_config()
+    try:
+        cb.set_config(tiny_algo=1, tiny_wg_waves=waves)
+        got, st = gpu_spgemm(A, B, semiring)
+        P = np.zeros(cb.MLP_PARAMS, np.float32)
+        P[-6:] = 1.0  # feature scale: a constant prediction (learned mode's tiny columns are exact either way)
+        cb.set_predictor(P)
+        got_l, st_l = gpu_spgemm(A, B, semiring, "learned")
+    finally:
+        cb.set_predictor(None)
+        cb.set_config(**base)
+    for C in (got, got_l):
+        if want is None:
+            check_fp(C, A, B)
+        else:
+            bit_exact(C, want)
+
+
 @pytest.mark.parametrize("name", sorted(p.name for p in GOLDEN.glob("spgemm_tiny_*.npz")))
 def test_tiny_golden(name):
     g = np.load(GOLDEN / name)
@@ -240,6 +270,99 @@ def test_symbolic_bitmap_sub_passes(m):
         cb.set_config(**base)
     want, _ = oracle.spgemm(A, B)
     assert st["units_light"] + st["units_heavy"] > 0
+    bit_exact(got, want)
+
+
+@pytest.mark.parametrize("band", [16, 64])
+@pytest.mark.parametrize("m,share", [((1 << 21) + 5000, 0.25), (300_007, 0.0), (300_007, 0.25)])
+def test_banded_symbolic_pass(band, m, share):
+    """sym_band: the unit tier's symbolic pass in bands of 16 / 64 row blocks
+    (one workgroup per column and band, then the per-column cut), on tall
+    operands with a partial last band (300,007 rows = 74 blocks) and several
+    256-block cutting passes (2^21 + 5000 rows), with and without row maps:
+    bit-exact against the oracle, as many units as the per-column pass."""
+    rng = np.random.default_rng(41)
+    n, k = 1500, 200
+    key = np.unique(rng.integers(0, n, 300_000) * m + rng.integers(0, m, 300_000))
+    A = cb.Dcsc.from_coo(m, n, key % m, key // m, rng.integers(-4, 5, key.size).astype(np.int64), "int64")
+    keyb = np.unique(rng.integers(0, k, 12_000) * n + rng.integers(0, n, 12_000))
+    B = cb.Dcsc.from_coo(n, k, keyb % n, keyb // n, rng.integers(-4, 5, keyb.size).astype(np.int64), "int64")
+    base = cb.get_config()
+    try:
+        # the per-column pass with the 16-block cut (no row lists), which the banded pass shares
+        cb.set_config(unit_split_nnz=4096, row_map_share=share, sym_band=0, unit_lists=0)
+        ref, st0 = gpu_spgemm(A, B)
+        cb.set_config(unit_split_nnz=4096, row_map_share=share, sym_band=band)
+        got, st = gpu_spgemm(A, B)
+        P = np.zeros(cb.MLP_PARAMS, np.float32)
+        P[-6:] = 1.0  # feature scale
+        P[-13] = 20.0  # output bias: every column predicted far above heavy_nnz, so all take the exact path
+        cb.set_predictor(P)
+        cb.set_config(unit_split_nnz=4096, row_map_share=share, sym_band=band, heavy_nnz=4096)
+        got_l, st_l = gpu_spgemm(A, B, "plus_times", "learned")  # its exact columns run the banded pass too
+    finally:
+        cb.set_predictor(None)
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, B)
+    assert st["units_light"] + st["units_heavy"] > 0
+    assert (st["units_light"], st["units_heavy"]) == (st0["units_light"], st0["units_heavy"])
+    for C in (ref, got, got_l):
+        bit_exact(C, want)
+
+
+def _spread(m, n, k, seed, dtype="int64"):
+    """A (m x n) with rows spread uniformly over all of m, B (n x k): the
+    product's columns hold ~10^4 outputs thinly over every 4096-row block.
+    Values are nonzero integers in [-4, 4]: no product is a signed zero, whose
+    sign under max would depend on the order of equal products."""
+    rng = np.random.default_rng(seed)
+    nz = np.array([-4, -3, -2, -1, 1, 2, 3, 4], np.int64)
+    key = np.unique(rng.integers(0, n, 400_000) * m + rng.integers(0, m, 400_000))
+    A = cb.Dcsc.from_coo(m, n, key % m, key // m, rng.choice(nz, key.size), "int64")
+    keyb = np.unique(rng.integers(0, k, 15_000) * n + rng.integers(0, n, 15_000))
+    B = cb.Dcsc.from_coo(n, k, keyb % n, keyb // n, rng.choice(nz, keyb.size), "int64")
+    return A.astype(dtype), B.astype(dtype)
+
+
+@pytest.mark.parametrize("m", [300_007, (1 << 21) + 5000])
+@pytest.mark.parametrize("semiring,dtype", [("plus_times", "int64"), ("min_plus", "float64"),
+                                            ("select_max", "float32"), ("plus_times", "bool")])
+def test_small_units_by_row_list(m, semiring, dtype):
+    """unit_lists: the symbolic pass writes the sorted rows of the small-unit
+    columns, cuts their units by output count over any number of row blocks,
+    and those wider than 16 blocks take their ranks from the list
+    (k_num_list).  Thin columns over every block of A's rows, in one 74-block
+    pass and in three 256-block passes: fewer units than the 16-block cut,
+    and the same product as without lists and as the oracle, bit for bit."""
+    A, B = _spread(m, 2000, 300, 43, dtype)
+    base = cb.get_config()
+    try:
+        cb.set_config(unit_lists=0)
+        plain, st0 = gpu_spgemm(A, B, semiring)
+        cb.set_config(unit_lists=1)
+        got, st = gpu_spgemm(A, B, semiring)
+    finally:
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, B, semiring)
+    assert 0 < st["units_light"] < st0["units_light"]
+    assert st["nnz_main"] == st0["nnz_main"] > 0
+    bit_exact(plain, want)
+    bit_exact(got, want)
+
+
+def test_row_list_units_with_large_units_and_row_maps():
+    """Row-list units beside large units (split lowered to 4096) that read
+    row maps, on several 256-block passes: per pass, columns over the split
+    take large units, the others list or bitmap units."""
+    A, B = _spread((1 << 21) + 5000, 2000, 300, 47)
+    base = cb.get_config()
+    try:
+        cb.set_config(unit_split_nnz=4096, row_map_share=0.25)
+        got, st = gpu_spgemm(A, B)
+    finally:
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, B)
+    assert st["units_heavy"] > 0
     bit_exact(got, want)
 
 
