@@ -102,6 +102,10 @@ __device__ __forceinline__ int ceil_log2_dev(long long x) {
 // is emitted in row order without sorting and LDS holds only real outputs.
 constexpr int kBR = 4096;        // rows per block (also the tile-pointer granularity)
 constexpr int kBRW = kBR / 32;   // 32-bit bitmap words per block
+// Row lists (k_sym_units) cover the blocks with at most this many outputs:
+// sixteen denser blocks already fill a 4096-output unit, so a unit holding a
+// denser block spans at most 16 blocks and takes its ranks from a bitmap.
+constexpr int kListBlockMax = 256;
 
 struct Unit {
   int32_t col;     // B stored-column position
@@ -1097,13 +1101,14 @@ __global__ __launch_bounds__(NT) void k_num_dense(const int32_t* __restrict__ or
 // rows, through the tile table P), so the LDS bitmap stays small enough for
 // several workgroups per CU.  Writes nnz_j and the column's units.
 //
-// With `lists`, the column's sorted output rows are also written, as 12-bit
-// offsets within their 4096-row blocks, into the column's region
+// With `lists`, the column's sorted output rows in blocks of at most
+// kListBlockMax outputs are also written, as 12-bit offsets within their
+// 4096-row blocks, at their output positions in the column's region
 // lists[list_off[slot] .. list_off[slot + 1]) (slot = the workgroup), and its
 // per-block counts into bcnt16[slot * S ..].  The small units of a pass whose
-// rows all fit the region are then cut by output count alone, over any
-// number of blocks: those wider than rblk_l blocks go to units_s, whose
-// numeric pass takes ranks from the list instead of a bitmap.
+// rows fit the region are then cut by output count alone over runs of such
+// blocks: those wider than rblk_l blocks go to units_s, whose numeric pass
+// takes ranks from the list instead of a bitmap.
 template <int NT, int U, int IBLK>
 __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ order, int64_t first,
                                                   const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
@@ -1130,6 +1135,7 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
   __shared__ SegStage<uint8_t, NT> st;
   __shared__ long long wsum[NT / kWave];
   __shared__ int s_nu, s_slot, s_ns, s_sslot;
+  __shared__ unsigned long long s_dense[BPP / kWave];  // blocks of the pass above kListBlockMax outputs
   Unit* su2 = reinterpret_cast<Unit*>(&st);  // list units of a pass (the stage is idle while cutting)
   const int col = order[first + blockIdx.x];
   const int64_t b0 = cpB[col], b1 = cpB[col + 1];
@@ -1202,6 +1208,7 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
         if (listok) {
           for (int b = wave_id(); b < inb; b += NT / kWave) {
             const int boff = __shfl(bincl - bc, b, kWave);
+            if (__shfl(bc, b, kWave) > kListBlockMax) continue;  // dense block: its units use the bitmap
             const uint32_t* w = bits + b * kBRW;
             uint64_t x = (uint64_t)w[2 * lane_id()] | ((uint64_t)w[2 * lane_id() + 1] << 32);
             const int c = __popcll(x);
@@ -1219,7 +1226,7 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
         __syncthreads();  // the bitmap is read before the next sub-pass clears it
       }
     }
-    if (wave_id() == 0) {  // inclusive prefix over blocks
+    if (wave_id() == 0) {  // inclusive prefix over blocks, and the mask of the blocks without a list
       int carry = 0;
       for (int b0i = 0; b0i < nblk; b0i += kWave) {
         const int b = b0i + lane_id();
@@ -1227,6 +1234,8 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
         const int inc = wave_incl_scan(v) + carry;
         if (b < nblk) bpre[b] = inc;
         carry = __shfl(inc, kWave - 1, kWave);
+        const unsigned long long dense = __ballot(b < nblk && v > kListBlockMax);
+        if (lane_id() == 0) s_dense[b0i / kWave] = dense;
       }
     }
     __syncthreads();
@@ -1237,10 +1246,21 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
       const bool heavy = pass_total > heavy_thr;
       const bool by_list = !heavy && listok;
       const int kmax = heavy ? kmax_h : kmax_l, rblk = heavy ? rblk_h : rblk_l;
+      // first block at or after `from` whose rows were not listed (nblk if none)
+      auto next_dense = [&](int from) {
+        for (int q = from / kWave; q * kWave < nblk; ++q) {
+          unsigned long long msk = s_dense[q];
+          if (q == from / kWave) msk &= ~0ull << (from % kWave);
+          if (msk) return min(nblk, q * kWave + __ffsll(msk) - 1);
+        }
+        return nblk;
+      };
       int start = 0, before = 0, nu = 0, ns = 0;
       while (start < nblk) {
         const int lim = before + kmax;
-        const int endmax = by_list ? nblk : min(nblk, start + rblk);
+        // a unit wider than rblk blocks takes its ranks from the list, so it
+        // stops before the next block whose rows were not listed
+        const int endmax = by_list ? min(nblk, max(start + rblk, next_dense(start))) : min(nblk, start + rblk);
         int lo = start, hi = endmax - 1;  // last b in [start, endmax) with bpre[b] <= lim
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
@@ -2635,12 +2655,58 @@ struct UnitLists {
 };
 
 // Region of symbolic slot i in the row lists: the column's outputs in the
-// small-unit passes are at most min(flop_j, unit_split_nnz, m).
-__global__ void k_list_regions(const int64_t* __restrict__ flop_s, int64_t n, int64_t cap,
-                               int64_t* __restrict__ sz) {
+// small-unit passes are at most min(flop_j, unit_split_nnz, m).  Slot i is
+// column order[i] (flop indexed by column), or column i of flop when order
+// is null (flop already in slot order).
+__global__ void k_list_regions(const int32_t* __restrict__ order, const int64_t* __restrict__ flop, int64_t n,
+                               int64_t cap, int64_t* __restrict__ sz) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) sz[i] = flop_s[i] < cap ? flop_s[i] : cap;
-  else if (i == n) sz[i] = 0;
+  if (i < n) {
+    const int64_t f = flop[order ? order[i] : i];
+    sz[i] = f < cap ? f : cap;
+  } else if (i == n) {
+    sz[i] = 0;
+  }
+}
+
+// Region offsets, row lists, block counts and list-unit array for the
+// symbolic slots order[0..n) (order null: slot i is flop[i]), when they fit a
+// fifth of the free memory; otherwise ul.lists stays null.  The buffers are
+// owned by the caller's DevBufs / CachedBuf.
+static cbx_status alloc_unit_lists(const int32_t* order, const int64_t* flop, int64_t n, int S, int64_t cap_units,
+                                   int64_t region_cap, int* counter, void* tmp, size_t tmp_bytes, DevBuf& b_lsz,
+                                   DevBuf& b_loff, DevBuf& b_lists, DevBuf& b_lbc, CachedBuf& b_us, hipStream_t s,
+                                   UnitLists& ul) {
+  ul = UnitLists{};
+  int64_t *lsz, *off;
+  CBX_TRY(dalloc(b_lsz, n + 1, s, &lsz));
+  CBX_TRY(dalloc(b_loff, n + 1, s, &off));
+  k_list_regions<<<nblk(n + 1, 256), 256, 0, s>>>(order, flop, n, region_cap, lsz);
+  size_t need = 0;
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, lsz, off, (int)(n + 1), s));
+  DevBuf b_t3;
+  void* t3 = tmp;
+  if (need > tmp_bytes) CBX_TRY(dalloc(b_t3, need, s, (uint8_t**)&t3));
+  CBX_HIP(hipcub::DeviceScan::ExclusiveSum(t3, need, lsz, off, (int)(n + 1), s));
+  int64_t total = 0;
+  CBX_HIP(hipMemcpyAsync(&total, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  size_t freeb = 0, totb = 0;
+  const double bytes = 2.0 * (double)total + 2.0 * (double)n * S + (double)sizeof(Unit) * cap_units;
+  uint16_t *lists = nullptr, *bcnt = nullptr;
+  Unit* units = nullptr;
+  if (hipMemGetInfo(&freeb, &totb) == hipSuccess && bytes <= 0.2 * (double)(freeb + cache_idle_bytes()) &&
+      dalloc(b_lists, std::max<int64_t>(total, 1), s, &lists) == CBX_OK &&
+      dalloc(b_lbc, n * (int64_t)S, s, &bcnt) == CBX_OK && calloc_cached(b_us, cap_units, s, &units) == CBX_OK) {
+    ul.lists = lists;
+    ul.off = off;
+    ul.bcnt = bcnt;
+    ul.units = units;
+    ul.n = counter;
+    ul.cap = (int)cap_units;
+  }
+  (void)hipGetLastError();
+  return CBX_OK;
 }
 
 // The unit tier's symbolic pass: per column (sym_band 0, optionally with the
@@ -2695,8 +2761,9 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   const int64_t nzcB = B->nzc, m = A->m;
   DevBuf b_feat, b_lp, b_hist, b_w, b_blo, b_bhi, b_uc, b_ub, b_pn, b_units, b_status, b_tick, b_ct, b_ovf, b_hcs, b_sum,
       b_hflag, b_hcols, b_nh, b_hu;
-  CachedBuf b_ul, b_uh;
-  DevBuf b_bcnt;
+  CachedBuf b_ul, b_uh, b_us;
+  DevBuf b_bcnt, b_lsz, b_loff, b_lists, b_lbc;
+  UnitLists ul;
   uint8_t* heavy_flag;
   int32_t* hcols;
   int64_t* nh_dev;
@@ -2726,7 +2793,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   CBX_TRY(dalloc(b_hflag, nzcB, s, &heavy_flag));
   CBX_TRY(dalloc(b_hcols, nzcB, s, &hcols));
   CBX_TRY(dalloc(b_nh, 1, s, &nh_dev));
-  CBX_TRY(dalloc(b_hu, 2, s, &hucnt));
+  CBX_TRY(dalloc(b_hu, 3, s, &hucnt));
   // prediction: the network on the columns' features, or (without it) the
   // min-hash estimates of estimateNNZ_sampling
   if (cfg.sampling_rounds > 0) {
@@ -2794,7 +2861,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
     cap_h = cap_l;
     CBX_TRY(calloc_cached(b_ul, cap_l, s, &units_l));
     CBX_TRY(calloc_cached(b_uh, cap_h, s, &units_h));
-    CBX_HIP(hipMemsetAsync(hucnt, 0, 2 * sizeof(int), s));
+    CBX_HIP(hipMemsetAsync(hucnt, 0, 3 * sizeof(int), s));
     if (cfg.row_map_share > 0.0f) {
       const int64_t rm_words = nheavy * (int64_t)S * kBRW;
       size_t freeb = 0, totb = 0;
@@ -2804,8 +2871,12 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
         rm_cols = nheavy;
       (void)hipGetLastError();
     }
+    // the small units' row lists, as in the two-phase path
+    if (cfg.unit_lists && cfg.sym_band == 0)
+      CBX_TRY(alloc_unit_lists(hcols, flop, nheavy, S, cap_l, std::min<int64_t>(cfg.unit_split_nnz, m), hucnt + 2, tmp,
+                               tmp_bytes, b_lsz, b_loff, b_lists, b_lbc, b_us, s, ul));
     CBX_TRY(launch_sym_units(cfg, hcols, nheavy, A, B, bpos, rowsA, P, S, nnzcol, units_l, hucnt, (int)cap_l, units_h,
-                             hucnt + 1, (int)cap_h, rm.p, rm_cols, err, b_bcnt, s));
+                             hucnt + 1, (int)cap_h, rm.p, rm_cols, err, b_bcnt, s, ul));
     CBX_HIP(hipGetLastError());
   }
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
@@ -2881,17 +2952,19 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
     return CBX_OK;
   }
   // the heavy columns' values, at the offsets the scan above fixed
-  int hu[2] = {0, 0};
+  int hu[3] = {0, 0, 0};  // small (bitmap), large, and small list units
   CBX_HIP(hipMemcpy(hu, hucnt, sizeof(hu), hipMemcpyDeviceToHost));
   (void)hipEventRecord(kev[2], s);  // numeric of the heavy columns, after the offsets
-  CBX_RETURN_IF(hu[0] > cap_l || hu[1] > cap_h, CBX_ERR_HIP, "cbx_spgemm: unit list overflow");
-  CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32), CBX_ERR_LIMIT,
-                "cbx_spgemm: too many units for a single launch");
-  info.heavy_units = hu[0] + hu[1];
-  DevBuf b_ul_sorted, b_uh_sorted;
+  CBX_RETURN_IF(hu[0] > cap_l || hu[1] > cap_h || hu[2] > cap_l, CBX_ERR_HIP, "cbx_spgemm: unit list overflow");
+  CBX_RETURN_IF((int64_t)hu[0] * kLightNT >= (1ll << 32) || (int64_t)hu[1] * kHeavyNT >= (1ll << 32) ||
+                    (int64_t)hu[2] * kLightNT >= (1ll << 32),
+                CBX_ERR_LIMIT, "cbx_spgemm: too many units for a single launch");
+  info.heavy_units = hu[0] + hu[1] + hu[2];
+  DevBuf b_ul_sorted, b_uh_sorted, b_us_sorted;
   if (cfg.unit_order == 1) {
     if (hu[0] > 1) CBX_TRY(sort_units_by_block(units_l, hu[0], S, s, b_ul_sorted, &units_l));
     if (hu[1] > 1) CBX_TRY(sort_units_by_block(units_h, hu[1], S, s, b_uh_sorted, &units_h));
+    if (hu[2] > 1) CBX_TRY(sort_units_by_block(ul.units, hu[2], S, s, b_us_sorted, &ul.units));
   }
   if (hu[1] && rm.p)
     k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
@@ -2903,6 +2976,10 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   if (hu[0])
     k_num_units<K, T, kLightNT, 4, kLightK, kLightRB, 2><<<(unsigned)hu[0], kLightNT, 0, s>>>(
         units_l, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err);
+  if (hu[2])
+    k_num_list<K, T, kLightNT, 4, kLightK><<<(unsigned)hu[2], kLightNT, 0, s>>>(
+        ul.units, ul.lists, ul.off, ul.bcnt, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC,
+        C->ir, (T*)C->val, err);
   CBX_HIP(hipGetLastError());
   (void)hipEventRecord(kev[1], s);
   CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -3221,33 +3298,9 @@ static cbx_status spgemm_typed_once(const cbx_dcsc* A, const cbx_dcsc* B, cbx_sp
     // of min(flop_j, unit_split_nnz, m) 2-byte entries per column, plus the
     // per-block counts, when they fit a fifth of the free memory
     if (cfg.unit_lists && cfg.sym_band == 0 && symbolic_out == nullptr) {
-      int64_t* lsz;
-      CBX_TRY(dalloc(b_lsz, n_dense + 1, s, &lsz));
-      CBX_TRY(dalloc(b_loff, n_dense + 1, s, &ul.off));
-      k_list_regions<<<nblk(n_dense + 1, 256), 256, 0, s>>>(flop_s, n_dense, std::min<int64_t>(cfg.unit_split_nnz, m),
-                                                            lsz);
-      size_t need = 0;
-      CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, lsz, ul.off, (int)(n_dense + 1), s));
-      DevBuf b_t3;
-      void* t3 = tmp;
-      if (need > tmp_bytes) CBX_TRY(dalloc(b_t3, need, s, (uint8_t**)&t3));
-      CBX_HIP(hipcub::DeviceScan::ExclusiveSum(t3, need, lsz, ul.off, (int)(n_dense + 1), s));
-      int64_t list_total = 0;
-      CBX_HIP(hipMemcpyAsync(&list_total, ul.off + n_dense, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-      CBX_HIP(hipStreamSynchronize(s));
-      size_t freeb = 0, totb = 0;
-      const double bytes = 2.0 * (double)list_total + 2.0 * (double)n_dense * S + (double)sizeof(Unit) * cap_l;
-      if (hipMemGetInfo(&freeb, &totb) == hipSuccess && bytes <= 0.2 * (double)(freeb + cache_idle_bytes()) &&
-          dalloc(b_lists, std::max<int64_t>(list_total, 1), s, &ul.lists) == CBX_OK &&
-          dalloc(b_lbc, n_dense * (int64_t)S, s, &ul.bcnt) == CBX_OK &&
-          calloc_cached(b_us, cap_l, s, &ul.units) == CBX_OK) {
-        ul.n = ucnt + 2;
-        ul.cap = (int)cap_l;
-        if (lists_used) *lists_used = true;
-      } else {
-        ul.lists = nullptr;
-      }
-      (void)hipGetLastError();
+      CBX_TRY(alloc_unit_lists(nullptr, flop_s, n_dense, S, cap_l, std::min<int64_t>(cfg.unit_split_nnz, m), ucnt + 2,
+                               tmp, tmp_bytes, b_lsz, b_loff, b_lists, b_lbc, b_us, s, ul));
+      if (ul.lists && lists_used) *lists_used = true;
     }
     CBX_TRY(launch_sym_units(cfg, order_s, n_dense, A, B, bpos, rowsA, P, S, nnzcol, units_l, ucnt, (int)cap_l, units_h,
                              ucnt + 1, (int)cap_h, rmap, rm_cols, err, b_bcnt, s, ul));

@@ -336,18 +336,27 @@ def test_small_units_by_row_list(m, semiring, dtype):
     and the same product as without lists and as the oracle, bit for bit."""
     A, B = _spread(m, 2000, 300, 43, dtype)
     base = cb.get_config()
+    P = np.zeros(cb.MLP_PARAMS, np.float32)
+    P[-6:] = 1.0  # feature scale
+    P[-13] = 20.0  # output bias: predictions clamp to min(flop_j, m), above heavy_nnz: every column exact
     try:
         cb.set_config(unit_lists=0)
         plain, st0 = gpu_spgemm(A, B, semiring)
         cb.set_config(unit_lists=1)
         got, st = gpu_spgemm(A, B, semiring)
+        cb.set_predictor(P)  # learned mode: its exact columns cut list units too
+        cb.set_config(unit_lists=1, heavy_nnz=4096)
+        got_l, st_l = gpu_spgemm(A, B, semiring, "learned")
     finally:
+        cb.set_predictor(None)
         cb.set_config(**base)
     want, _ = oracle.spgemm(A, B, semiring)
     assert 0 < st["units_light"] < st0["units_light"]
     assert st["nnz_main"] == st0["nnz_main"] > 0
+    assert st_l["units_heavy"] == st["units_light"] + st["units_heavy"]  # the exact columns' units
     bit_exact(plain, want)
     bit_exact(got, want)
+    bit_exact(got_l, want)
 
 
 def test_row_list_units_with_large_units_and_row_maps():
