@@ -491,7 +491,7 @@ def main() -> int:
     main_nnz = st_last["nnz_main"]
     main_bytes = 16 * main_nnz + dcsc_bytes(A.nnz, A.nzc) * 2
     achieved = main_bytes / (main_ms * 1e-3) / 1e9 if main_ms > 0 else None
-    kernel = "k_num_learned" if primary == "learned" else "k_num_units (light+heavy launches)"
+    kernel = "k_num_learned" if primary == "learned" else "unit kernels (k_num_units, k_num_list, k_num_units_rm)"
     traffic = None  # HBM bytes per launch from the committed rocprofv3 --pmc summary of this kernel
     tpath = os.path.join(ROOT, "profiles", f"traffic_rmat_s{args.scale}_ef{args.edgefactor}.json")
     if os.path.exists(tpath):
