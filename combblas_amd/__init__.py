@@ -90,7 +90,7 @@ class cbx_spgemm_config(C.Structure):
                 ("heavy_nnz", C.c_int64), ("unit_split_nnz", C.c_int64), ("learned_target", C.c_float),
                 ("capacity_margin", C.c_float), ("row_map_share", C.c_float), ("tiny_wg_waves", C.c_int32),
                 ("unit_order", C.c_int32), ("tiny_algo", C.c_int32), ("sym_band", C.c_int32),
-                ("mcl_fuse", C.c_int32), ("unit_lists", C.c_int32)]
+                ("mcl_fuse", C.c_int32), ("unit_lists", C.c_int32), ("unit_xcd_group", C.c_int32)]
 
 
 _lib_handle = None

@@ -83,10 +83,11 @@ GRID = {
     "hash_mid_flop": [1024],
     "tiny_flop": [256],
     "unit_split_nnz": [131072, 262144],
-    "unit_order": [0, 1],
+    "unit_order": [1],
     "tiny_algo": [0, 1],
     "sym_band": [0, 16],
     "unit_lists": [0, 1],
+    "unit_xcd_group": [0, 32],
 }
 # the learned single pass adds its own knobs; the hash tiers are swept more
 # coarsely there to keep the product of the grids small

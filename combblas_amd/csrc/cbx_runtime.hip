@@ -375,7 +375,7 @@ extern "C" cbx_status cbx_spgemm_set_sampling_predictor(int32_t nrounds) {
 
 // ------------------------------------------------------------- tunables
 static std::mutex g_config_mu;
-static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f, 4, 1, 0, 0, 1, 1};
+static cbx_spgemm_config g_config = {4096, 1024, 256, 65536, 131072, 0.75f, 0.15f, 0.25f, 4, 1, 0, 0, 1, 1, 0};
 
 SpgemmConfig cbx::current_config() {
   SpgemmConfig c;
@@ -395,6 +395,7 @@ SpgemmConfig cbx::current_config() {
     c.sym_band = g_config.sym_band;
     c.mcl_fuse = g_config.mcl_fuse != 0;
     c.unit_lists = g_config.unit_lists;
+    c.unit_xcd_group = g_config.unit_xcd_group;
   }
   c.predictor = predictor_params();
   c.sampling_rounds = sampling_rounds_setting();
@@ -437,6 +438,11 @@ extern "C" cbx_status cbx_spgemm_set_config(const cbx_spgemm_config* cfg) {
                 "cbx_spgemm_set_config: mcl_fuse must be 0 or 1");
   CBX_RETURN_IF(cfg->unit_lists != 0 && cfg->unit_lists != 1, CBX_ERR_INVALID,
                 "cbx_spgemm_set_config: unit_lists must be 0 or 1");
+  {
+    const int g = cfg->unit_xcd_group;
+    CBX_RETURN_IF(g != 0 && g != 8 && g != 16 && g != 32 && g != 64 && g != 128, CBX_ERR_INVALID,
+                  "cbx_spgemm_set_config: unit_xcd_group must be 0, 8, 16, 32, 64 or 128");
+  }
   std::lock_guard<std::mutex> g(g_config_mu);
   g_config = *cfg;
   return CBX_OK;

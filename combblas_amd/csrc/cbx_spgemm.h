@@ -63,6 +63,9 @@ struct SpgemmConfig {
   // rows of the small-unit columns, whose units are then cut by output count
   // over any number of row blocks (k_num_list).
   int unit_lists = 1;
+  // Numeric unit launches: runs of this many consecutive (row-block sorted)
+  // units per XCD (0: the hardware's round-robin order).
+  int unit_xcd_group = 0;
 };
 
 // The process-wide tunables (cbx_spgemm_set_config) with the installed

@@ -116,6 +116,22 @@ struct Unit {
   int32_t slot;    // the column's row-map slot (its workgroup in the symbolic launch)
 };
 
+// The unit of workgroup b (of n) when runs of g consecutive units go to one
+// XCD.  The hardware deals a launch's workgroups to the 8 XCDs in turn
+// (workgroup b runs on XCD b % 8), so with units sorted by row block the
+// identity mapping spreads every row band over all eight L2 caches; here,
+// within each super-group of 8g workgroups, XCD x takes units x*g .. x*g+g-1,
+// and the runs still interleave across the XCDs.  The tail after the last
+// full super-group keeps the identity; g = 0 is the identity.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_unit(int b, int n, int g) {
+  if (g <= 0) return b;
+  const int sg = kXcds * g;
+  if (b >= n / sg * sg) return b;
+  const int r = b % sg;
+  return b - r + (r % kXcds) * g + r / kXcds;
+}
+
 // Fused threshold of the MCL expansion (MCLPruneRecoverySelect's hard
 // threshold, ParFriends.h:185-345, when no column can need recovery): with
 // `on`, the numeric kernels emit only the entries with !(v < thr) of every
@@ -1551,7 +1567,7 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
                                                   const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                                                   const T* __restrict__ valA, const int32_t* __restrict__ P, int S,
                                                   const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
-                                                  T* __restrict__ valC, int* __restrict__ err, Filter<T> flt = {}) {
+                                                  T* __restrict__ valC, int* __restrict__ err, Filter<T> flt = {}, int xg = 0) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int NW32 = RBLK * kBRW;   // bitmap words
@@ -1562,7 +1578,7 @@ __global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units
   __shared__ uint16_t pre[NW64];
   __shared__ SegStage<T, NT> st;
   __shared__ long long wsum[NT / kWave];
-  const Unit un = units[blockIdx.x];
+  const Unit un = units[xcd_unit((int)blockIdx.x, (int)gridDim.x, xg)];
   const int col = un.col;
   const int t_lo = un.blk_lo, t_hi = un.blk_hi;
   const int32_t row0 = t_lo * kBR;
@@ -1745,7 +1761,7 @@ __global__ __launch_bounds__(NT) void k_num_units_rm(const Unit* __restrict__ un
                                                      const int32_t* __restrict__ P, int S, int64_t rm_cols,
                                                      const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
                                                      T* __restrict__ valC, int* __restrict__ err,
-                                                     Filter<T> flt = {}) {
+                                                     Filter<T> flt = {}, int xg = 0) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int NW32 = RBLK * kBRW;
@@ -1756,7 +1772,7 @@ __global__ __launch_bounds__(NT) void k_num_units_rm(const Unit* __restrict__ un
   __shared__ uint16_t pre[NW64];
   __shared__ SegStage<T, NT> st;
   __shared__ long long wsum[NT / kWave];
-  const Unit un = units[blockIdx.x];
+  const Unit un = units[xcd_unit((int)blockIdx.x, (int)gridDim.x, xg)];
   const int col = un.col;
   const int t_lo = un.blk_lo, t_hi = un.blk_hi;
   const int32_t row0 = t_lo * kBR;
@@ -1848,7 +1864,7 @@ __global__ __launch_bounds__(NT) void k_num_list(const Unit* __restrict__ units,
                                                  const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                                                  const T* __restrict__ valA, const int32_t* __restrict__ P, int S,
                                                  const int64_t* __restrict__ cpC, int64_t* __restrict__ irC,
-                                                 T* __restrict__ valC, int* __restrict__ err, Filter<T> flt = {}) {
+                                                 T* __restrict__ valC, int* __restrict__ err, Filter<T> flt = {}, int xg = 0) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   constexpr int BPP = 256;  // blocks of one symbolic cutting pass
@@ -1858,7 +1874,7 @@ __global__ __launch_bounds__(NT) void k_num_list(const Unit* __restrict__ units,
   __shared__ int dir[BPP + 1];  // list position of each block's first row; dir[nb] = cnt
   __shared__ SegStage<T, NT> st;
   __shared__ long long wsum[NT / kWave];
-  const Unit un = units[blockIdx.x];
+  const Unit un = units[xcd_unit((int)blockIdx.x, (int)gridDim.x, xg)];
   const int col = un.col;
   const int t_lo = un.blk_lo, t_hi = un.blk_hi, nb = t_hi - t_lo;
   const int32_t row0 = t_lo * kBR;
@@ -2969,17 +2985,19 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   if (hu[1] && rm.p)
     k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
         units_h, rm.p, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, rm_cols, cpC, C->ir,
-        (T*)C->val, err);
+        (T*)C->val, err, Filter<T>{}, cfg.unit_xcd_group);
   else if (hu[1])
     k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
-        units_h, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err);
+        units_h, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err,
+        Filter<T>{}, cfg.unit_xcd_group);
   if (hu[0])
     k_num_units<K, T, kLightNT, 4, kLightK, kLightRB, 2><<<(unsigned)hu[0], kLightNT, 0, s>>>(
-        units_l, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err);
+        units_l, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err,
+        Filter<T>{}, cfg.unit_xcd_group);
   if (hu[2])
     k_num_list<K, T, kLightNT, 4, kLightK><<<(unsigned)hu[2], kLightNT, 0, s>>>(
         ul.units, ul.lists, ul.off, ul.bcnt, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC,
-        C->ir, (T*)C->val, err);
+        C->ir, (T*)C->val, err, Filter<T>{}, cfg.unit_xcd_group);
   CBX_HIP(hipGetLastError());
   (void)hipEventRecord(kev[1], s);
   CBX_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -3405,17 +3423,18 @@ static cbx_status spgemm_typed_once(const cbx_dcsc* A, const cbx_dcsc* B, cbx_sp
     // registers
     if (hu[1] && rmap)
       k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
-          units_h, rmap, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, rm_cols, cpC, C->ir, (T*)C->val, err, flt);
+          units_h, rmap, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, rm_cols, cpC, C->ir, (T*)C->val, err, flt,
+          cfg.unit_xcd_group);
     else if (hu[1])
       k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
-          units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err, flt);
+          units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err, flt, cfg.unit_xcd_group);
     if (hu[0])
       k_num_units<K, T, kLightNT, 4, kLightK, kLightRB, 2><<<(unsigned)hu[0], kLightNT, 0, s>>>(
-          units_l, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err, flt);
+          units_l, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err, flt, cfg.unit_xcd_group);
     if (hu[2])
       k_num_list<K, T, kLightNT, 4, kLightK><<<(unsigned)hu[2], kLightNT, 0, s>>>(
           ul.units, ul.lists, ul.off, ul.bcnt, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val,
-          err, flt);
+          err, flt, cfg.unit_xcd_group);
   } else if (n_dense) {
     k_num_dense<K, T, kDenseNT><<<(unsigned)n_dense, kDenseNT, 0, s>>>(order_s, 0, B->cp, bpos, valB, A->cp, rowsA,
                                                                      valA, P, S, cpC, C->ir, (T*)C->val, err);

@@ -177,7 +177,7 @@ enum {
  * sizes the reference hard-codes: the heap/hash switch cr < 2 at
  * mtSpGEMM.h:311 and minHashTableSize / hashScale at :366-367).  Picked per
  * workload by combblas_amd/autotune.py; defaults {4096, 1024, 256, 65536,
- * 131072, 0.75, 0.15, 0.25, 4, 1, 0, 0, 1, 1}. */
+ * 131072, 0.75, 0.15, 0.25, 4, 1, 0, 0, 1, 1, 0}. */
 typedef struct cbx_spgemm_config {
   int64_t hash_max_flop;   /* columns with flop_j <= this use the LDS hash tier (<= 4096) */
   int64_t hash_mid_flop;   /* ... with flop_j > this use the 8192-slot table, else the 2048-slot one (<= 1024) */
@@ -210,6 +210,10 @@ typedef struct cbx_spgemm_config {
                               the free memory), so those columns are cut by output count alone, over any number
                               of row blocks, and each such unit gathers its products once; 0 cuts them at 16
                               blocks and counts each unit's rows in a first sweep */
+  int32_t unit_xcd_group;  /* numeric pass of the row-range units: 0 leaves the hardware's round-robin deal of
+                              workgroups to the 8 XCDs; 8, 16, 32, 64 or 128 give each XCD runs of that many
+                              consecutive units (sorted by row block with unit_order 1), so an XCD's L2 caches
+                              one band of A's rows at a time */
 } cbx_spgemm_config;
 
 cbx_status cbx_spgemm_get_config(cbx_spgemm_config* cfg);

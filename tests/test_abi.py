@@ -64,6 +64,7 @@ def test_tunables_validated_without_gpu():
         assert base["unit_split_nnz"] == 131072 and base["heavy_nnz"] == 65536
         assert abs(base["row_map_share"] - 0.25) < 1e-6 and base["tiny_wg_waves"] == 4 and base["unit_order"] == 1
         assert base["unit_lists"] == 1 and base["mcl_fuse"] == 1 and base["sym_band"] == 0
+        assert base["unit_xcd_group"] == 0
         got = cb.set_config(hash_max_flop=2048, heavy_nnz=131072, unit_split_nnz=262144)
         assert got["hash_max_flop"] == 2048 and cb.get_config()["heavy_nnz"] == 131072
         assert cb.get_config()["unit_split_nnz"] == 262144
@@ -72,7 +73,8 @@ def test_tunables_validated_without_gpu():
                     {"capacity_margin": -1.0}, {"row_map_share": 1.5}, {"row_map_share": -0.1},
                     {"tiny_wg_waves": 3}, {"tiny_wg_waves": 0}, {"tiny_wg_waves": 8}, {"unit_order": 2},
                     {"unit_order": -1}, {"tiny_algo": 2}, {"sym_band": 32},
-                    {"mcl_fuse": 2}, {"unit_lists": 2}, {"unit_lists": -1}):
+                    {"mcl_fuse": 2}, {"unit_lists": 2}, {"unit_lists": -1}, {"unit_xcd_group": 3},
+                    {"unit_xcd_group": 256}, {"unit_xcd_group": -8}):
             with pytest.raises(cb.CbxError):
                 cb.set_config(**bad)
         with pytest.raises(ValueError):

@@ -359,6 +359,36 @@ def test_small_units_by_row_list(m, semiring, dtype):
     bit_exact(got_l, want)
 
 
+@pytest.mark.parametrize("group", [8, 16, 32, 64, 128])
+def test_xcd_grouped_unit_order(group):
+    """unit_xcd_group: the unit launches hand runs of `group` consecutive
+    units to each XCD (a permutation of workgroups to units, identity on the
+    tail after the last full super-group of 8 * group): the same product as
+    the round-robin order and as the oracle, in both modes (large units split
+    at 4096 outputs so all three unit kernels run)."""
+    A = cb.gen_rmat(16, 16)
+    dA = A.to_device()
+    base = cb.get_config()
+    P = np.zeros(cb.MLP_PARAMS, np.float32)
+    P[-6:] = 1.0
+    P[-13] = 20.0  # learned mode: every unit-tier column exact, through the same unit kernels
+    try:
+        cb.set_config(unit_split_nnz=4096, unit_order=1)
+        ref, st0 = cb.spgemm(dA, dA)
+        cb.set_config(unit_split_nnz=4096, unit_order=1, unit_xcd_group=group)
+        got, st = cb.spgemm(dA, dA)
+        cb.set_predictor(P)
+        cb.set_config(unit_split_nnz=4096, unit_order=1, unit_xcd_group=group, heavy_nnz=4096)
+        got_l, _ = cb.spgemm(dA, dA, "plus_times", "learned")
+    finally:
+        cb.set_predictor(None)
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, A)
+    assert st["units_light"] + st["units_heavy"] > 8 * group
+    for C in (ref, got, got_l):
+        bit_exact(C.to_host(), want)
+
+
 def test_row_list_units_with_large_units_and_row_maps():
     """Row-list units beside large units (split lowered to 4096) that read
     row maps, on several 256-block passes: per pass, columns over the split
