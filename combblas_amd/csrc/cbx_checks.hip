@@ -91,13 +91,18 @@ __global__ void k_check_zero(unsigned long long* __restrict__ cnt) {
   if (threadIdx.x < 3) cnt[threadIdx.x] = 0ull;
 }
 
+static unsigned check_blocks(int64_t n) {
+  return (unsigned)std::min<int64_t>(std::max<int64_t>((n + kCheckNT - 1) / kCheckNT, 1), kCheckBlocks);
+}
+
 void dcsc_check_launch(const cbx_dcsc* X, unsigned long long* cnt, hipStream_t s) {
-  auto blocks = [](int64_t n) {
-    return (unsigned)std::min<int64_t>(std::max<int64_t>((n + kCheckNT - 1) / kCheckNT, 1), kCheckBlocks);
-  };
+  dcsc_check_columns_launch(X, cnt, s);
+  if (X->nnz > 0) k_check_entries<<<check_blocks(X->nnz), kCheckNT, 0, s>>>(X->ir, X->nnz, X->m, cnt);
+}
+
+void dcsc_check_columns_launch(const cbx_dcsc* X, unsigned long long* cnt, hipStream_t s) {
   k_check_zero<<<1, 64, 0, s>>>(cnt);
-  k_check_columns<<<blocks(X->nzc), kCheckNT, 0, s>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->n, cnt);
-  if (X->nnz > 0) k_check_entries<<<blocks(X->nnz), kCheckNT, 0, s>>>(X->ir, X->nnz, X->m, cnt);
+  k_check_columns<<<check_blocks(X->nzc), kCheckNT, 0, s>>>(X->cp, X->jc, X->ir, X->nzc, X->nnz, X->n, cnt);
 }
 
 cbx_status dcsc_check_result(const unsigned long long* cnt, hipStream_t s) {

@@ -7,20 +7,25 @@
 // the SpDCCols(SpTuples) conversion after it (SpDCCols.cpp:109-184): entries
 // with equal (row, col) are combined with SR::add, nothing is dropped
 // (explicit zeros stay), the result is column-major with ascending rows.  The
-// reference merges k column-sorted lists with a heap; here every entry gets a
-// 64-bit (col, row) key (each part's keys are already in order), a tree of
-// pairwise merges orders all of them in ceil(log2 count) passes, a
-// reduce-by-key applies the semiring's add to equal keys, and a run-length
-// encoding of the key's column half gives the DCSC columns.  Equal keys are
-// combined in part order (the merges are stable), the reference in heap
-// order: this changes only floating-point rounding.
+// reference merges k column-sorted lists with a heap; here the merge of c
+// parts P_s (m x n) is the product [P_0 ... P_{c-1}] * B of their column
+// concatenation (m x cn) with B (cn x n) stacking c identities whose entries
+// are the semiring's multiplicative one (1 for plus-times and select-max,
+// -0.0 / 0 for min-plus: x + -0.0 = x keeps a -0.0 entry's sign), so C(:,j)
+// combines the parts' columns j with the semiring's add: the SpGEMM's own
+// tiers do the merge and write DCSC directly.  Equal entries are combined in
+// the accumulation's order, the reference in heap order: this changes only
+// floating-point rounding.
+//
+// cbx_dcsc_from_coo (the Graph500 edge-list constructor, further below) keys
+// the entries, radix-sorts them and assembles with reduce-by-key.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
-#include <rocprim/device/device_merge.hpp>
 
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "cbx_device.h"
@@ -31,21 +36,6 @@ namespace cbx {
 namespace {
 
 constexpr int kMergeNT = 256;
-
-// keys[out + e] = (col << 32) | row for every entry of one part, values copied;
-// one wave per stored column
-template <class T>
-__global__ void k_merge_keys(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc,
-                             const int64_t* __restrict__ ir, const T* __restrict__ val, int64_t nzc, int64_t out,
-                             uint64_t* __restrict__ keys, T* __restrict__ vals) {
-  const int64_t q = global_wave();
-  if (q >= nzc) return;
-  const uint64_t hi = (uint64_t)jc[q] << 32;
-  for (int64_t e = cp[q] + lane_id(); e < cp[q + 1]; e += kWave) {
-    keys[out + e] = hi | (uint64_t)ir[e];
-    vals[out + e] = val[e];
-  }
-}
 
 // The semiring's add on stored values (SR::add of the reference's semirings).
 template <int K, class T>
@@ -138,22 +128,61 @@ cbx_status reduce_assemble(const uint64_t* keys, const T* vals, int N, uint64_t*
   return CBX_OK;
 }
 
+// The parts as column blocks of one matrix: cp shifted by the entries before
+// the part, jc by the part's first column (its index times n).
+__global__ void k_cat_columns(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc, int64_t nzc,
+                              int64_t entry_base, int64_t col_base, int64_t* __restrict__ cpA,
+                              int64_t* __restrict__ jcA) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < nzc) {
+    cpA[q] = cp[q] + entry_base;
+    jcA[q] = jc[q] + col_base;
+  }
+}
+
+__global__ void k_cat_close(int64_t* __restrict__ cpA, int64_t nzc, int64_t nnz) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) cpA[nzc] = nnz;
+}
+
+// B (c n x n): column j holds c entries, at rows s n + j, each the one.
+template <class T>
+__global__ void k_stacked_identity(int64_t n, int c, T one, int64_t* __restrict__ cpB, int64_t* __restrict__ jcB,
+                                   int64_t* __restrict__ irB, T* __restrict__ valB) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j > n) return;
+  cpB[j] = j * c;
+  if (j == n) return;
+  jcB[j] = j;
+  for (int p = 0; p < c; ++p) {
+    irB[j * c + p] = p * n + j;
+    valB[j * c + p] = one;
+  }
+}
+
+template <int K, class T>
+T semiring_one() {
+  if constexpr (K == kMinPlus) {
+    if constexpr (std::is_floating_point<T>::value) return (T)-0.0;
+    else return (T)0;
+  } else {
+    return (T)1;
+  }
+}
+
 template <int K, class T>
 cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipStream_t s) {
   const int64_t m = parts[0].m, n = parts[0].n;
-  int64_t total = 0;
-  std::vector<int64_t> base(count);
+  int64_t nnz = 0, nzc = 0;
   for (int32_t i = 0; i < count; ++i) {
-    base[i] = total;
-    total += parts[i].nnz;
+    nnz += parts[i].nnz;
+    nzc += parts[i].nzc;
   }
   std::memset(out, 0, sizeof(*out));
   out->m = m;
   out->n = n;
   out->dtype = parts[0].dtype;
   out->location = CBX_DEVICE;
-  CBX_RETURN_IF(total >= (1ll << 31), CBX_ERR_LIMIT, "cbx_dcsc_merge: more than 2^31 entries in one merge");
-  if (total == 0) {  // an empty m x n matrix
+  if (nnz == 0) {  // an empty m x n matrix
     cbx_status st = CBX_OK;
     if ((st = dev_alloc((void**)&out->cp, sizeof(int64_t))) != CBX_OK ||
         (st = dev_alloc((void**)&out->jc, sizeof(int64_t))) != CBX_OK ||
@@ -166,64 +195,55 @@ cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipS
     CBX_HIP(hipStreamSynchronize(s));
     return CBX_OK;
   }
-  const int N = (int)total;
-  uint64_t *k0 = nullptr, *k1 = nullptr, *ukeys = nullptr;
-  T *v0 = nullptr, *v1 = nullptr;
-  int64_t *nruns = nullptr, *counts = nullptr;
-  void* tmp = nullptr;
-  size_t tb = 0, t1 = 0, t2 = 0;
-  CBX_HIP(rocprim::merge(nullptr, t1, k0, k0, k1, v0, v0, v1, (size_t)N, (size_t)N, rocprim::less<uint64_t>(), s));
-  CBX_HIP((assemble_tmp_bytes<K, T>(N, s, t2)));
-  tb = std::max(t1, t2);
+  CBX_RETURN_IF((double)count * (double)n >= 9.0e18, CBX_ERR_LIMIT, "cbx_dcsc_merge: too many columns in the parts");
+  cbx_dcsc A{}, B{};
+  A.m = m;
+  A.n = (int64_t)count * n;
+  A.nnz = nnz;
+  A.nzc = nzc;
+  A.dtype = B.dtype = parts[0].dtype;
+  A.location = B.location = CBX_DEVICE;
+  B.m = A.n;
+  B.n = n;
+  B.nnz = (int64_t)count * n;
+  B.nzc = n;
   ScratchSet ws(s);
-  auto cleanup = [&]() { ws.release(); };
-  if (!ws.take(&k0, N) || !ws.take(&k1, N) || !ws.take(&ukeys, N) || !ws.take(&v0, N) || !ws.take(&v1, N) ||
-      !ws.take(&nruns, 2) || !ws.take(&counts, N) || !ws.take_bytes(&tmp, std::max<size_t>(tb, 16))) {
-    (void)hipGetLastError();
-    cleanup();
+  int64_t *cpA, *jcA, *irA, *cpB, *jcB, *irB;
+  T *valA, *valB;
+  if (!ws.take(&cpA, nzc + 1) || !ws.take(&jcA, nzc) || !ws.take(&irA, nnz) || !ws.take(&valA, nnz) ||
+      !ws.take(&cpB, n + 1) || !ws.take(&jcB, n) || !ws.take(&irB, B.nnz) || !ws.take(&valB, B.nnz))
     return set_error(CBX_ERR_NOMEM, "cbx_dcsc_merge: workspace allocation failed");
-  }
-  for (int32_t i = 0; i < count; ++i)
-    if (parts[i].nzc)
-      k_merge_keys<T><<<wave_grid(parts[i].nzc), 256, 0, s>>>(parts[i].cp, parts[i].jc, parts[i].ir,
-                                                                (const T*)parts[i].val, parts[i].nzc, base[i], k0,
-                                                                v0);
-  hipError_t e = hipGetLastError();
-  // pairwise merges of adjacent sorted runs until one remains (in k1/v1)
-  std::vector<int64_t> runs(base);
-  runs.push_back(total);  // run r is [runs[r], runs[r+1])
-  uint64_t *ksrc = k0, *kdst = k1;
-  T *vsrc = v0, *vdst = v1;
-  while (e == hipSuccess && runs.size() > 2) {
-    std::vector<int64_t> next;
-    for (size_t r = 0; r + 1 < runs.size() && e == hipSuccess; r += 2) {
-      next.push_back(runs[r]);
-      const int64_t a = runs[r], b = runs[r + 1], c = r + 2 < runs.size() ? runs[r + 2] : b;
-      if (c > b)
-        e = rocprim::merge(tmp, tb, ksrc + a, ksrc + b, kdst + a, vsrc + a, vsrc + b, vdst + a, (size_t)(b - a),
-                           (size_t)(c - b), rocprim::less<uint64_t>(), s);
-      else if (b > a)  // an odd run out: copied through
-        e = hipMemcpyAsync(kdst + a, ksrc + a, sizeof(uint64_t) * (b - a), hipMemcpyDeviceToDevice, s) ||
-                    hipMemcpyAsync(vdst + a, vsrc + a, sizeof(T) * (b - a), hipMemcpyDeviceToDevice, s)
-                ? hipErrorUnknown
-                : hipSuccess;
+  A.cp = cpA;
+  A.jc = jcA;
+  A.ir = irA;
+  A.val = valA;
+  B.cp = cpB;
+  B.jc = jcB;
+  B.ir = irB;
+  B.val = valB;
+  int64_t eb = 0, qb = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const cbx_dcsc& P = parts[i];
+    if (P.nzc) {
+      k_cat_columns<<<(unsigned)((P.nzc + kMergeNT - 1) / kMergeNT), kMergeNT, 0, s>>>(
+          P.cp, P.jc, P.nzc, eb, (int64_t)i * n, cpA + qb, jcA + qb);
+      CBX_HIP(hipMemcpyAsync(irA + eb, P.ir, sizeof(int64_t) * P.nnz, hipMemcpyDeviceToDevice, s));
+      CBX_HIP(hipMemcpyAsync(valA + eb, P.val, sizeof(T) * P.nnz, hipMemcpyDeviceToDevice, s));
     }
-    next.push_back(total);
-    runs.swap(next);
-    std::swap(ksrc, kdst);
-    std::swap(vsrc, vdst);
+    eb += P.nnz;
+    qb += P.nzc;
   }
-  if (e == hipSuccess && ksrc != k1) {  // one run: bring it to k1/v1
-    e = hipMemcpyAsync(k1, ksrc, sizeof(uint64_t) * N, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(v1, vsrc, sizeof(T) * N, hipMemcpyDeviceToDevice, s);
-  }
-  if (e != hipSuccess) {
-    cleanup();
-    return set_error(CBX_ERR_HIP, "cbx_dcsc_merge: %s", hipGetErrorString(e));
-  }
-  // distinct (col, row) keys and their combined values (values reuse v0)
-  const cbx_status st = reduce_assemble<K, T>(k1, v1, N, ukeys, v0, nruns, counts, tmp, tb, out, s, "cbx_dcsc_merge");
-  cleanup();
+  k_cat_close<<<1, 64, 0, s>>>(cpA, nzc, nnz);
+  k_stacked_identity<T><<<(unsigned)((n + 1 + kMergeNT - 1) / kMergeNT), kMergeNT, 0, s>>>(
+      n, count, semiring_one<K, T>(), cpB, jcB, irB, valB);
+  CBX_HIP(hipGetLastError());
+  // the parts' structure is validated as A's (rows in range and ascending per
+  // column, columns ascending, cp consistent); B is built here
+  SpgemmConfig cfg = current_config();
+  cfg.trusted_b = true;
+  const cbx_semiring sr = K == kPlusTimes ? CBX_SR_PLUS_TIMES : K == kMinPlus ? CBX_SR_MIN_PLUS : CBX_SR_SELECT_MAX;
+  const cbx_status st = spgemm_dispatch(&A, &B, sr, CBX_SPGEMM_TWO_PHASE, cfg, out, nullptr, s);
+  if (st != CBX_OK) cbx_dcsc_free(out);
   return st;
 }
 
@@ -320,7 +340,6 @@ extern "C" cbx_status cbx_dcsc_merge(int32_t count, const cbx_dcsc* parts, cbx_s
     if ((st = check_dcsc(&parts[i], "part", CBX_DEVICE)) != CBX_OK) return st;
     CBX_RETURN_IF(parts[i].m != parts[0].m || parts[i].n != parts[0].n || parts[i].dtype != parts[0].dtype,
                   CBX_ERR_INVALID, "cbx_dcsc_merge: part %d differs in shape or dtype from part 0", i);
-    if ((st = cbx_dcsc_check(&parts[i])) != CBX_OK) return st;  // keys index by the parts' rows and columns
   }
   hipStream_t s = (hipStream_t)stream;
   const cbx_dtype dt = parts[0].dtype;

@@ -145,6 +145,11 @@ cbx_status spgemm_estimate_sampling(const cbx_dcsc* A, const cbx_dcsc* B, int ro
 // s; the result call waits for s and reads them.
 void dcsc_check_launch(const cbx_dcsc* X, unsigned long long* cnt, hipStream_t s);
 cbx_status dcsc_check_result(const unsigned long long* cnt, hipStream_t s);
+// The per-column half alone (cnt[0..3) cleared first; cnt[0] violations,
+// cnt[2] descents at column starts), for a pass that reads every entry
+// anyway and does the per-entry half itself (rows in range, the descents
+// ir[e+1] <= ir[e] of all entries, which must equal cnt[2]).
+void dcsc_check_columns_launch(const cbx_dcsc* X, unsigned long long* cnt, hipStream_t s);
 
 cbx_status spgemm_dispatch(const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring sr, cbx_spgemm_mode mode,
                            const SpgemmConfig& cfg, cbx_dcsc* C, cbx_spgemm_stats* stats, hipStream_t s);

@@ -927,17 +927,73 @@ __global__ void k_spmv_dense(const int64_t* __restrict__ cp, const int64_t* __re
   }
 }
 
-// pass 1: hist[b * ntiles + t] = entries of tile t in row block b
-__global__ __launch_bounds__(kSvNT) void k_svb_count(const int64_t* __restrict__ ir, int64_t nnz, int nblk,
-                                                     int64_t ntiles, int64_t* __restrict__ hist) {
+// pass 1: hist[b * ntiles + t] = entries of tile t in row block b.  It also
+// does the per-entry half of the structure check: part[2t] counts the tile's
+// rows outside [0, nrows) (only counted, never used as an index) and
+// part[2t+1] its descents ir[e+1] <= ir[e]; k_svb_verdict sums them (one slot
+// per tile: thousands of workgroups adding to one counter would serialise).
+__global__ __launch_bounds__(kSvNT) void k_svb_count(const int64_t* __restrict__ ir, int64_t nnz, int64_t nrows,
+                                                     int nblk, int64_t ntiles, int64_t* __restrict__ hist,
+                                                     long long* __restrict__ part) {
   __shared__ int h[kSvMaxBlocks];
+  __shared__ long long ws[2][kSvNT / kWave];
   const int64_t t = blockIdx.x;
   for (int b = threadIdx.x; b < nblk; b += kSvNT) h[b] = 0;
   __syncthreads();
   const int64_t e0 = t * kSvTile, e1 = min<int64_t>(nnz, e0 + kSvTile);
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += kSvNT) atomicAdd(&h[ir[e] >> kSvLB], 1);
-  __syncthreads();
+  long long bad = 0, desc = 0;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += kSvNT) {
+    const int64_t r = ir[e];
+    if (r < 0 || r >= nrows) ++bad;
+    else atomicAdd(&h[r >> kSvLB], 1);
+    if (e + 1 < nnz) desc += ir[e + 1] <= r;
+  }
+  bad = wave_sum_ll(bad);
+  desc = wave_sum_ll(desc);
+  if (lane_id() == 0) {
+    ws[0][wave_id()] = bad;
+    ws[1][wave_id()] = desc;
+  }
+  __syncthreads();  // (also orders the histogram before it is read)
+  if (threadIdx.x == 0) {
+    long long a = 0, d = 0;
+    for (int w = 0; w < kSvNT / kWave; ++w) {
+      a += ws[0][w];
+      d += ws[1][w];
+    }
+    part[2 * t] = a;
+    part[2 * t + 1] = d;
+  }
   for (int b = threadIdx.x; b < nblk; b += kSvNT) hist[(int64_t)b * ntiles + t] = h[b];
+}
+
+// The structure check's verdict: *malformed = 1 when a column violates the
+// rules (cnt[0], from k_check_columns), a row is out of range, or the
+// descents differ from those at column starts (cnt[2]).  One workgroup.
+__global__ __launch_bounds__(kSvNT) void k_svb_verdict(const unsigned long long* __restrict__ cnt,
+                                                       const long long* __restrict__ part, int64_t ntiles,
+                                                       int* __restrict__ malformed) {
+  __shared__ long long ws[2][kSvNT / kWave];
+  long long bad = 0, desc = 0;
+  for (int64_t t = threadIdx.x; t < ntiles; t += kSvNT) {
+    bad += part[2 * t];
+    desc += part[2 * t + 1];
+  }
+  bad = wave_sum_ll(bad);
+  desc = wave_sum_ll(desc);
+  if (lane_id() == 0) {
+    ws[0][wave_id()] = bad;
+    ws[1][wave_id()] = desc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long a = 0, d = 0;
+    for (int w = 0; w < kSvNT / kWave; ++w) {
+      a += ws[0][w];
+      d += ws[1][w];
+    }
+    *malformed = (cnt[0] != 0 || a != 0 || (unsigned long long)d != cnt[2]) ? 1 : 0;
+  }
 }
 
 // largest q in [lo, hi] with c[q] <= e (c ascending, c[lo] <= e)
@@ -959,11 +1015,12 @@ __global__ __launch_bounds__(kSvNT) void k_svb_scatter(const int64_t* __restrict
                                                        const void* __restrict__ valA_, int64_t nzc, int64_t nnz,
                                                        const T* __restrict__ x, int nblk, int64_t ntiles,
                                                        const int64_t* __restrict__ offs, uint16_t* __restrict__ brow,
-                                                       T* __restrict__ bval) {
+                                                       T* __restrict__ bval, const int* __restrict__ malformed) {
   using SR = Sem<K, T>;
   __shared__ int h[kSvMaxBlocks];
   __shared__ int64_t scp[kSvTile + 1];  // cp of the columns the tile spans (columns are non-empty)
   __shared__ int64_t s_q[2];
+  if (*malformed) return;  // the structure check failed: cp, jc or ir cannot be trusted as indices
   const int64_t t = blockIdx.x;
   const int64_t e0 = t * kSvTile, e1 = min<int64_t>(nnz, e0 + kSvTile);
   for (int b = threadIdx.x; b < nblk; b += kSvNT) h[b] = 0;
@@ -1024,10 +1081,12 @@ __global__ __launch_bounds__(kSvNT) void k_svb_accumulate(const int64_t* __restr
                                                           const int64_t* __restrict__ cstart, int nblk,
                                                           int64_t ntiles, int64_t nnz, int64_t m,
                                                           const uint16_t* __restrict__ brow,
-                                                          const T* __restrict__ bval, T* __restrict__ y) {
+                                                          const T* __restrict__ bval, T* __restrict__ y,
+                                                          const int* __restrict__ malformed) {
   using SR = Sem<K, T>;
   using Acc = typename SR::Acc;
   __shared__ Acc acc[kSvBR];
+  if (*malformed) return;  // y is left as it was
   const int64_t g = blockIdx.x;
   if (g >= cstart[nblk]) return;
   const int b = (int)last_le(cstart, 0, nblk - 1, g);  // blocks without chunks share cstart with the next
@@ -1057,6 +1116,14 @@ static cbx_status spmv_dense_typed(const cbx_dcsc* A, bool boolA, const T* x, T*
   const int64_t nnz = A->nnz, m = A->m;
   const int64_t nblk64 = (m + kSvBR - 1) / kSvBR;
   if (nnz < kSvMinNnz || nblk64 > kSvMaxBlocks) {
+    {  // the structure check first: the kernel indexes y by A's rows and x by its columns
+      ScratchSet cs(s);
+      unsigned long long* chk = nullptr;
+      if (!cs.take(&chk, 3)) return set_error(CBX_ERR_NOMEM, "cbx_spmv_dense: workspace allocation failed");
+      dcsc_check_launch(A, chk, s);
+      const cbx_status cst = dcsc_check_result(chk, s);
+      if (cst != CBX_OK) return cst;
+    }
     if (boolA)
       k_spmv_dense<K, T, true><<<wave_grid(A->nzc), 256, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, x, y);
     else
@@ -1075,31 +1142,44 @@ static cbx_status spmv_dense_typed(const cbx_dcsc* A, bool boolA, const T* x, T*
   uint16_t* brow = nullptr;
   T* bval = nullptr;
   void* tmp = nullptr;
+  unsigned long long* chk = nullptr;  // structure-check counters, then the device-side verdict
+  long long* part = nullptr;          // per-tile counts of the per-entry check
   size_t tb = 0;
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hist, offs, (int)nh, s));
   ScratchSet ws(s);
   auto cleanup = [&]() { ws.release(); };
   if (!ws.take(&hist, nh) || !ws.take(&offs, nh) || !ws.take(&cstart, nblk + 1) || !ws.take(&brow, nnz) ||
-      !ws.take(&bval, nnz) || !ws.take_bytes(&tmp, std::max<size_t>(tb, 16))) {
+      !ws.take(&bval, nnz) || !ws.take_bytes(&tmp, std::max<size_t>(tb, 16)) || !ws.take(&chk, 4) ||
+      !ws.take(&part, 2 * ntiles)) {
     cleanup();
     return set_error(CBX_ERR_NOMEM, "cbx_spmv_dense: workspace allocation failed");
   }
-  k_svb_count<<<(unsigned)ntiles, kSvNT, 0, s>>>(A->ir, nnz, nblk, ntiles, hist);
+  int* malformed = (int*)(chk + 3);
+  // the structure check (the rules of cbx_dcsc_check) runs inside the
+  // launches: its per-column half here, the per-entry half in the counting
+  // pass, and the kernels that index by the structure wait for the verdict
+  dcsc_check_columns_launch(A, chk, s);
+  k_svb_count<<<(unsigned)ntiles, kSvNT, 0, s>>>(A->ir, nnz, m, nblk, ntiles, hist, part);
+  k_svb_verdict<<<1, kSvNT, 0, s>>>(chk, part, ntiles, malformed);
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, hist, offs, (int)nh, s);
   if (e == hipSuccess) {
     if (boolA)
       k_svb_scatter<K, T, true><<<(unsigned)ntiles, kSvNT, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, nnz, x, nblk,
-                                                                 ntiles, offs, brow, bval);
+                                                                 ntiles, offs, brow, bval, malformed);
     else
       k_svb_scatter<K, T, false><<<(unsigned)ntiles, kSvNT, 0, s>>>(A->cp, A->jc, A->ir, A->val, A->nzc, nnz, x,
-                                                                  nblk, ntiles, offs, brow, bval);
+                                                                  nblk, ntiles, offs, brow, bval, malformed);
     k_svb_chunks<<<1, kSvChunkNT, 0, s>>>(offs, nblk, ntiles, nnz, cstart);
-    k_svb_accumulate<K, T><<<(unsigned)max_chunks, kSvNT, 0, s>>>(offs, cstart, nblk, ntiles, nnz, m, brow, bval, y);
+    k_svb_accumulate<K, T><<<(unsigned)max_chunks, kSvNT, 0, s>>>(offs, cstart, nblk, ntiles, nnz, m, brow, bval, y,
+                                                                 malformed);
     e = hipGetLastError();
   }
+  int bad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, malformed, sizeof(int), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   cleanup();
   if (e != hipSuccess) return set_error(CBX_ERR_HIP, "cbx_spmv_dense: %s", hipGetErrorString(e));
+  if (bad) return set_error(CBX_ERR_INVALID, "cbx_dcsc_check: malformed DCSC");
   return CBX_OK;
 }
 
@@ -1217,8 +1297,10 @@ extern "C" cbx_status cbx_spmv_dense(const cbx_dcsc* A, const void* x, void* y, 
   const bool boolA = A->dtype == CBX_BOOL;
   CBX_RETURN_IF(!boolA && A->dtype != xdtype, CBX_ERR_UNSUPPORTED,
                 "cbx_spmv_dense: A must be boolean or share the vectors' dtype");
-  cbx_status st = cbx_dcsc_check(A);  // the kernel indexes y by A's rows and x by its columns
-  if (st != CBX_OK) return st;
+  // (the structure check, the rules of cbx_dcsc_check, runs with the product)
+  CBX_RETURN_IF(A->nzc == 0 && A->nnz != 0, CBX_ERR_INVALID, "cbx_dcsc_check: nnz without columns");
+  const cbx_status init = init_device_once();
+  if (init != CBX_OK) return init;
   hipStream_t s = (hipStream_t)stream;
   switch (semiring * 8 + xdtype) {
 #define CBX_SV(SRK, SRE, T, DT) \

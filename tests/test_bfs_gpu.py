@@ -166,6 +166,49 @@ def test_spmv_dense_bucketed_and_direct_forms(shape):
     assert np.allclose(got, oracle.spmv_dense(Af, xf, yf, "plus_times"), rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("shape", ["rmat_s16", "tall"])
+@pytest.mark.parametrize("fault", ["row_out_of_range", "rows_unsorted", "column_out_of_range", "cp_descends"])
+def test_spmv_dense_rejects_malformed_matrix(shape, fault):
+    """The structure check of dcsc_gespmv's operand runs with the product (the
+    bucketed form: its per-column half beside the launches, its per-entry half
+    in the counting pass, and the kernels that index through the structure
+    wait for the verdict; the direct form checks first).  A malformed matrix
+    is reported as CBX_ERR_INVALID and y is left as it was."""
+    rng = np.random.default_rng(23)
+    if shape == "rmat_s16":
+        A = cb.gen_rmat(16, 16, seed=3)
+    else:
+        m, n, k = (1 << 24) + 4096, 5000, 50_000
+        key = np.unique(rng.integers(0, n, k) * m + rng.integers(0, m, k))
+        A = cb.Dcsc.from_coo(m, n, key % m, key // m, np.ones(key.size, np.int64), "int64")
+    bad = cb.Dcsc(A.m, A.n, A.cp.copy(), A.jc.copy(), A.ir.copy(), A.val.copy())
+    q = int(np.argmax(np.diff(bad.cp) >= 2))  # a column with two entries or more
+    if fault == "row_out_of_range":
+        bad.ir[bad.cp[q + 1] - 1] = A.m  # the column's last row, one past the end
+    elif fault == "rows_unsorted":
+        b0 = bad.cp[q]
+        bad.ir[b0], bad.ir[b0 + 1] = bad.ir[b0 + 1], bad.ir[b0]
+    elif fault == "column_out_of_range":
+        bad.jc[-1] = A.n
+    else:
+        bad.cp[q + 1] = bad.cp[q]  # an empty stored column
+    dA = bad.to_device()
+    x = rng.integers(-20, 20, A.n).astype(np.int64)
+    y0 = rng.integers(-50, 50, A.m).astype(np.int64)
+    dx, dy = cb.DeviceArray.from_host(x), cb.DeviceArray.from_host(y0)
+    try:
+        with pytest.raises(cb.CbxError) as ei:
+            cb.spmv_dense_device(dA, dx, dy, "plus_times")
+        assert ei.value.status == cb.ERR_INVALID
+        assert np.array_equal(dy.to_host(), y0)
+        good = A.to_device()
+        cb.spmv_dense_device(good, dx, dy, "plus_times")  # the library is usable afterwards
+        assert np.array_equal(dy.to_host(), oracle.spmv_dense(A, x, y0, "plus_times"))
+    finally:
+        dx.release()
+        dy.release()
+
+
 @pytest.mark.parametrize("scale", [12, 16])
 def test_bfs_directions_give_identical_parents(scale):
     """Top-down only, bottom-up only and the per-level switch give the same
