@@ -366,15 +366,21 @@ def bench_merge(cb, scale: int = 17, ef: int = 16, nstages: int = 4, reps: int =
         parts.append(C)
         As.release()
         Bs.release()
-    M = cb.merge(parts)  # warm-up
-    M.release()
-    cb.synchronize()
-    t = time.perf_counter()
-    for i in range(reps):
-        M = cb.merge(parts)
-        if i + 1 < reps:
-            M.release()
-    cb.synchronize()
+    from combblas_amd import autotune
+    base = cb.get_config()
+    tuned = autotune.apply("merge_s17") is not None if (scale, ef, nstages) == (17, 16, 4) else False
+    try:
+        M = cb.merge(parts)  # warm-up
+        M.release()
+        cb.synchronize()
+        t = time.perf_counter()
+        for i in range(reps):
+            M = cb.merge(parts)
+            if i + 1 < reps:
+                M.release()
+        cb.synchronize()
+    finally:
+        cb.set_config(**base)
     ms = (time.perf_counter() - t) * 1e3 / reps
     exact = product_matches(M, A, A)
     nnz_in = sum(p.nnz for p in parts)
@@ -382,7 +388,7 @@ def bench_merge(cb, scale: int = 17, ef: int = 16, nstages: int = 4, reps: int =
     out = {"workload": f"stage merge of {nstages} SUMMA stage products of R-MAT s{scale} ef{ef} A*A, "
                        "PlusTimes<int64>, DCSC in HBM",
            "ms_per_merge": round(ms, 3), "nnz_in": nnz_in, "nnz_out": M.nnz, "alg_bytes": alg,
-           "gbps_algorithmic": round(alg / (ms * 1e-3) / 1e9, 1), "merged_equals_product": exact}
+           "gbps_algorithmic": round(alg / (ms * 1e-3) / 1e9, 1), "merged_equals_product": exact, "tuned": tuned}
     M.release()
     for p in parts:
         p.release()

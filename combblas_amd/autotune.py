@@ -66,6 +66,35 @@ def _galerkin_workload():
     return setup
 
 
+def _merge_workload(scale: int, ef: int, nstages: int):
+    """SUMMA's stage merge: the local products of nstages stages of R-MAT
+    A*A (computed once, on the configuration in force at setup), merged."""
+    def setup():
+        import numpy as np
+        A = cb.gen_rmat(scale, ef)
+        cuts = np.linspace(0, A.n, nstages + 1).astype(np.int64)
+        parts = []
+        for s in range(nstages):
+            As = A.column_range(int(cuts[s]), int(cuts[s + 1])).to_device()
+            Bs = A.row_range(int(cuts[s]), int(cuts[s + 1])).to_device()
+            C, _ = cb.spgemm(As, Bs)
+            parts.append(C)
+            As.release()
+            Bs.release()
+
+        def run():
+            M = cb.merge(parts)
+            nnz = M.nnz
+            M.release()
+            return nnz
+
+        def done():
+            for p in parts:
+                p.release()
+        return run, done
+    return setup
+
+
 def _mcl_workload(n: int, mode: str):
     def setup():
         dA = cb.gen_powerlaw_stochastic(n).to_device()
@@ -115,6 +144,14 @@ MCL_GRID = {
     "mcl_fuse": [0, 1],
     "unit_lists": [0, 1],
 }
+# the merge's product: each output column combines the stages' columns, so
+# its flops are the stages' column lengths summed
+MERGE_GRID = {
+    "hash_max_flop": [1024, 2048, 4096],
+    "hash_mid_flop": [1024],
+    "tiny_algo": [0, 1],
+    "unit_lists": [0, 1],
+}
 MCL_LEARNED_GRID = {
     "hash_max_flop": [2048, 4096],
     "learned_target": [0.75, 1.0],
@@ -129,6 +166,8 @@ WORKLOADS = {
     "rmat_s16_ef16": (_spgemm_workload(16, 16, "two_phase"), "R-MAT s16 ef16 A*A", "two_phase", GRID),
     "galerkin_256": (_galerkin_workload(), "256 batched Galerkin products S*(A*T), grid edges 16..96, FP64",
                      "two_phase", GALERKIN_GRID),
+    "merge_s17": (_merge_workload(17, 16, 4), "stage merge of 4 SUMMA stage products of R-MAT s17 ef16 A*A",
+                  "two_phase", MERGE_GRID),
     "mcl_n1m": (_mcl_workload(1_000_000, "two_phase"), "MCL expansion A*A + prune, power-law n=10^6, FP32",
                 "two_phase", MCL_GRID),
     "mcl_n1m_learned": (_mcl_workload(1_000_000, "learned"), "MCL expansion A*A + prune, power-law n=10^6, FP32",
