@@ -383,7 +383,7 @@ __device__ __forceinline__ void bfs_add_pieces(int32_t v, int64_t deg, int32_t* 
   for (int k = 0; k < np; ++k) pieces[b + k] = ((long long)v << 32) | k;
 }
 
-__global__ void k_bfs_reset(int64_t n, long long* __restrict__ cand, int64_t* __restrict__ parents,
+__global__ void k_bfs_reset(int64_t n, int32_t* __restrict__ cand, int64_t* __restrict__ parents,
                             uint32_t* __restrict__ visited, int64_t nw, int32_t* __restrict__ f0,
                             int32_t* __restrict__ counts, int32_t* __restrict__ pcount,
                             long long* __restrict__ pieces0, const int64_t* __restrict__ beg,
@@ -425,7 +425,7 @@ __global__ void k_bfs_level_begin(const int32_t* __restrict__ cin, int32_t* __re
 // largest offer in cand[r]; the first offer appends r to the next frontier
 // (one atomic per wave for the appends of this step).  All lanes call it.
 __device__ __forceinline__ void bfs_offer(int32_t r, int32_t k, const uint32_t* __restrict__ visited,
-                                          long long* __restrict__ cand, int32_t* __restrict__ fout,
+                                          int32_t* __restrict__ cand, int32_t* __restrict__ fout,
                                           int32_t* __restrict__ fout_count) {
   bool push = false;
   if (r >= 0 && !((visited[r >> 5] >> (r & 31)) & 1u)) {
@@ -434,9 +434,9 @@ __device__ __forceinline__ void bfs_offer(int32_t r, int32_t k, const uint32_t* 
     // this large makes the atomic a no-op: skip it.  Hubs receive thousands
     // of offers per level and only the record-breaking ones reach the
     // serialising atomic.
-    const long long seen = __hip_atomic_load(&cand[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (seen < (long long)k) {
-      const long long old = atomicMax(&cand[r], (long long)k);
+    const int32_t seen = __hip_atomic_load(&cand[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (seen < k) {
+      const int32_t old = atomicMax(&cand[r], k);
       push = old == -1;
     }
   }
@@ -457,7 +457,7 @@ template <int NT, int U>
 __global__ __launch_bounds__(NT) void k_bfs_expand(const int64_t* __restrict__ beg, const int64_t* __restrict__ end,
                                                    const int32_t* __restrict__ rows, const int32_t* __restrict__ fin,
                                                    const int32_t* __restrict__ fin_count,
-                                                   const uint32_t* __restrict__ visited, long long* __restrict__ cand,
+                                                   const uint32_t* __restrict__ visited, int32_t* __restrict__ cand,
                                                    int32_t* __restrict__ fout, int32_t* __restrict__ fout_count) {
   __shared__ SegStage<int32_t, NT> st;
   __shared__ long long wsum[NT / kWave];
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(NT) void k_bfs_expand_hubs(const int64_t* __restric
                                                         const long long* __restrict__ pieces,
                                                         const int32_t* __restrict__ pcount,
                                                         const uint32_t* __restrict__ visited,
-                                                        long long* __restrict__ cand, int32_t* __restrict__ fout,
+                                                        int32_t* __restrict__ cand, int32_t* __restrict__ fout,
                                                         int32_t* __restrict__ fout_count) {
   static_assert(kBfsHub % NT == 0, "whole rounds per piece");
   const int np = *pcount;
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(NT) void k_bfs_expand_hubs(const int64_t* __restric
 constexpr int kBfsFinNT = 256;
 __global__ __launch_bounds__(kBfsFinNT) void k_bfs_finalize(const int32_t* __restrict__ f,
                                                             const int32_t* __restrict__ count,
-                                                            const long long* __restrict__ cand,
+                                                            const int32_t* __restrict__ cand,
                                                             const int64_t* __restrict__ beg,
                                                             const int64_t* __restrict__ end,
                                                             int64_t* __restrict__ parents,
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(256) void k_bfs_bottom_up(const int64_t* __restrict
                                                        const int32_t* __restrict__ rows, int64_t n,
                                                        const uint32_t* __restrict__ fbits,
                                                        const uint32_t* __restrict__ visited,
-                                                       long long* __restrict__ cand, int32_t* __restrict__ fout,
+                                                       int32_t* __restrict__ cand, int32_t* __restrict__ fout,
                                                        int32_t* __restrict__ fout_count) {
   const int lane = lane_id();
   for (int64_t v0 = blockIdx.x * (int64_t)blockDim.x; v0 < n; v0 += (int64_t)gridDim.x * blockDim.x) {
@@ -644,7 +644,7 @@ struct cbx_bfs_plan_s {
   int32_t *rows = nullptr, *f[2] = {nullptr, nullptr}, *counts = nullptr;
   int32_t* pcount = nullptr;                  // hub pieces of the two frontier buffers
   long long* pieces[2] = {nullptr, nullptr};
-  long long* cand = nullptr;
+  int32_t* cand = nullptr;  // the largest offer per vertex (vertex ids fit 32 bits: n < 2^31)
   uint32_t* visited = nullptr;
   unsigned long long* edges = nullptr;  // degree sums of the two frontier buffers
   uint32_t* fbits = nullptr;             // frontier bitmap of a bottom-up level
@@ -747,7 +747,7 @@ extern "C" cbx_status cbx_bfs_plan_create(const cbx_dcsc* A, cbx_bfs_plan_s** ou
   if (hipMalloc(&p->beg, 8 * std::max<int64_t>(n, 1)) || hipMalloc(&p->end, 8 * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->parents, 8 * std::max<int64_t>(n, 1)) || hipMalloc(&p->rows, 4 * std::max<int64_t>(p->nnz, 1)) ||
       hipMalloc(&p->f[0], 4 * std::max<int64_t>(n, 1)) || hipMalloc(&p->f[1], 4 * std::max<int64_t>(n, 1)) ||
-      hipMalloc(&p->counts, 16) || hipMalloc(&p->cand, 8 * std::max<int64_t>(n, 1)) ||
+      hipMalloc(&p->counts, 16) || hipMalloc(&p->cand, sizeof(int32_t) * std::max<int64_t>(n, 1)) ||
       hipMalloc(&p->visited, 4 * p->nw) || hipMalloc(&p->edges, 2 * sizeof(unsigned long long)) ||
       hipMalloc(&p->fbits, 4 * p->nw) || hipMalloc(&p->stat, 2 * sizeof(unsigned long long)) ||
       hipMalloc(&p->pcount, 2 * sizeof(int32_t)) || hipMalloc(&p->pieces[0], 8 * max_pieces) ||
