@@ -459,9 +459,10 @@ cbx_status cbx_dcsc_transpose(const cbx_dcsc* X, cbx_dcsc* out, void* stream);
  * (SpDCCols.cpp:109-184).  out = the sum under the semiring's add of count
  * device DCSC parts of one shape and dtype (an array of count structs), as a
  * new device DCSC; entries with equal (row, col) are combined, none is
- * dropped.  Integer semirings are exact; floating-point sums of duplicates
- * are formed in part order (the reference's heap order may round
- * differently).  At most 2^31 - 1 entries over all parts. */
+ * dropped.  Integer, min-plus and select-max results are exact; floating-
+ * point plus-times sums of duplicates are formed in the product's
+ * accumulation order (the reference's heap order may round differently).
+ * The parts are validated as cbx_dcsc_check does (CBX_ERR_INVALID). */
 cbx_status cbx_dcsc_merge(int32_t count, const cbx_dcsc* parts, cbx_semiring semiring, cbx_dcsc* out, void* stream);
 
 /* Edge list to matrix on the device: the Graph500 constructor SpTuples(maxnnz,
