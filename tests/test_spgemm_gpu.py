@@ -324,16 +324,18 @@ def _spread(m, n, k, seed, dtype="int64"):
     return A.astype(dtype), B.astype(dtype)
 
 
-@pytest.mark.parametrize("m", [300_007, (1 << 21) + 5000])
+@pytest.mark.parametrize("m", [200_003, 300_007, (1 << 21) + 5000])
 @pytest.mark.parametrize("semiring,dtype", [("plus_times", "int64"), ("min_plus", "float64"),
                                             ("select_max", "float32"), ("plus_times", "bool")])
 def test_small_units_by_row_list(m, semiring, dtype):
     """unit_lists: the symbolic pass writes the sorted rows of the small-unit
     columns, cuts their units by output count over any number of row blocks,
     and those wider than 16 blocks take their ranks from the list
-    (k_num_list).  Thin columns over every block of A's rows, in one 74-block
-    pass and in three 256-block passes: fewer units than the 16-block cut,
-    and the same product as without lists and as the oracle, bit for bit."""
+    (k_num_list).  Thin columns over every block of A's rows: 49 blocks (the
+    symbolic pass's single-sub-pass form), one 74-block pass in two
+    sub-passes, and three 256-block passes: fewer units than the 16-block
+    cut, and the same product as without lists and as the oracle, bit for
+    bit, in both modes."""
     A, B = _spread(m, 2000, 300, 43, dtype)
     base = cb.get_config()
     P = np.zeros(cb.MLP_PARAMS, np.float32)
