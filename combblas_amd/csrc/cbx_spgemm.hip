@@ -1125,8 +1125,15 @@ __global__ __launch_bounds__(NT) void k_num_dense(const int32_t* __restrict__ or
 // rows fit the region are then cut by output count alone over runs of such
 // blocks: those wider than rblk_l blocks go to units_s, whose numeric pass
 // takes ranks from the list instead of a bitmap.
+//
+// Eight waves per SIMD: the pass is latency-bound on its gathers, so the
+// kernel asks for 64 VGPRs (six of its values spill to scratch) and keeps
+// its LDS under 40 KB (the unit staging lives in the bitmap, idle while a
+// pass is cut), four workgroups per CU.  At the compiler's own choice (80
+// VGPRs, 47 KB: six waves) the pass took 64 ms at R-MAT s20, here 55.7 ms.
 template <int NT, int U, int IBLK>
-__global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ order, int64_t first,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sym_units(
+    const int32_t* __restrict__ order, int64_t first,
                                                   const int64_t* __restrict__ cpB, const int32_t* __restrict__ bpos,
                                                   const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                                                   const int32_t* __restrict__ P, int S, int kmax_l, int rblk_l,
@@ -1147,10 +1154,14 @@ __global__ __launch_bounds__(NT) void k_sym_units(const int32_t* __restrict__ or
   static_assert(sizeof(SegStage<uint8_t, NT>) >= BPP * sizeof(Unit), "list units are staged in the gather stage");
   __shared__ uint32_t bits[BMW];
   __shared__ int bpre[BPP];           // block counts, then their inclusive prefix
-  __shared__ Unit su[BPP];
   __shared__ SegStage<uint8_t, NT> st;
   __shared__ long long wsum[NT / kWave];
   __shared__ int s_nu, s_slot, s_ns, s_sslot;
+  // the units of a pass, staged while it is cut: in the bitmap, which every
+  // thread has finished reading (barrier) and the next pass clears only after
+  // the end-of-pass barrier
+  static_assert(sizeof(uint32_t) * BMW >= BPP * sizeof(Unit), "the bitmap holds a pass's units");
+  Unit* su = reinterpret_cast<Unit*>(bits);
   __shared__ unsigned long long s_dense[BPP / kWave];  // blocks of the pass above kListBlockMax outputs
   Unit* su2 = reinterpret_cast<Unit*>(&st);  // list units of a pass (the stage is idle while cutting)
   const int col = order[first + blockIdx.x];
