@@ -1572,8 +1572,19 @@ __device__ __forceinline__ void unit_emit(bool direct, int ninit, int cnt, int n
 // accumulates every product at its row's rank.  When the column's B entries
 // fit one chunk, the first CR rounds of products (row, product value) stay in
 // registers between the sweeps, so sweep 2 re-reads nothing for them.
+// Waves per SIMD the small-unit kernel asks for: eight where its LDS allows
+// four 512-thread workgroups per CU (a value window of at most 16 KB: 4-byte
+// accumulators), which then caps it at 64 VGPRs; otherwise the compiler's
+// choice (the LDS limits 8-byte windows to six waves anyway).
+template <class Acc, int NT, int KMAX>
+struct UnitWaves {
+  static constexpr int value = (NT == 512 && sizeof(Acc) * KMAX <= 16384) ? 8 : 1;
+};
+
 template <int K, class T, int NT, int U, int KMAX, int RBLK, int CR>
-__global__ __launch_bounds__(NT) void k_num_units(const Unit* __restrict__ units, const int64_t* __restrict__ cpB,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
+    UnitWaves<typename Sem<K, T>::Acc, NT, KMAX>::value))) void k_num_units(const Unit* __restrict__ units,
+                                                                            const int64_t* __restrict__ cpB,
                                                   const int32_t* __restrict__ bpos, const T* __restrict__ valB,
                                                   const int64_t* __restrict__ cpA, const int32_t* __restrict__ rowsA,
                                                   const T* __restrict__ valA, const int32_t* __restrict__ P, int S,
@@ -2644,6 +2655,13 @@ constexpr int kSymBandNT = 256;                     // banded symbolic pass
 // light units span at most 16 blocks: ~53 KB of LDS, three workgroups per CU
 constexpr int kLightNT = 512, kLightK = 4096, kLightRB = 16;    // <= 4096 outputs, <= 64K rows
 constexpr int kHeavyNT = 1024, kHeavyK = 16384, kHeavyRB = 16;  // <= 16384 outputs, <= 64K rows
+// The large units' value window: 16384 outputs, or 12288 for 4-byte
+// accumulators, whose large-unit kernel then takes about 76 KB of LDS
+// instead of 92 KB: two workgroups per CU, eight waves per SIMD, not four.
+template <int K, class T>
+constexpr int heavy_k() {
+  return sizeof(typename Sem<K, T>::Acc) <= 4 ? 12288 : kHeavyK;
+}
 // learned units: one workgroup shape for all (512 threads, 4096-entry value
 // window, 32 row blocks), cut to target * 4096 predicted outputs
 constexpr int kLearnNT = 512, kLearnK = 4096, kLearnRB = 32, kLearnCR = 2;
@@ -2742,11 +2760,11 @@ static cbx_status alloc_unit_lists(const int32_t* order, const int64_t* flop, in
 cbx_status launch_sym_units(const SpgemmConfig& cfg, const int32_t* order, int64_t ncols, const cbx_dcsc* A,
                             const cbx_dcsc* B, const int32_t* bpos, const int32_t* rowsA, const int32_t* P, int S,
                             int64_t* nnzcol, Unit* units_l, int* n_l, int cap_l, Unit* units_h, int* n_h, int cap_h,
-                            uint32_t* rmap, int64_t rm_cols, int* err, DevBuf& bcnt_buf, hipStream_t s,
+                            uint32_t* rmap, int64_t rm_cols, int* err, DevBuf& bcnt_buf, int kmax_h, hipStream_t s,
                             const UnitLists& ul = {}) {
   if (cfg.sym_band == 0) {
     k_sym_units<kSymUnitNT, 4, kSymUnitIBLK><<<(unsigned)ncols, kSymUnitNT, 0, s>>>(
-        order, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kHeavyK, kHeavyRB, cfg.unit_split_nnz, nnzcol,
+        order, 0, B->cp, bpos, A->cp, rowsA, P, S, kLightK, kLightRB, kmax_h, kHeavyRB, cfg.unit_split_nnz, nnzcol,
         units_l, n_l, cap_l, units_h, n_h, cap_h, rmap, rm_cols, err, ul.lists, ul.off, ul.bcnt, ul.units, ul.n,
         ul.cap);
     return CBX_OK;
@@ -2761,7 +2779,7 @@ cbx_status launch_sym_units(const SpgemmConfig& cfg, const int32_t* order, int64
   else
     k_sym_band<kSymBandNT, 4, 64><<<g, kSymBandNT, 0, s>>>(order, 0, B->cp, bpos, A->cp, rowsA, P, S, bcnt, rmap,
                                                           rm_cols);
-  k_sym_cut<<<(unsigned)ncols, 256, 0, s>>>(order, 0, bcnt, S, kLightK, kLightRB, kHeavyK, kHeavyRB,
+  k_sym_cut<<<(unsigned)ncols, 256, 0, s>>>(order, 0, bcnt, S, kLightK, kLightRB, kmax_h, kHeavyRB,
                                             cfg.unit_split_nnz, nnzcol, units_l, n_l, cap_l, units_h, n_h, cap_h, err);
   return CBX_OK;
 }
@@ -2903,7 +2921,7 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
       CBX_TRY(alloc_unit_lists(hcols, flop, nheavy, S, cap_l, std::min<int64_t>(cfg.unit_split_nnz, m), hucnt + 2, tmp,
                                tmp_bytes, b_lsz, b_loff, b_lists, b_lbc, b_us, s, ul));
     CBX_TRY(launch_sym_units(cfg, hcols, nheavy, A, B, bpos, rowsA, P, S, nnzcol, units_l, hucnt, (int)cap_l, units_h,
-                             hucnt + 1, (int)cap_h, rm.p, rm_cols, err, b_bcnt, s, ul));
+                             hucnt + 1, (int)cap_h, rm.p, rm_cols, err, b_bcnt, heavy_k<K, T>(), s, ul));
     CBX_HIP(hipGetLastError());
   }
   CBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nnzcol, hcs, (int)(nzcB + 1), s));
@@ -2994,11 +3012,11 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
     if (hu[2] > 1) CBX_TRY(sort_units_by_block(ul.units, hu[2], S, s, b_us_sorted, &ul.units));
   }
   if (hu[1] && rm.p)
-    k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+    k_num_units_rm<K, T, kHeavyNT, 4, heavy_k<K, T>(), kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
         units_h, rm.p, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, rm_cols, cpC, C->ir,
         (T*)C->val, err, Filter<T>{}, cfg.unit_xcd_group);
   else if (hu[1])
-    k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+    k_num_units<K, T, kHeavyNT, 4, heavy_k<K, T>(), kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
         units_h, B->cp, bpos, (const T*)B->val, A->cp, rowsA, (const T*)A->val, P, S, cpC, C->ir, (T*)C->val, err,
         Filter<T>{}, cfg.unit_xcd_group);
   if (hu[0])
@@ -3332,7 +3350,7 @@ static cbx_status spgemm_typed_once(const cbx_dcsc* A, const cbx_dcsc* B, cbx_sp
       if (ul.lists && lists_used) *lists_used = true;
     }
     CBX_TRY(launch_sym_units(cfg, order_s, n_dense, A, B, bpos, rowsA, P, S, nnzcol, units_l, ucnt, (int)cap_l, units_h,
-                             ucnt + 1, (int)cap_h, rmap, rm_cols, err, b_bcnt, s, ul));
+                             ucnt + 1, (int)cap_h, rmap, rm_cols, err, b_bcnt, heavy_k<K, T>(), s, ul));
   } else if (n_dense) {
     k_sym_dense<kSymNT, kSymRBW><<<(unsigned)n_dense, kSymNT, 0, s>>>(order_s, 0, B->cp, bpos, A->cp, rowsA, P, S, G,
                                                                        TR, nnzcol);
@@ -3433,11 +3451,11 @@ static cbx_status spgemm_typed_once(const cbx_dcsc* A, const cbx_dcsc* B, cbx_sp
     // units keep the two-sweep kernel, whose second sweep mostly replays
     // registers
     if (hu[1] && rmap)
-      k_num_units_rm<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+      k_num_units_rm<K, T, kHeavyNT, 4, heavy_k<K, T>(), kHeavyRB><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
           units_h, rmap, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, rm_cols, cpC, C->ir, (T*)C->val, err, flt,
           cfg.unit_xcd_group);
     else if (hu[1])
-      k_num_units<K, T, kHeavyNT, 4, kHeavyK, kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
+      k_num_units<K, T, kHeavyNT, 4, heavy_k<K, T>(), kHeavyRB, 4><<<(unsigned)hu[1], kHeavyNT, 0, s>>>(
           units_h, B->cp, bpos, valB, A->cp, rowsA, valA, P, S, cpC, C->ir, (T*)C->val, err, flt, cfg.unit_xcd_group);
     if (hu[0])
       k_num_units<K, T, kLightNT, 4, kLightK, kLightRB, 2><<<(unsigned)hu[0], kLightNT, 0, s>>>(

@@ -229,6 +229,30 @@ def test_large_units_with_and_without_row_maps(split, share):
 
 
 @pytest.mark.parametrize("share", [0.25, 0.0])
+@pytest.mark.parametrize("dtype", ["float32", "bool"])
+def test_large_units_with_4_byte_accumulators(share, dtype):
+    """Large units of 4-byte accumulators (FP32, bool) have a 12288-output
+    value window (16384 for 8-byte ones): split lowered to 4096 so most
+    columns take them, with and without row maps.  FP32 values k/16, k in
+    {1, 2}: products are multiples of 2^-8 and a column's at most 660,596 of
+    them sum below 2^14, exact in 22 bits, so the product equals the
+    oracle's bit for bit."""
+    A = cb.gen_rmat(16, 16)
+    rng = np.random.default_rng(5)
+    A.val = (rng.integers(1, 3, A.nnz) / 16.0).astype(np.float32) if dtype == "float32" else np.ones(A.nnz, np.uint8)
+    A = cb.Dcsc(A.m, A.n, A.cp, A.jc, A.ir, A.val, dtype)
+    base = cb.get_config()
+    try:
+        cb.set_config(unit_split_nnz=4096, row_map_share=share)
+        got, st = gpu_spgemm(A, A)
+    finally:
+        cb.set_config(**base)
+    want, _ = oracle.spgemm(A, A)
+    assert st["units_heavy"] > 0
+    bit_exact(got, want)
+
+
+@pytest.mark.parametrize("share", [0.25, 0.0])
 def test_large_units_on_tall_matrices(share):
     """m above 2^20 rows: the symbolic pass runs several bitmap passes and
     writes the row map pass by pass; large units (split lowered to 4096) read
