@@ -126,8 +126,9 @@ LEARNED_GRID = {
     "tiny_flop": [256],
     "unit_split_nnz": [131072],
     "learned_target": [0.75, 0.9],
-    "heavy_nnz": [8192, 16384, 32768],
+    "heavy_nnz": [4096, 8192, 16384],
     "sym_band": [0, 16],
+    "unit_lists": [0, 1],
 }
 # Galerkin columns hold 216 (A*T) and 64 (S*(A*T)) products: the tiny tier's
 # bound, form and workgroup shape decide where they run

@@ -186,7 +186,10 @@ extern "C" cbx_status cbx_device_name(char* buf, size_t len) {
   hipDeviceProp_t prop;
   CBX_HIP(hipGetDevice(&dev));
   CBX_HIP(hipGetDeviceProperties(&prop, dev));
-  std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+  if (prop.name[0])
+    std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+  else  // some runtimes report no marketing name
+    std::snprintf(buf, len, "%s", prop.gcnArchName);
   return CBX_OK;
 }
 

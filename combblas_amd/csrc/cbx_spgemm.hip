@@ -2602,6 +2602,32 @@ __global__ void k_unit_permute(const Unit* __restrict__ src, const int32_t* __re
 }
 
 // A copy of units[0, n) sorted by blk_lo (stable), kept in `keep`.
+__global__ void k_gather_flop(const int32_t* __restrict__ cols, int64_t n, const int64_t* __restrict__ flop,
+                              int64_t* __restrict__ key) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) key[i] = flop[cols[i]];
+}
+
+// Reorders cols[0..n) by descending flop (into `keep`, returned in *out), so a
+// per-column launch over them starts its longest columns first, as the
+// two-phase path's flop-sorted order does.
+cbx_status sort_cols_by_flop(const int32_t* cols, int64_t n, const int64_t* flop, hipStream_t s, DevBuf& keep,
+                             int32_t** out) {
+  DevBuf b_k, b_k2, b_t;
+  int64_t *k1, *k2;
+  uint8_t* t;
+  CBX_TRY(dalloc(b_k, n, s, &k1));
+  CBX_TRY(dalloc(b_k2, n, s, &k2));
+  CBX_TRY(dalloc(keep, n, s, out));
+  size_t tb = 0;
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, k1, k2, cols, *out, (int)n, 0, 64, s));
+  CBX_TRY(dalloc(b_t, std::max<size_t>(tb, 16), s, &t));
+  k_gather_flop<<<nblk(n, 256), 256, 0, s>>>(cols, n, flop, k1);
+  CBX_HIP(hipcub::DeviceRadixSort::SortPairsDescending(t, tb, k1, k2, cols, *out, (int)n, 0, 64, s));
+  CBX_HIP(hipGetLastError());
+  return CBX_OK;
+}
+
 cbx_status sort_units_by_block(const Unit* units, int n, int S, hipStream_t s, DevBuf& keep, Unit** out) {
   DevBuf b_k, b_k2, b_i, b_i2, b_t;
   int32_t *k1, *k2, *i1, *i2;
@@ -2901,6 +2927,12 @@ static cbx_status learned_numeric(const cbx_dcsc* A, const cbx_dcsc* B, const Sp
   RowMap rm;
   rm.s = s;
   int64_t rm_cols = 0;
+  DevBuf b_hsorted;
+  if (nheavy > 1) {  // longest columns first
+    int32_t* hs = nullptr;
+    CBX_TRY(sort_cols_by_flop(hcols, nheavy, flop, s, b_hsorted, &hs));
+    hcols = hs;
+  }
   if (nheavy) {
     cap_l = std::max<int64_t>(1, std::min<int64_t>(nheavy * (int64_t)S, 1ll << 30));
     cap_h = cap_l;
