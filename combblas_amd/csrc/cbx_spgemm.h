@@ -94,9 +94,11 @@ size_t cache_idle_bytes();  // bytes held by the cache and free for reuse
 // Scratch of one call on stream s.  Blocks of kScratchCacheMin bytes or more
 // come from the caching allocator above (plain hipMalloc on a miss, which is
 // fast); smaller ones from the stream-ordered pool.  Growing the pool by
-// gigabytes has taken up to 1.5 s per allocation, so large scratch stays out
-// of it.  scratch_free waits for s before it caches a large block.
-constexpr size_t kScratchCacheMin = (size_t)64 << 20;
+// gigabytes has taken up to 1.5 s per allocation, and hipFreeAsync of pool
+// blocks of a few megabytes 2-4 ms of host time each (7 ms per R-MAT s20
+// product at a 64 MB bound), so all but the smallest scratch stays out of
+// it.  scratch_free waits for s before it caches a block.
+constexpr size_t kScratchCacheMin = (size_t)1 << 20;
 cbx_status scratch_alloc(void** p, size_t bytes, hipStream_t s);
 void scratch_free(void* p, size_t bytes, hipStream_t s);
 
