@@ -103,15 +103,33 @@ cbx_status scratch_alloc(void** p, size_t bytes, hipStream_t s);
 void scratch_free(void* p, size_t bytes, hipStream_t s);
 
 // The scratch blocks of one call, released together (at the latest when the
-// set goes out of scope).
+// set goes out of scope).  Requests of up to 64 KB are carved (256-byte
+// aligned) from 1 MB chunks of the caching allocator: counters, verdicts and
+// scan storage then cost no stream-pool call each (a pool free has taken
+// 10-20 us of host time, several per call).
 struct ScratchSet {
+  static constexpr size_t kSmall = (size_t)64 << 10, kChunk = (size_t)1 << 20;
   hipStream_t s;
   std::vector<std::pair<void*, size_t>> blocks;
+  uint8_t* chunk = nullptr;
+  size_t chunk_used = 0;
   explicit ScratchSet(hipStream_t st) : s(st) {}
   ScratchSet(const ScratchSet&) = delete;
   ScratchSet& operator=(const ScratchSet&) = delete;
   bool take_bytes(void** p, size_t bytes) {
-    bytes = bytes < 256 ? 256 : bytes;
+    bytes = bytes < 256 ? 256 : (bytes + 255) & ~(size_t)255;
+    if (bytes <= kSmall) {
+      if (!chunk || chunk_used + bytes > kChunk) {
+        void* c = nullptr;
+        if (scratch_alloc(&c, kChunk, s) != CBX_OK) return false;
+        blocks.emplace_back(c, kChunk);
+        chunk = (uint8_t*)c;
+        chunk_used = 0;
+      }
+      *p = chunk + chunk_used;
+      chunk_used += bytes;
+      return true;
+    }
     if (scratch_alloc(p, bytes, s) != CBX_OK) return false;
     blocks.emplace_back(*p, bytes);
     return true;
@@ -126,6 +144,8 @@ struct ScratchSet {
   void release() {
     for (auto& b : blocks) scratch_free(b.first, b.second, s);
     blocks.clear();
+    chunk = nullptr;
+    chunk_used = 0;
   }
   ~ScratchSet() { release(); }
 };
