@@ -46,6 +46,7 @@ struct Seg {
   const void* val;
   int64_t row_off, col_off;  // added to rows / column ids
   int64_t nz_off, c_off;     // first entry / first stored column in the packed matrix
+  int64_t m, n, nnz, nzc;    // the member's own shape and sizes (for its bounds)
 };
 
 struct OutSeg {
@@ -82,19 +83,36 @@ __global__ __launch_bounds__(256) void k_pack_entries(const Seg* __restrict__ se
   }
 }
 
+// Packs the column arrays and checks each member against its own bounds,
+// which the packed matrix's validation cannot see (a row or column id that
+// strays into the next member's block is in range there): column ids in
+// [0, n_i), cp starting at 0 and closing at the member's nnz, and every
+// non-empty column's first and last rows in [0, m_i) — all its rows, since
+// the product's validation of the packed matrix checks that rows ascend
+// within columns.  bad[0] counts violations.
 __global__ __launch_bounds__(256) void k_pack_cols(const Seg* __restrict__ segs, int count, int64_t total,
                                                    int64_t total_nnz, int64_t* __restrict__ cp,
-                                                   int64_t* __restrict__ jc) {
+                                                   int64_t* __restrict__ jc, unsigned long long* __restrict__ bad) {
   __shared__ int64_t off[kMaxBatch];
   for (int s = threadIdx.x; s < count; s += blockDim.x) off[s] = segs[s].c_off;
   __syncthreads();
+  long long nbad = 0;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
     const int s = find_seg(off, count, q);
     const int64_t i = q - off[s];
-    cp[q] = segs[s].cp[i] + segs[s].nz_off;
-    jc[q] = segs[s].jc[i] + segs[s].col_off;
+    const Seg& g = segs[s];
+    const int64_t b = g.cp[i], e = g.cp[i + 1], j = g.jc[i];
+    bool err = j < 0 || j >= g.n || (i == 0 && b != 0) || (i + 1 == g.nzc && e != g.nnz);
+    if (!err && 0 <= b && b < e && e <= g.nnz) {  // a non-empty column inside the member's entries
+      const int64_t r0 = g.ir[b], r1 = g.ir[e - 1];
+      err = r0 < 0 || r0 >= g.m || r1 < 0 || r1 >= g.m;
+    }
+    nbad += err;
+    cp[q] = b + g.nz_off;
+    jc[q] = j + g.col_off;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) cp[total] = total_nnz;
+  block_add_counts<256>(nbad, 0, &bad[0], &bad[1]);
 }
 
 // qb[s] = first stored column of the packed matrix with column id >= bound[s].
@@ -234,30 +252,40 @@ cbx_status launch_unpack_entries(const OutSeg* segs, int count, int64_t total, c
 }  // namespace
 
 // Packs X[0..count) into one block-diagonal DCSC (rows and columns offset by
-// the running sums of m and n).
+// the running sums of m and n), checking each member against its own bounds
+// (k_pack_cols); the product validates the packed matrix as any operand.
 static cbx_status pack(const cbx_dcsc* X, int64_t count, cbx_dcsc* out, hipStream_t s) {
   std::vector<Seg> segs(count);
   int64_t m = 0, n = 0, nnz = 0, nzc = 0;
   for (int64_t i = 0; i < count; ++i) {
-    segs[i] = Seg{X[i].cp, X[i].jc, X[i].ir, X[i].val, m, n, nnz, nzc};
+    segs[i] = Seg{X[i].cp, X[i].jc, X[i].ir, X[i].val, m, n, nnz, nzc, X[i].m, X[i].n, X[i].nnz, X[i].nzc};
     m += X[i].m;
     n += X[i].n;
     nnz += X[i].nnz;
     nzc += X[i].nzc;
   }
   CBX_TRY(alloc_dev_dcsc(out, m, n, nnz, nzc, X[0].dtype));
-  DevMem dsegs;
+  DevMem dsegs, dbad;
   Seg* d;
+  unsigned long long* bad;
   CBX_TRY(dsegs.get(&d, count));
+  CBX_TRY(dbad.get(&bad, 2));
   CBX_HIP(hipMemcpyAsync(d, segs.data(), sizeof(Seg) * count, hipMemcpyHostToDevice, s));
-  k_pack_cols<<<grid_for(nzc), 256, 0, s>>>(d, (int)count, nzc, nnz, out->cp, out->jc);
+  CBX_HIP(hipMemsetAsync(bad, 0, 2 * sizeof(unsigned long long), s));
+  k_pack_cols<<<grid_for(nzc), 256, 0, s>>>(d, (int)count, nzc, nnz, out->cp, out->jc, bad);
   CBX_HIP(hipGetLastError());
   switch (dtype_size(X[0].dtype)) {
     case 8: CBX_TRY(launch_pack_entries<uint64_t>(d, (int)count, nnz, out->ir, out->val, s)); break;
     case 4: CBX_TRY(launch_pack_entries<uint32_t>(d, (int)count, nnz, out->ir, out->val, s)); break;
     default: CBX_TRY(launch_pack_entries<uint8_t>(d, (int)count, nnz, out->ir, out->val, s)); break;
   }
-  CBX_HIP(hipStreamSynchronize(s));  // the segment table is released at return
+  unsigned long long hbad = 0;
+  CBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));  // the segment table and counter are released at return
+  if (hbad) {
+    cbx_dcsc_free(out);
+    return set_error(CBX_ERR_INVALID, "cbx_dcsc_check: a batch member is malformed (index outside its own shape)");
+  }
   return CBX_OK;
 }
 

@@ -59,3 +59,47 @@ def test_galerkin_batched_256():
     SATs, st = cb.galerkin_batched(As, Ts)
     for g, S in zip(grids, SATs):
         bit_exact(S.to_host(), oracle.galerkin_direct(g))
+
+
+def test_batched_operands_checked_against_their_own_bounds():
+    """Each batch member is checked against its own shape, not only the packed
+    block-diagonal matrix: a row equal to the member's m (a valid row of the
+    packed matrix, in the next member's block), a column id equal to its n,
+    rows descending inside a column, and a cp that does not close at nnz are
+    rejected, and the valid batch still gives the single products."""
+    rng = np.random.default_rng(9)
+    As = [random_dcsc(rng, 50, 40, 0.1, lo=1, hi=9) for _ in range(3)]
+    Bs = [random_dcsc(rng, 40, 30, 0.1, lo=1, hi=9) for _ in range(3)]
+    Cs, _ = cb.spgemm_batched([A.to_device() for A in As], [B.to_device() for B in Bs])
+    for A, B, Cd in zip(As, Bs, Cs):
+        bit_exact(Cd.to_host(), oracle.spgemm(A, B)[0])
+    X = As[1]
+    c = int(np.argmax(np.diff(X.cp) >= 2))
+
+    def variant(edit):
+        Y = cb.Dcsc(X.m, X.n, X.cp.copy(), X.jc.copy(), X.ir.copy(), X.val.copy())
+        edit(Y)
+        return Y
+
+    def row_at_m(Y):
+        Y.ir[-1] = Y.m
+
+    def column_at_n(Y):
+        Y.jc[-1] = Y.n
+
+    def rows_descend(Y):
+        b = int(Y.cp[c])
+        Y.ir[b], Y.ir[b + 1] = Y.ir[b + 1], Y.ir[b]
+
+    def cp_short(Y):
+        Y.cp[-1] -= 1
+
+    for edit in (row_at_m, column_at_n, rows_descend, cp_short):
+        bad = variant(edit)
+        with pytest.raises(cb.CbxError):
+            cb.spgemm_batched([As[0].to_device(), bad.to_device(), As[2].to_device()],
+                              [B.to_device() for B in Bs])
+        Ts = [random_dcsc(rng, 50, 12, 0.2, lo=1, hi=3) for _ in range(3)]
+        with pytest.raises(cb.CbxError):
+            cb.galerkin_batched([As[0].to_device(), bad.to_device(), As[2].to_device()],
+                                [T.to_device() for T in Ts])
