@@ -129,14 +129,20 @@ cbx_status reduce_assemble(const uint64_t* keys, const T* vals, int N, uint64_t*
 }
 
 // The parts as column blocks of one matrix: cp shifted by the entries before
-// the part, jc by the part's first column (its index times n).
+// the part, jc by the part's first column (its index times n).  Checks what
+// the concatenation's own validation cannot see: column ids in [0, n) (one
+// beyond would fall into the next part's block and merge into the wrong
+// output column) and cp starting at 0 and closing at the part's nnz; *bad
+// counts violations.
 __global__ void k_cat_columns(const int64_t* __restrict__ cp, const int64_t* __restrict__ jc, int64_t nzc,
-                              int64_t entry_base, int64_t col_base, int64_t* __restrict__ cpA,
-                              int64_t* __restrict__ jcA) {
+                              int64_t nnz, int64_t n, int64_t entry_base, int64_t col_base,
+                              int64_t* __restrict__ cpA, int64_t* __restrict__ jcA, unsigned long long* __restrict__ bad) {
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q < nzc) {
-    cpA[q] = cp[q] + entry_base;
-    jcA[q] = jc[q] + col_base;
+    const int64_t b = cp[q], j = jc[q];
+    if (j < 0 || j >= n || (q == 0 && b != 0) || (q + 1 == nzc && cp[q + 1] != nnz)) atomicAdd(bad, 1ull);
+    cpA[q] = b + entry_base;
+    jcA[q] = j + col_base;
   }
 }
 
@@ -210,9 +216,12 @@ cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipS
   ScratchSet ws(s);
   int64_t *cpA, *jcA, *irA, *cpB, *jcB, *irB;
   T *valA, *valB;
+  unsigned long long* bad;
   if (!ws.take(&cpA, nzc + 1) || !ws.take(&jcA, nzc) || !ws.take(&irA, nnz) || !ws.take(&valA, nnz) ||
-      !ws.take(&cpB, n + 1) || !ws.take(&jcB, n) || !ws.take(&irB, B.nnz) || !ws.take(&valB, B.nnz))
+      !ws.take(&cpB, n + 1) || !ws.take(&jcB, n) || !ws.take(&irB, B.nnz) || !ws.take(&valB, B.nnz) ||
+      !ws.take(&bad, 1))
     return set_error(CBX_ERR_NOMEM, "cbx_dcsc_merge: workspace allocation failed");
+  CBX_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
   A.cp = cpA;
   A.jc = jcA;
   A.ir = irA;
@@ -226,7 +235,7 @@ cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipS
     const cbx_dcsc& P = parts[i];
     if (P.nzc) {
       k_cat_columns<<<(unsigned)((P.nzc + kMergeNT - 1) / kMergeNT), kMergeNT, 0, s>>>(
-          P.cp, P.jc, P.nzc, eb, (int64_t)i * n, cpA + qb, jcA + qb);
+          P.cp, P.jc, P.nzc, P.nnz, n, eb, (int64_t)i * n, cpA + qb, jcA + qb, bad);
       CBX_HIP(hipMemcpyAsync(irA + eb, P.ir, sizeof(int64_t) * P.nnz, hipMemcpyDeviceToDevice, s));
       CBX_HIP(hipMemcpyAsync(valA + eb, P.val, sizeof(T) * P.nnz, hipMemcpyDeviceToDevice, s));
     }
@@ -237,8 +246,12 @@ cbx_status merge_typed(int32_t count, const cbx_dcsc* parts, cbx_dcsc* out, hipS
   k_stacked_identity<T><<<(unsigned)((n + 1 + kMergeNT - 1) / kMergeNT), kMergeNT, 0, s>>>(
       n, count, semiring_one<K, T>(), cpB, jcB, irB, valB);
   CBX_HIP(hipGetLastError());
-  // the parts' structure is validated as A's (rows in range and ascending per
-  // column, columns ascending, cp consistent); B is built here
+  unsigned long long hbad = 0;
+  CBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, s));
+  CBX_HIP(hipStreamSynchronize(s));
+  if (hbad) return set_error(CBX_ERR_INVALID, "cbx_dcsc_check: a part is malformed (column outside [0, n) or cp)");
+  // the rest of the parts' structure is validated as A's (rows in range and
+  // ascending per column, columns ascending, cp consistent); B is built here
   SpgemmConfig cfg = current_config();
   cfg.trusted_b = true;
   const cbx_semiring sr = K == kPlusTimes ? CBX_SR_PLUS_TIMES : K == kMinPlus ? CBX_SR_MIN_PLUS : CBX_SR_SELECT_MAX;

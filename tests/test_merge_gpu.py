@@ -53,6 +53,30 @@ def test_merge_edge_cases():
         cb.merge([])
 
 
+def test_merge_rejects_parts_outside_their_shape():
+    """A part's column id equal to n lies inside the concatenation's next
+    block (the merge's own operand check would accept it and merge the
+    column into the wrong output column), and a cp that does not close at
+    the part's nnz: both are rejected; the parts as given still merge."""
+    rng = np.random.default_rng(3)
+    parts = [random_part(rng, 50, 40, 60) for _ in range(3)]
+    want = oracle.merge(parts)
+    same(cb.merge([P.to_device() for P in parts]).to_host(), want)
+
+    def column_at_n(P):
+        P.jc[-1] = P.n
+
+    def cp_short(P):
+        P.cp[-1] -= 1
+
+    for edit in (column_at_n, cp_short):
+        bad = cb.Dcsc(parts[0].m, parts[0].n, parts[0].cp.copy(), parts[0].jc.copy(), parts[0].ir.copy(),
+                      parts[0].val.copy(), parts[0].dtype)
+        edit(bad)
+        with pytest.raises(cb.CbxError):
+            cb.merge([bad.to_device(), parts[1].to_device(), parts[2].to_device()])
+
+
 @pytest.mark.parametrize("semiring,dtype,loops", [("plus_times", "int64", True), ("plus_times", "int64", False),
                                                   ("min_plus", "int64", False), ("select_max", "float64", True),
                                                   ("plus_times", "bool", True), ("plus_times", "float64", False)])
