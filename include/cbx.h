@@ -462,7 +462,8 @@ cbx_status cbx_dcsc_transpose(const cbx_dcsc* X, cbx_dcsc* out, void* stream);
  * dropped.  Integer, min-plus and select-max results are exact; floating-
  * point plus-times sums of duplicates are formed in the product's
  * accumulation order (the reference's heap order may round differently).
- * The parts are validated as cbx_dcsc_check does (CBX_ERR_INVALID). */
+ * The parts are validated as cbx_dcsc_check does, each against its own
+ * m x n (CBX_ERR_INVALID). */
 cbx_status cbx_dcsc_merge(int32_t count, const cbx_dcsc* parts, cbx_semiring semiring, cbx_dcsc* out, void* stream);
 
 /* Edge list to matrix on the device: the Graph500 constructor SpTuples(maxnnz,
@@ -482,14 +483,16 @@ cbx_status cbx_dcsc_from_coo(int64_t m, int64_t n, const int64_t* rows, const in
  * block-diagonal product, so every SpGEMM kernel is launched once for the
  * whole batch, then split into count new device matrices.  Each C[i] equals
  * what cbx_spgemm(&A[i], &B[i], ...) returns.  1 <= count <= 4096; the packed
- * batch must hold fewer than 2^31 rows, columns and nonzeros. */
+ * batch must hold fewer than 2^31 rows, columns and nonzeros.  Every member
+ * is validated against its own shape (CBX_ERR_INVALID for a malformed one). */
 cbx_status cbx_spgemm_batched(int64_t count, const cbx_dcsc* A, const cbx_dcsc* B, cbx_semiring semiring,
                               cbx_spgemm_mode mode, cbx_dcsc* C, cbx_spgemm_stats* stats, void* stream);
 
 /* Batched Galerkin triple products (ReleaseTests/GalerkinNew.cpp:94-112):
  * for each i, S = T[i]^T, AT = A[i]*T[i], SAT[i] = S*AT with PlusTimes, where
  * A[i] is square and T[i] = R^T is the fine x coarse prolongation.  One
- * transpose and two batched products for the whole batch. */
+ * transpose and two batched products for the whole batch; members are
+ * validated as in cbx_spgemm_batched. */
 cbx_status cbx_galerkin_batched(int64_t count, const cbx_dcsc* A, const cbx_dcsc* T, cbx_dcsc* SAT,
                                 cbx_spgemm_stats* stats, void* stream);
 
