@@ -413,7 +413,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's block of the local product; --gpus {args.gpus} is not supported "
-              "(the node-local kernel runs on one GPU; see DESIGN.md for how the product shards).", file=sys.stderr)
+              "(the node-local kernel runs on one GPU; across GPUs each process of SUMMA's grid runs its own "
+              "local products and stage merge).", file=sys.stderr)
         return 2
 
     import combblas_amd as cb
